@@ -1,0 +1,265 @@
+/*
+ * path_amd: C ABI of the MI355X path tracer.
+ *
+ * Plain pointers and sizes only. Every function returns PT_OK (0) or a
+ * non-zero status; pt_last_error() returns the message of the last failure on
+ * the calling thread. The reference aborts on these conditions
+ * (include/lib/assert.h:14-50, include/lib/cuda/utils.h:18-26); returning a
+ * status instead is a deliberate deviation so callers and tests can observe
+ * them.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   scene::Scene + ScenefileLoader   include/scene/scene.h:29-130,
+ *                                    src/scene/scenefile_compat/scenefile_loader.cpp:23-319
+ *   render::Settings + config_io     include/render/settings.h:27-70,
+ *                                    src/render/config_io.cpp:14-34
+ *   render::Renderer::render         include/render/renderer.h:43-70
+ *   RendererFromFiles                include/render/renderer_from_files.h:22-43
+ */
+#ifndef PATH_AMD_H
+#define PATH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pt_status {
+  PT_OK = 0,
+  PT_ERR_INVALID = 1,     /* bad argument / unsupported combination */
+  PT_ERR_IO = 2,          /* file not found or unparsable */
+  PT_ERR_HIP = 3,         /* HIP runtime error */
+  PT_ERR_NO_DEVICE = 4,   /* no GPU visible */
+  PT_ERR_UNSUPPORTED = 5, /* setting accepted by the reference but not built */
+};
+
+const char *pt_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Materials: POD mirror of bsdf::UnionBSDF + emission after the MTL        */
+/* conversion (include/scene/tinyobj_material_conversion.h:13-101).         */
+/* ------------------------------------------------------------------------ */
+enum pt_bsdf_type {
+  PT_BSDF_DIFFUSE = 0,
+  PT_BSDF_GLOSSY = 1,
+  PT_BSDF_DIFFUSE_GLOSSY = 2,
+  PT_BSDF_DIFFUSE_MIRROR = 3,
+  PT_BSDF_MIRROR = 4,
+  PT_BSDF_DIELECTRIC_REFRACTIVE = 5,
+};
+
+typedef struct pt_material {
+  uint32_t type;
+  float diffuse[3];
+  float specular[3];
+  float shininess;
+  float ior;
+  float diffuse_weight; /* Combined lobe weights (DiffuseGlossy/DiffuseMirror) */
+  float glossy_weight;
+  float emission[3];
+} pt_material;
+
+/* ------------------------------------------------------------------------ */
+/* Scene (scene::Scene)                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct pt_scene pt_scene;
+
+/* Load a CS123 XML scene with its OBJ/MTL meshes (ScenefileLoader::load_scene,
+ * src/scene/scenefile_compat/scenefile_loader.cpp:23-62). film_to_world is the
+ * camera transform (row-major 3x4) for the given aspect ratio. */
+int pt_scene_load(const char *path, float width_height_ratio, int quiet,
+                  pt_scene **out, float film_to_world[12]);
+/* Empty scene; meshes are added with pt_scene_add_mesh. */
+int pt_scene_create(pt_scene **out);
+/* Add one mesh + one instance of it. verts: n_tris*9 object-space vertices;
+ * normals: n_tris*9 vertex normals or NULL for face normals; tri_mats: index
+ * into mats; transform: row-major 3x4 object_to_world or NULL for identity.
+ * Emissive clusters are derived as in scenefile_loader.cpp:181-276. */
+int pt_scene_add_mesh(pt_scene *scene, const float *verts, const float *normals,
+                      const uint32_t *tri_mats, uint32_t n_tris,
+                      const pt_material *mats, uint32_t n_mats,
+                      const float *transform);
+/* Add a mesh from an OBJ file (tinyobjloader-compatible triangulation). */
+int pt_scene_add_obj(pt_scene *scene, const char *path, const float *transform);
+void pt_scene_destroy(pt_scene *scene);
+/* Flattened world-space sizes (flat_triangle.h:79-146). */
+int pt_scene_counts(const pt_scene *scene, uint32_t *n_tris,
+                    uint32_t *n_materials, uint32_t *n_light_tris);
+/* Flattened world-space arrays; any pointer may be NULL.
+ * tris: n_tris*9 (v0, e1 = v1 - v0, e2 = v2 - v0); normals: n_tris*9;
+ * light_tris: n_light_tris*9; light_cdf: n_light_tris (inclusive, last = 1). */
+int pt_scene_flat(const pt_scene *scene, float *tris, float *normals,
+                  uint32_t *tri_mat, pt_material *mats, float *light_tris,
+                  float *light_cdf);
+/* get_camera_transform (src/scene/camera.cpp:4-27). */
+int pt_camera_from_look(const float look[3], const float up[3],
+                        const float pos[3], float height_angle_deg,
+                        float width_height_ratio, float film_to_world[12]);
+
+/* ------------------------------------------------------------------------ */
+/* Settings: POD mirror of render::Settings (include/render/settings.h).     */
+/* ------------------------------------------------------------------------ */
+typedef struct pt_work_division_settings {
+  uint32_t block_size;
+  uint32_t target_x_block_size;
+  uint32_t force_target_samples;
+  uint32_t forced_target_samples_per_thread;
+  uint32_t base_num_threads;
+  float samples_per_thread_scaling_power;
+  uint32_t max_samples_per_thread;
+} pt_work_division_settings;
+
+enum pt_kernel_approach { PT_MEGA_KERNEL = 0, PT_STREAMING = 1 };
+enum pt_accel_type {
+  PT_ACCEL_LOOP_ALL = 0,
+  PT_ACCEL_NAIVE_PARTITION_BVH = 1,
+  PT_ACCEL_SBVH = 2,
+  PT_ACCEL_DIRECTION_GRID = 3,
+};
+enum pt_light_sampler { PT_LIGHT_NO_LIGHT_SAMPLING = 0, PT_LIGHT_RANDOM_TRIANGLE = 1 };
+enum pt_dir_sampler { PT_DIR_UNIFORM = 0, PT_DIR_BSDF = 1 };
+enum pt_term_prob {
+  PT_TERM_CONSTANT = 0,
+  PT_TERM_N_ITERS = 1,
+  PT_TERM_MULTIPLIER_FUNC = 2,
+  PT_TERM_NORMALIZE = 3,
+};
+enum pt_rng { PT_RNG_UNIFORM = 0, PT_RNG_SOBEL = 1 };
+
+typedef struct pt_settings {
+  uint32_t kernel_approach; /* pt_kernel_approach */
+  /* MegaKernelSettings::ComputationSettings (mega_kernel_settings.h:9-24) */
+  uint32_t max_blocks_per_launch_gpu;
+  uint32_t max_blocks_per_launch_cpu;
+  pt_work_division_settings render_work_division;
+  pt_work_division_settings reduce_work_division;
+  /* StreamingSettings::ComputationSettings (streaming_settings.h:9-16) */
+  pt_work_division_settings init_samples_division;
+  uint32_t max_num_samples_per_launch;
+  uint32_t min_num_init_blocks;
+  uint32_t to_bulk_max_size; /* ToBulkSettings::max_size (to_bulk.h:10-14) */
+  /* accel (IndividuallyIntersectableSettings, sbvh/settings.h) */
+  uint32_t accel; /* pt_accel_type */
+  float overlap_threshold;
+  uint32_t use_spatial_splits;
+  float traversal_per_intersect_cost;
+  uint32_t target_objects;
+  uint32_t print_stats;
+  /* light sampler */
+  uint32_t light_sampler;
+  uint32_t binary_search_threshold;
+  /* dir sampler */
+  uint32_t dir_sampler;
+  /* term prob */
+  uint32_t term_prob;
+  float constant_prob;
+  uint32_t n_iters;
+  float multiplier_func_exp;
+  float multiplier_func_min_prob;
+  float normalize_min_prob;
+  /* rng (RngFromSequenceGenSettings::max_sample_size) */
+  uint32_t rng;
+  uint32_t max_sample_size;
+  /* RenderingEquationSettings */
+  uint32_t back_cull_emission;
+} pt_settings;
+
+/* Reference defaults (the initializers of the settings structs). */
+void pt_settings_default(pt_settings *s);
+/* Load the reference YAML schema (configs/benchmark.yaml; enum names as in
+ * the reference). Unknown enum names are an error. */
+int pt_settings_load_yaml(const char *path, pt_settings *s);
+/* print_config: write the YAML form into buf (NUL-terminated, truncated to
+ * cap); *needed receives the full length. */
+int pt_settings_to_yaml(const pt_settings *s, char *buf, size_t cap,
+                        size_t *needed);
+
+/* ------------------------------------------------------------------------ */
+/* Renderer (render::Renderer)                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct pt_renderer pt_renderer;
+
+enum pt_sample_spec_type {
+  PT_SAMPLE_SQUARE_IMAGE = 0,       /* SquareImage{x_dim, y_dim, film_to_world} */
+  PT_SAMPLE_INITIAL_RAYS = 1,       /* Span<const Ray> */
+  PT_SAMPLE_INITIAL_IDX_AND_DIR = 2 /* Span<const InitialIdxAndDirSpec> */
+};
+
+typedef struct pt_sample_spec {
+  uint32_t type;
+  uint32_t x_dim, y_dim;
+  float film_to_world[12];
+  const float *rays; /* INITIAL_RAYS / INITIAL_IDX_AND_DIR: n_rays*6 (o, d) */
+  const uint32_t *initial_idx; /* INITIAL_IDX_AND_DIR: flattened triangle idx */
+  uint32_t n_rays;
+} pt_sample_spec;
+
+enum pt_output_type {
+  PT_OUTPUT_BGRA = 0,     /* uint32 per pixel, memory order B,G,R,0 */
+  PT_OUTPUT_FLOAT_RGB = 1, /* float[3] per pixel */
+  PT_OUTPUT_PER_STEP = 2,  /* n_steps float[3] images, one per path vertex */
+};
+
+typedef struct pt_output {
+  uint32_t type;
+  void *data;         /* BGRA / FLOAT_RGB: host buffer of x_dim*y_dim pixels */
+  float **per_step;   /* PER_STEP: n_steps host buffers of float[3] pixels */
+  uint32_t n_steps;
+} pt_output;
+
+int pt_renderer_create(int device, pt_renderer **out);
+void pt_renderer_destroy(pt_renderer *r);
+
+/* Renderer::render (include/render/renderer.h:51-54). *seconds receives the
+ * render time excluding the acceleration structure and sampler build, as in
+ * general_render_impl.h:174. */
+int pt_render(pt_renderer *r, const pt_sample_spec *spec, const pt_output *out,
+              const pt_scene *scene, uint32_t samples_per,
+              const pt_settings *settings, int show_progress, int show_times,
+              double *seconds);
+
+/* ------------------------------------------------------------------------ */
+/* Lower-level entry points of the same path (tests, benchmarks, tools).     */
+/* ------------------------------------------------------------------------ */
+/* Closest hit of n host rays (o, d: 6 floats each) against the scene's GPU
+ * BVH (BVH::intersect_objects, include/intersect/accel/detail/bvh/bvh_impl.h:12-89).
+ * idx = flattened triangle index or 0xffffffff for a miss. */
+int pt_intersect(pt_renderer *r, const pt_scene *scene, const float *rays,
+                 uint32_t n, float *t, uint32_t *idx, uint32_t *back);
+/* Sobol table vals[sample * dim_bound + dim] computed on the GPU
+ * (replaces SobelSequenceGen::gen, src/rng/sobel/sobel.cu:19-93). */
+int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
+                   float *out);
+/* The Joe-Kuo direction numbers used: dim_bound*32 words. */
+int pt_sobol_direction_numbers(uint32_t dim_bound, uint32_t *out);
+/* Work division (src/kernel/work_division.cpp:12-69) as 11 words:
+ * block_size, base_samples_per_thread, n_threads_per_unit_extra,
+ * sample_block_size, x_block_size, y_block_size, num_sample_blocks,
+ * num_x_blocks, num_y_blocks, x_dim, y_dim. */
+int pt_work_division(const pt_work_division_settings *s, uint32_t samples_per,
+                     uint32_t x_dim, uint32_t y_dim, uint32_t out[11]);
+/* Host SAH BVH of the flattened scene in the reference node layout
+ * (node.h:22-126): per node lo[3], hi[3] (float) and is_leaf, a, b (uint32);
+ * perm[n_tris]. Pass NULL arrays to query n_nodes only. */
+int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
+                 uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
+                 uint32_t *max_depth);
+
+/* Timing of the last pt_render: kernel milliseconds measured with HIP events
+ * around the launches (stream-accurate), rays traced, samples. */
+typedef struct pt_render_stats {
+  double kernel_ms;
+  double trace_ms;     /* intersection kernels only (wavefront) */
+  uint64_t rays;       /* camera + extension + shadow rays traced */
+  uint64_t samples;
+  uint64_t trace_launches;
+} pt_render_stats;
+int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATH_AMD_H */

@@ -1,0 +1,489 @@
+// CPU checker for the GPU path tracer. TEST INFRASTRUCTURE ONLY: loaded by the
+// tests, by __graft_entry__.smoke() and by bench.py's CPU-baseline leg; nothing
+// in the product links or calls it.
+//
+// It restates the reference's algorithm in plain sequential C++, templated on
+// the scalar type (float: bit-for-bit twin of the GPU arithmetic; double: the
+// FP64 oracle), reusing the per-vertex math of path_amd/csrc/core (pinned by
+// its own closed-form tests):
+//   - closest hit: brute force over all triangles (LoopAll,
+//     include/intersect/accel/loop_all/loop_all.h:15-42) and the reference BVH
+//     traversal (bvh_impl.h:12-89: DFS, left child first, leaves gathered until
+//     target_objects triangles are pending) over the host SAH tree;
+//   - Sobol table evaluated point by point (direct Gray-code XOR);
+//   - the path loop of rendering_equation.h:27-129 with full closest hits for
+//     light rays, per-thread sequential sums over the work division of
+//     src/kernel/work_division.cpp and the per-pixel reduction tree;
+//   - per-ray traversal counts (nodes visited, triangles tested) for the
+//     algorithmic-bytes roofline.
+//
+// Deviations shared with the GPU path (documented in the product): equal-t
+// ties go to the smaller flattened triangle index; BVH boxes are padded by
+// bvh_padding() and a box is skipped only if its entry distance exceeds the
+// best hit (so results do not depend on traversal order).
+//
+// Parity pinning: see tests/test_oracle_*.py (brute force vs BVH, reference
+// KATs, Sobol closed forms, furnace and direct-lighting closed forms).
+#include "../include/path_amd.h"
+#include "../path_amd/csrc/core/camera.h"
+#include "../path_amd/csrc/core/rng.h"
+#include "../path_amd/csrc/core/shading.h"
+#include "../path_amd/csrc/core/work_division.h"
+#include "../path_amd/csrc/host/bvh.h"
+
+#include <omp.h>
+
+#include <atomic>
+#include <cstring>
+#include <vector>
+
+using namespace pt;
+
+namespace {
+
+template <class T> struct OScene {
+  std::vector<TriGeom<T>> tris;
+  std::vector<V3<T>> normals;
+  std::vector<uint32_t> tri_mat;
+  std::vector<Material<T>> mats;
+  std::vector<TriGeom<T>> light_tris;
+  std::vector<T> light_cdf;
+  host::Bvh bvh;
+  std::vector<V3<T>> lo, hi; // padded node boxes
+  SceneView<T> view() const {
+    return {tris.data(), normals.data(), tri_mat.data(), mats.data(),
+            light_tris.data(), light_cdf.data(), uint32_t(light_tris.size())};
+  }
+};
+
+template <class T> TriGeom<T> load_tri(const float *p) {
+  return {{T(p[0]), T(p[1]), T(p[2])},
+          {T(p[3]), T(p[4]), T(p[5])},
+          {T(p[6]), T(p[7]), T(p[8])}};
+}
+
+std::vector<TriGeom<float>> float_tris(const float *tris, uint32_t n) {
+  std::vector<TriGeom<float>> v(n);
+  for (uint32_t i = 0; i < n; ++i)
+    v[i] = load_tri<float>(tris + 9 * i);
+  return v;
+}
+
+template <class T>
+void build_accel(OScene<T> &s, const std::vector<TriGeom<float>> &ftris) {
+  s.bvh = host::build_sah_bvh(ftris);
+  float pad = host::bvh_padding(ftris);
+  s.lo.resize(s.bvh.nodes.size());
+  s.hi.resize(s.bvh.nodes.size());
+  for (size_t i = 0; i < s.bvh.nodes.size(); ++i) {
+    const auto &n = s.bvh.nodes[i];
+    s.lo[i] = {T(n.lo.x - pad), T(n.lo.y - pad), T(n.lo.z - pad)};
+    s.hi[i] = {T(n.hi.x + pad), T(n.hi.y + pad), T(n.hi.z + pad)};
+  }
+}
+
+struct Counts {
+  uint64_t nodes = 0, tris = 0;
+};
+
+template <class T>
+Hit<T> loop_all(const std::vector<TriGeom<T>> &tris, const Ray<T> &ray) {
+  Hit<T> best{T(0), kNoHit, 0};
+  for (uint32_t i = 0; i < tris.size(); ++i) {
+    T t;
+    uint32_t back;
+    if (intersect_triangle(ray, tris[i], &t, &back) &&
+        (best.idx == kNoHit || hit_before(t, i, best.t, best.idx)))
+      best = {t, i, back};
+  }
+  return best;
+}
+
+// reference: include/intersect/accel/detail/bvh/bvh_impl.h:12-89
+template <class T>
+Hit<T> bvh_closest(const OScene<T> &s, const Ray<T> &ray, Counts *c) {
+  Hit<T> best{T(0), kNoHit, 0};
+  if (s.bvh.nodes.empty())
+    return best;
+  const uint32_t target_objects = 4;
+  V3<T> inv = inv_direction(ray.d);
+  uint32_t stack[128];
+  uint32_t sp = 0;
+  stack[sp++] = 0;
+  uint32_t objects[64];
+  uint32_t n_obj = 0;
+  while (sp != 0) {
+    while (n_obj < target_objects && sp != 0) {
+      uint32_t ni = stack[--sp];
+      const host::BvhNode &node = s.bvh.nodes[ni];
+      if (c)
+        ++c->nodes;
+      T tmin, tmax;
+      bool hit = slab_test(s.lo[ni], s.hi[ni], ray.o, inv, &tmin, &tmax);
+      if (hit && (best.idx == kNoHit || !(best.t < tmin))) {
+        if (!node.is_leaf) {
+          stack[sp++] = node.b;
+          stack[sp++] = node.a;
+        } else {
+          for (uint32_t k = node.a; k < node.b && n_obj < 64; ++k)
+            objects[n_obj++] = k;
+        }
+      }
+    }
+    if (n_obj == 0)
+      break;
+    for (uint32_t k = 0; k < n_obj; ++k) {
+      uint32_t orig = s.bvh.perm[objects[k]];
+      if (c)
+        ++c->tris;
+      T t;
+      uint32_t back;
+      if (intersect_triangle(ray, s.tris[orig], &t, &back) &&
+          (best.idx == kNoHit || hit_before(t, orig, best.t, best.idx)))
+        best = {t, orig, back};
+    }
+    n_obj = 0;
+  }
+  return best;
+}
+
+template <class T> struct RenderArgs {
+  const OScene<T> *scene;
+  IntegratorSettings settings;
+  WorkDivision wd;
+  T film_to_world[12];
+  const float *sobol; // vals[sample * dim_bound + dim]
+  uint32_t dim_bound, initial_dim_bound, samples_per;
+  uint32_t rng_type;
+};
+
+template <class T> struct TableRngT {
+  const float *vals;
+  uint32_t row, dim, dim_bound;
+  T next() {
+    T out = T(vals[row + dim]);
+    dim = dim + 1 == dim_bound ? 0 : dim + 1;
+    return out;
+  }
+};
+
+template <class T> struct HashRngT {
+  HashRng r;
+  T next() { return T(r.next()); }
+};
+
+template <class T, class R>
+V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
+                       uint32_t location, Counts *cnt, uint64_t *rays) {
+  V3<T> total = {T(0), T(0), T(0)};
+  const SceneView<T> view = a.scene->view();
+  for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
+    R rng;
+    if constexpr (std::is_same_v<R, TableRngT<T>>)
+      rng = {a.sobol, sample * a.dim_bound,
+             initial_dimension(location, a.initial_dim_bound), a.dim_bound};
+    else
+      rng = {make_hash_rng(sample + location * a.samples_per)};
+    T xo = rng.next();
+    T yo = rng.next();
+    Ray<T> rays_[2];
+    rays_[0] = camera_ray(a.film_to_world, T(float(info.x)) + xo,
+                          T(float(info.y)) + yo, a.wd.x_dim, a.wd.y_dim);
+    uint32_t n_rays = 1;
+    PathState<T> st = initial_path_state<T>();
+    for (;;) {
+      Hit<T> hits[2];
+      for (uint32_t i = 0; i < n_rays; ++i)
+        hits[i] = bvh_closest(*a.scene, rays_[i], cnt);
+      *rays += n_rays;
+      PathState<T> ns;
+      Ray<T> nr[2];
+      uint32_t n_new;
+      if (rendering_equation_iteration(st, rng, rays_[n_rays - 1], hits,
+                                       a.settings, view, &ns, nr, &n_new)) {
+        total += ns.total;
+        break;
+      }
+      st = ns;
+      rays_[0] = nr[0];
+      rays_[1] = nr[1];
+      n_rays = n_new;
+    }
+  }
+  return total;
+}
+
+// Renders the pixels [pixel_begin, pixel_end) (row-major) or all when
+// pixel_end == 0. Output: per pixel RGB (already divided by spp when there is a
+// single sample block, else the sum over sample blocks divided by spp).
+template <class T>
+int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end,
+                int n_threads, T *out, uint64_t *n_rays, uint64_t *n_nodes,
+                uint64_t *n_tris) {
+  const WorkDivision &wd = a.wd;
+  const uint32_t n_pixels = wd.x_dim * wd.y_dim;
+  if (pixel_end == 0 || pixel_end > n_pixels)
+    pixel_end = n_pixels;
+  std::vector<T> partial(size_t(n_pixels) * wd.num_sample_blocks * 3, T(0));
+  std::atomic<uint64_t> rays{0}, nodes{0}, tris{0};
+  const int64_t total_blocks = wd.total_num_blocks();
+  if (n_threads > 0)
+    omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t block = 0; block < total_blocks; ++block) {
+    std::vector<V3<T>> thread_tot(wd.block_size);
+    bool any = false;
+    Counts c;
+    uint64_t r = 0;
+    for (uint32_t th = 0; th < wd.block_size; ++th) {
+      ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
+      thread_tot[th] = {T(0), T(0), T(0)};
+      if (info.exit)
+        continue;
+      uint32_t loc = info.x + info.y * wd.x_dim;
+      if (loc < pixel_begin || loc >= pixel_end)
+        continue;
+      any = true;
+      if (a.rng_type == kRngSobel)
+        thread_tot[th] = integrate_thread<T, TableRngT<T>>(a, info, loc, &c, &r);
+      else
+        thread_tot[th] = integrate_thread<T, HashRngT<T>>(a, info, loc, &c, &r);
+    }
+    if (!any)
+      continue;
+    uint32_t sbs = wd.sample_block_size;
+    for (uint32_t th = 0; th < wd.block_size; th += sbs) {
+      ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
+      if (info.exit)
+        continue;
+      V3<T> sum = tree_reduce(thread_tot.data() + th, sbs);
+      size_t idx = size_t(wd.sample_block_idx(uint32_t(block))) +
+                   size_t(wd.num_sample_blocks) * (info.x + size_t(info.y) * wd.x_dim);
+      partial[3 * idx] = sum.x;
+      partial[3 * idx + 1] = sum.y;
+      partial[3 * idx + 2] = sum.z;
+    }
+    rays += r;
+    nodes += c.nodes;
+    tris += c.tris;
+  }
+  T spp = T(float(a.samples_per));
+  for (uint32_t p = pixel_begin; p < pixel_end; ++p) {
+    T s[3] = {T(0), T(0), T(0)};
+    for (uint32_t k = 0; k < wd.num_sample_blocks; ++k)
+      for (int ch = 0; ch < 3; ++ch)
+        s[ch] = wd.num_sample_blocks == 1
+                    ? partial[3 * (size_t(p) * wd.num_sample_blocks + k) + ch]
+                    : s[ch] + partial[3 * (size_t(p) * wd.num_sample_blocks + k) + ch];
+    for (int ch = 0; ch < 3; ++ch)
+      out[3 * size_t(p) + ch] = s[ch] / spp;
+  }
+  if (n_rays)
+    *n_rays = rays;
+  if (n_nodes)
+    *n_nodes = nodes;
+  if (n_tris)
+    *n_tris = tris;
+  return 0;
+}
+
+template <class T>
+void fill_scene(OScene<T> &s, const float *tris, const float *normals,
+                const uint32_t *tri_mat, uint32_t n_tris, const pt_material *mats,
+                uint32_t n_mats, const float *light_tris, const float *light_cdf,
+                uint32_t n_lights) {
+  s.tris.resize(n_tris);
+  s.normals.resize(size_t(n_tris) * 3);
+  s.tri_mat.assign(tri_mat, tri_mat + n_tris);
+  for (uint32_t i = 0; i < n_tris; ++i) {
+    s.tris[i] = load_tri<T>(tris + 9 * i);
+    for (int k = 0; k < 3; ++k)
+      s.normals[3 * i + k] = {T(normals[9 * i + 3 * k]), T(normals[9 * i + 3 * k + 1]),
+                              T(normals[9 * i + 3 * k + 2])};
+  }
+  for (uint32_t i = 0; i < n_mats; ++i) {
+    const pt_material &m = mats[i];
+    s.mats.push_back(make_material<T>(
+        m.type, V3<T>{T(m.diffuse[0]), T(m.diffuse[1]), T(m.diffuse[2])},
+        V3<T>{T(m.specular[0]), T(m.specular[1]), T(m.specular[2])},
+        T(m.shininess), T(m.ior), T(m.diffuse_weight), T(m.glossy_weight),
+        V3<T>{T(m.emission[0]), T(m.emission[1]), T(m.emission[2])}));
+  }
+  for (uint32_t i = 0; i < n_lights; ++i) {
+    s.light_tris.push_back(load_tri<T>(light_tris + 9 * i));
+    s.light_cdf.push_back(T(light_cdf[i]));
+  }
+  build_accel(s, float_tris(tris, n_tris));
+}
+
+} // namespace
+
+extern "C" {
+
+// Brute-force closest hit (float). rays: n*6.
+int oracle_loopall(const float *tris, uint32_t n_tris, const float *rays,
+                   uint32_t n, float *t, uint32_t *idx, uint32_t *back) {
+  std::vector<TriGeom<float>> tv = float_tris(tris, n_tris);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const float *r = rays + 6 * i;
+    Ray<float> ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+    Hit<float> h = loop_all(tv, ray);
+    t[i] = h.t;
+    idx[i] = h.idx;
+    back[i] = h.back;
+  }
+  return 0;
+}
+
+// Reference BVH traversal over the host SAH tree (float), with per-ray counts.
+int oracle_bvh_intersect(const float *tris, uint32_t n_tris, const float *rays,
+                         uint32_t n, float *t, uint32_t *idx, uint32_t *back,
+                         uint32_t *nodes_visited, uint32_t *tris_tested) {
+  OScene<float> s;
+  s.tris = float_tris(tris, n_tris);
+  build_accel(s, s.tris);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const float *r = rays + 6 * i;
+    Ray<float> ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+    Counts c;
+    Hit<float> h = bvh_closest(s, ray, &c);
+    t[i] = h.t;
+    idx[i] = h.idx;
+    back[i] = h.back;
+    if (nodes_visited)
+      nodes_visited[i] = uint32_t(c.nodes);
+    if (tris_tested)
+      tris_tested[i] = uint32_t(c.tris);
+  }
+  return 0;
+}
+
+// Sobol table, direct evaluation of each point. dirs: dim_bound*32.
+int oracle_sobol_table(const uint32_t *dirs, uint32_t dim_bound, uint32_t count,
+                       float *out) {
+  for (uint32_t s = 0; s < count; ++s)
+    for (uint32_t d = 0; d < dim_bound; ++d)
+      out[size_t(s) * dim_bound + d] = sobol_table_value(dirs, d, s);
+  return 0;
+}
+
+uint32_t oracle_fnv_hash(uint32_t v) { return fnv_hash(v); }
+
+int oracle_work_division(const pt_work_division_settings *s, uint32_t spp,
+                         uint32_t x, uint32_t y, uint32_t *thread_info_out,
+                         uint32_t max_threads) {
+  // Writes (x, y, start, end, exit) per (block, thread) in launch order.
+  WorkDivisionSettings ws;
+  ws.block_size = s->block_size;
+  ws.target_x_block_size = s->target_x_block_size;
+  ws.force_target_samples = s->force_target_samples;
+  ws.forced_target_samples_per_thread = s->forced_target_samples_per_thread;
+  ws.base_num_threads = s->base_num_threads;
+  ws.samples_per_thread_scaling_power = s->samples_per_thread_scaling_power;
+  ws.max_samples_per_thread = s->max_samples_per_thread;
+  WorkDivision wd;
+  if (!WorkDivision::make(ws, spp, x, y, &wd))
+    return -1;
+  uint64_t total = uint64_t(wd.total_num_blocks()) * wd.block_size;
+  if (total > max_threads)
+    return int(total > 0x7fffffff ? 0x7fffffff : total);
+  for (uint32_t b = 0; b < wd.total_num_blocks(); ++b)
+    for (uint32_t t = 0; t < wd.block_size; ++t) {
+      ThreadInfo i = wd.get_thread_info(b, t);
+      uint32_t *o = thread_info_out + 5 * (size_t(b) * wd.block_size + t);
+      o[0] = i.x, o[1] = i.y, o[2] = i.start_sample, o[3] = i.end_sample,
+      o[4] = i.exit;
+    }
+  return int(total);
+}
+
+// Mega-kernel-equivalent render. precision: 32 or 64. Output out32 (float) or
+// out64 (double), x_dim*y_dim*3. Counts per render: rays traced, BVH nodes
+// visited and triangles tested by the reference traversal.
+int oracle_render(const float *tris, const float *normals, const uint32_t *tri_mat,
+                  uint32_t n_tris, const pt_material *mats, uint32_t n_mats,
+                  const float *light_tris, const float *light_cdf,
+                  uint32_t n_lights, const float *film_to_world,
+                  const pt_settings *settings, uint32_t samples_per,
+                  uint32_t x_dim, uint32_t y_dim, const float *sobol,
+                  int precision, int n_threads, uint32_t pixel_begin,
+                  uint32_t pixel_end, float *out32, double *out64,
+                  uint64_t *counts /* rays, nodes, tris */) {
+  WorkDivisionSettings ws;
+  const pt_work_division_settings &w = settings->render_work_division;
+  ws.block_size = w.block_size;
+  ws.target_x_block_size = w.target_x_block_size;
+  ws.force_target_samples = w.force_target_samples;
+  ws.forced_target_samples_per_thread = w.forced_target_samples_per_thread;
+  ws.base_num_threads = w.base_num_threads;
+  ws.samples_per_thread_scaling_power = w.samples_per_thread_scaling_power;
+  ws.max_samples_per_thread = w.max_samples_per_thread;
+  WorkDivision wd;
+  if (!WorkDivision::make(ws, samples_per, x_dim, y_dim, &wd))
+    return -1;
+  IntegratorSettings is{};
+  is.light_sampler = settings->light_sampler;
+  is.dir_sampler = settings->dir_sampler;
+  is.term_prob = settings->term_prob;
+  is.rng = settings->rng;
+  is.back_cull_emission = settings->back_cull_emission;
+  is.n_iters = settings->n_iters;
+  is.constant_prob = settings->constant_prob;
+  is.mf_exp = settings->multiplier_func_exp;
+  is.mf_min_prob = settings->multiplier_func_min_prob;
+  is.normalize_min_prob = settings->normalize_min_prob;
+  uint32_t dim_bound = settings->max_sample_size;
+  uint32_t init_bound = dim_bound < 64 ? dim_bound : 64;
+  uint64_t c[3] = {0, 0, 0};
+  int rc;
+  if (precision == 64) {
+    OScene<double> s;
+    fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
+               light_cdf, n_lights);
+    RenderArgs<double> a{&s, is, wd, {}, sobol, dim_bound, init_bound,
+                         samples_per, settings->rng};
+    for (int i = 0; i < 12; ++i)
+      a.film_to_world[i] = film_to_world[i];
+    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out64, &c[0], &c[1], &c[2]);
+  } else {
+    OScene<float> s;
+    fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
+               light_cdf, n_lights);
+    RenderArgs<float> a{&s, is, wd, {}, sobol, dim_bound, init_bound,
+                        samples_per, settings->rng};
+    for (int i = 0; i < 12; ++i)
+      a.film_to_world[i] = film_to_world[i];
+    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out32, &c[0], &c[1], &c[2]);
+  }
+  if (counts)
+    std::memcpy(counts, c, sizeof(c));
+  return rc;
+}
+
+// Deterministic math exposed for accuracy tests against libm.
+void oracle_detmath(int fn, const float *x, const float *y, float *out,
+                    uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) {
+    switch (fn) {
+    case 0: {
+      float s, c;
+      detmath::sincos_2pi(x[i], &s, &c);
+      out[2 * i] = s;
+      out[2 * i + 1] = c;
+      break;
+    }
+    case 1:
+      out[i] = detmath::log2(x[i]);
+      break;
+    case 2:
+      out[i] = detmath::exp2(x[i]);
+      break;
+    default:
+      out[i] = detmath::pow(x[i], y[i]);
+    }
+  }
+}
+
+} // extern "C"

@@ -1,0 +1,132 @@
+"""ctypes binding of the CPU checker (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py. The product (path_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from path_amd import _lib as L
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_U = C.POINTER(C.c_uint32)
+_U64 = C.POINTER(C.c_uint64)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing; build with `python -m path_amd.build --oracle`")
+        l = C.CDLL(str(LIB_PATH))
+        l.oracle_loopall.argtypes = [_F, C.c_uint32, _F, C.c_uint32, _F, _U, _U]
+        l.oracle_bvh_intersect.argtypes = [_F, C.c_uint32, _F, C.c_uint32, _F, _U, _U, _U, _U]
+        l.oracle_sobol_table.argtypes = [_U, C.c_uint32, C.c_uint32, _F]
+        l.oracle_fnv_hash.argtypes = [C.c_uint32]
+        l.oracle_fnv_hash.restype = C.c_uint32
+        l.oracle_work_division.argtypes = [C.POINTER(L.PtWorkDivisionSettings), C.c_uint32,
+                                           C.c_uint32, C.c_uint32, _U, C.c_uint32]
+        l.oracle_render.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
+                                    _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
+                                    C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
+                                    C.c_uint32, _F, _D, _U64]
+        l.oracle_detmath.argtypes = [C.c_int, _F, _F, _F, C.c_uint32]
+        l.oracle_detmath.restype = None
+        _lib = l
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(_F)
+
+
+def _u(a):
+    return a.ctypes.data_as(_U)
+
+
+def loopall(tris: np.ndarray, rays: np.ndarray):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    t, idx, back = np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    lib().oracle_loopall(_f(tris), tris.shape[0], _f(rays), n, _f(t), _u(idx), _u(back))
+    return t, idx, back
+
+
+def bvh_intersect(tris: np.ndarray, rays: np.ndarray):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    t, idx, back = np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    nodes, ntri = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    lib().oracle_bvh_intersect(_f(tris), tris.shape[0], _f(rays), n, _f(t), _u(idx), _u(back),
+                               _u(nodes), _u(ntri))
+    return t, idx, back, nodes, ntri
+
+
+def sobol_table(dirs: np.ndarray, dim_bound: int, count: int) -> np.ndarray:
+    dirs = np.ascontiguousarray(dirs, np.uint32)
+    out = np.zeros((count, dim_bound), np.float32)
+    lib().oracle_sobol_table(_u(dirs), dim_bound, count, _f(out))
+    return out
+
+
+def fnv_hash(v: int) -> int:
+    return int(lib().oracle_fnv_hash(v))
+
+
+def thread_infos(wds: L.PtWorkDivisionSettings, spp: int, x: int, y: int, max_threads=1 << 22):
+    out = np.zeros((max_threads, 5), np.uint32)
+    n = lib().oracle_work_division(C.byref(wds), spp, x, y, _u(out), max_threads)
+    if n < 0:
+        raise ValueError("invalid work division")
+    if n > max_threads:
+        raise ValueError("too many threads")
+    return out[:n]
+
+
+def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
+           sobol: np.ndarray | None, precision: int = 32, threads: int = 0,
+           pixel_begin: int = 0, pixel_end: int = 0):
+    """Returns (image (y, x, 3), counts[rays, nodes, tris])."""
+    raw = settings.raw if hasattr(settings, "raw") else settings
+    ftw = np.ascontiguousarray(film_to_world, np.float32)
+    sob = np.zeros(1, np.float32) if sobol is None else np.ascontiguousarray(sobol, np.float32)
+    counts = np.zeros(3, np.uint64)
+    out32 = np.zeros((y_dim, x_dim, 3), np.float32)
+    out64 = np.zeros((y_dim, x_dim, 3), np.float64)
+    rc = lib().oracle_render(
+        _f(flat["tris"]), _f(flat["normals"]), _u(flat["tri_mat"]), flat["tris"].shape[0],
+        flat["materials"], flat["n_materials"], _f(flat["light_tris"]), _f(flat["light_cdf"]),
+        flat["light_tris"].shape[0], _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
+        precision, threads, pixel_begin, pixel_end, _f(out32), out64.ctypes.data_as(_D),
+        counts.ctypes.data_as(_U64))
+    if rc != 0:
+        raise ValueError("oracle render failed")
+    return (out64 if precision == 64 else out32), counts
+
+
+def detmath(fn: str, x, y=None):
+    code = {"sincos_2pi": 0, "log2": 1, "exp2": 2, "pow": 3}[fn]
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, np.float32)
+    out = np.zeros(x.size * (2 if code == 0 else 1), np.float32)
+    lib().oracle_detmath(code, _f(x), _f(y), _f(out), x.size)
+    return out.reshape(-1, 2) if code == 0 else out
+
+
+def joe_kuo_directions(dim_bound: int) -> np.ndarray:
+    """Joe-Kuo direction numbers as published through rocRAND's host API
+    (rocrand_get_direction_vectors32, set ROCRAND_DIRECTION_VECTORS_32_JOEKUO6)."""
+    rr = C.CDLL("/opt/rocm/lib/librocrand.so")
+    ptr = C.POINTER(C.c_uint32)()
+    if rr.rocrand_get_direction_vectors32(C.byref(ptr), 101) != 0:
+        raise RuntimeError("rocrand direction numbers unavailable")
+    return np.ctypeslib.as_array(ptr, shape=(dim_bound * 32,)).reshape(dim_bound, 32).copy()

@@ -1,0 +1,364 @@
+// C ABI: scene, settings and the GPU renderer.
+#include "../../include/path_amd.h"
+
+#include "core/camera.h"
+#include "core/rng.h"
+#include "gpu/kernels.h"
+#include "host/bvh.h"
+#include "host/renderer.h"
+#include "host/scene.h"
+#include "host/settings.h"
+
+#include <rocrand/rocrand.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+
+using namespace pt;
+using namespace pt::host;
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+} // namespace
+
+namespace pt {
+namespace host {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+} // namespace host
+} // namespace pt
+
+#define PT_GUARD_BEGIN try {
+#define PT_GUARD_END                                                           \
+  }                                                                            \
+  catch (const std::exception &e) {                                            \
+    return fail(PT_ERR_INVALID, std::string("internal error: ") + e.what());   \
+  }
+
+extern "C" {
+
+const char *pt_last_error(void) { return g_error.c_str(); }
+
+// ------------------------------- scene ----------------------------------
+
+int pt_scene_create(pt_scene **out) {
+  if (!out)
+    return fail(PT_ERR_INVALID, "null output pointer");
+  *out = new pt_scene();
+  return PT_OK;
+}
+
+void pt_scene_destroy(pt_scene *scene) { delete scene; }
+
+int pt_scene_load(const char *path, float width_height_ratio, int quiet,
+                  pt_scene **out, float film_to_world[12]) {
+  PT_GUARD_BEGIN
+  if (!path || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  auto sc = std::make_unique<pt_scene>();
+  Camera cam;
+  std::string err;
+  if (!load_scene_file(path, width_height_ratio, &sc->scene, &cam, &err))
+    return fail(PT_ERR_IO, err);
+  (void)quiet;
+  if (film_to_world)
+    std::memcpy(film_to_world, cam.film_to_world.m, 12 * sizeof(float));
+  sc->generation++;
+  *out = sc.release();
+  return PT_OK;
+  PT_GUARD_END
+}
+
+static MaterialParams from_pt(const pt_material &m) {
+  MaterialParams p;
+  p.type = m.type;
+  p.diffuse = {m.diffuse[0], m.diffuse[1], m.diffuse[2]};
+  p.specular = {m.specular[0], m.specular[1], m.specular[2]};
+  p.shininess = m.shininess;
+  p.ior = m.ior;
+  p.diffuse_weight = m.diffuse_weight;
+  p.glossy_weight = m.glossy_weight;
+  p.emission = {m.emission[0], m.emission[1], m.emission[2]};
+  return p;
+}
+
+static pt_material to_pt(const MaterialParams &p) {
+  pt_material m;
+  m.type = p.type;
+  m.diffuse[0] = p.diffuse.x, m.diffuse[1] = p.diffuse.y, m.diffuse[2] = p.diffuse.z;
+  m.specular[0] = p.specular.x, m.specular[1] = p.specular.y, m.specular[2] = p.specular.z;
+  m.shininess = p.shininess;
+  m.ior = p.ior;
+  m.diffuse_weight = p.diffuse_weight;
+  m.glossy_weight = p.glossy_weight;
+  m.emission[0] = p.emission.x, m.emission[1] = p.emission.y, m.emission[2] = p.emission.z;
+  return m;
+}
+
+static Affine affine_from(const float *t) {
+  Affine a;
+  if (t)
+    std::memcpy(a.m, t, 12 * sizeof(float));
+  return a;
+}
+
+int pt_scene_add_mesh(pt_scene *scene, const float *verts, const float *normals,
+                      const uint32_t *tri_mats, uint32_t n_tris,
+                      const pt_material *mats, uint32_t n_mats,
+                      const float *transform) {
+  PT_GUARD_BEGIN
+  if (!scene || (n_tris && (!verts || !tri_mats)) || (n_mats && !mats))
+    return fail(PT_ERR_INVALID, "null argument");
+  for (uint32_t i = 0; i < n_tris; ++i)
+    if (tri_mats[i] >= n_mats)
+      return fail(PT_ERR_INVALID, "triangle material index out of range");
+  for (uint32_t i = 0; i < n_mats; ++i)
+    if (mats[i].type > PT_BSDF_DIELECTRIC_REFRACTIVE)
+      return fail(PT_ERR_INVALID, "unknown material type");
+  std::vector<std::array<V3f, 3>> tris(n_tris);
+  for (uint32_t i = 0; i < n_tris; ++i)
+    for (int k = 0; k < 3; ++k)
+      tris[i][k] = {verts[9 * i + 3 * k], verts[9 * i + 3 * k + 1],
+                    verts[9 * i + 3 * k + 2]};
+  std::vector<uint32_t> tm(tri_mats, tri_mats + n_tris);
+  std::vector<MaterialParams> mp;
+  for (uint32_t i = 0; i < n_mats; ++i)
+    mp.push_back(from_pt(mats[i]));
+  add_mesh(scene->scene, tris, tm, mp, affine_from(transform), "memory");
+  if (normals) {
+    uint32_t base = uint32_t(scene->scene.triangles.size()) - n_tris;
+    for (uint32_t i = 0; i < n_tris; ++i)
+      for (int k = 0; k < 3; ++k)
+        scene->scene.triangle_data[base + i].normals[k] =
+            normalize(V3f{normals[9 * i + 3 * k], normals[9 * i + 3 * k + 1],
+                          normals[9 * i + 3 * k + 2]});
+  }
+  scene->generation++;
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_scene_add_obj(pt_scene *scene, const char *path, const float *transform) {
+  PT_GUARD_BEGIN
+  if (!scene || !path)
+    return fail(PT_ERR_INVALID, "null argument");
+  std::string err;
+  if (!load_obj_mesh(scene->scene, path, affine_from(transform), &err))
+    return fail(PT_ERR_IO, err);
+  scene->generation++;
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_scene_counts(const pt_scene *scene, uint32_t *n_tris,
+                    uint32_t *n_materials, uint32_t *n_light_tris) {
+  PT_GUARD_BEGIN
+  if (!scene)
+    return fail(PT_ERR_INVALID, "null scene");
+  const FlatScene &f = scene->flat();
+  if (n_tris)
+    *n_tris = uint32_t(f.tris.size());
+  if (n_materials)
+    *n_materials = uint32_t(f.materials.size());
+  if (n_light_tris)
+    *n_light_tris = uint32_t(f.light_tris.size());
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_scene_flat(const pt_scene *scene, float *tris, float *normals,
+                  uint32_t *tri_mat, pt_material *mats, float *light_tris,
+                  float *light_cdf) {
+  PT_GUARD_BEGIN
+  if (!scene)
+    return fail(PT_ERR_INVALID, "null scene");
+  const FlatScene &f = scene->flat();
+  if (tris)
+    std::memcpy(tris, f.tris.data(), f.tris.size() * sizeof(TriGeom<float>));
+  if (normals)
+    std::memcpy(normals, f.normals.data(), f.normals.size() * sizeof(V3f));
+  if (tri_mat)
+    std::memcpy(tri_mat, f.tri_mat.data(), f.tri_mat.size() * sizeof(uint32_t));
+  if (mats)
+    for (size_t i = 0; i < f.materials.size(); ++i)
+      mats[i] = to_pt(f.materials[i]);
+  if (light_tris)
+    std::memcpy(light_tris, f.light_tris.data(),
+                f.light_tris.size() * sizeof(TriGeom<float>));
+  if (light_cdf)
+    std::memcpy(light_cdf, f.light_cdf.data(), f.light_cdf.size() * sizeof(float));
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_camera_from_look(const float look[3], const float up[3],
+                        const float pos[3], float height_angle_deg,
+                        float width_height_ratio, float film_to_world[12]) {
+  if (!look || !up || !pos || !film_to_world)
+    return fail(PT_ERR_INVALID, "null argument");
+  Camera c = camera_from_look({look[0], look[1], look[2]}, {up[0], up[1], up[2]},
+                              {pos[0], pos[1], pos[2]}, height_angle_deg,
+                              width_height_ratio);
+  std::memcpy(film_to_world, c.film_to_world.m, 12 * sizeof(float));
+  return PT_OK;
+}
+
+int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
+                 uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
+                 uint32_t *max_depth) {
+  PT_GUARD_BEGIN
+  if (!scene || !n_nodes)
+    return fail(PT_ERR_INVALID, "null argument");
+  const Bvh &bvh = scene->bvh();
+  *n_nodes = uint32_t(bvh.nodes.size());
+  if (max_depth)
+    *max_depth = bvh.max_depth;
+  for (size_t i = 0; i < bvh.nodes.size(); ++i) {
+    const BvhNode &n = bvh.nodes[i];
+    if (lo)
+      lo[3 * i] = n.lo.x, lo[3 * i + 1] = n.lo.y, lo[3 * i + 2] = n.lo.z;
+    if (hi)
+      hi[3 * i] = n.hi.x, hi[3 * i + 1] = n.hi.y, hi[3 * i + 2] = n.hi.z;
+    if (is_leaf)
+      is_leaf[i] = n.is_leaf;
+    if (a)
+      a[i] = n.a;
+    if (b)
+      b[i] = n.b;
+  }
+  if (perm)
+    std::memcpy(perm, bvh.perm.data(), bvh.perm.size() * sizeof(uint32_t));
+  return PT_OK;
+  PT_GUARD_END
+}
+
+// ------------------------------ settings --------------------------------
+
+void pt_settings_default(pt_settings *s) {
+  if (s)
+    default_settings(s);
+}
+
+int pt_settings_load_yaml(const char *path, pt_settings *s) {
+  PT_GUARD_BEGIN
+  if (!path || !s)
+    return fail(PT_ERR_INVALID, "null argument");
+  std::string err;
+  if (!load_settings_yaml(path, s, &err))
+    return fail(PT_ERR_IO, err);
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_settings_to_yaml(const pt_settings *s, char *buf, size_t cap,
+                        size_t *needed) {
+  PT_GUARD_BEGIN
+  if (!s)
+    return fail(PT_ERR_INVALID, "null settings");
+  std::string y = settings_to_yaml(*s);
+  if (needed)
+    *needed = y.size();
+  if (buf && cap) {
+    size_t n = y.size() < cap - 1 ? y.size() : cap - 1;
+    std::memcpy(buf, y.data(), n);
+    buf[n] = 0;
+  }
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_work_division(const pt_work_division_settings *s, uint32_t samples_per,
+                     uint32_t x_dim, uint32_t y_dim, uint32_t out[11]) {
+  if (!s || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  WorkDivision d;
+  if (!WorkDivision::make(work_division_settings(*s), samples_per, x_dim, y_dim, &d))
+    return fail(PT_ERR_INVALID, "invalid work division settings");
+  uint32_t v[11] = {d.block_size,     d.base_samples_per_thread,
+                    d.n_threads_per_unit_extra, d.sample_block_size,
+                    d.x_block_size,   d.y_block_size,
+                    d.num_sample_blocks, d.num_x_blocks,
+                    d.num_y_blocks,   d.x_dim,
+                    d.y_dim};
+  std::memcpy(out, v, sizeof(v));
+  return PT_OK;
+}
+
+int pt_sobol_direction_numbers(uint32_t dim_bound, uint32_t *out) {
+  if (!out || dim_bound == 0 || dim_bound > 20000)
+    return fail(PT_ERR_INVALID, "dim_bound must be in [1, 20000]");
+  const unsigned int *v = nullptr;
+  if (rocrand_get_direction_vectors32(&v, ROCRAND_DIRECTION_VECTORS_32_JOEKUO6) !=
+          ROCRAND_STATUS_SUCCESS ||
+      !v)
+    return fail(PT_ERR_UNSUPPORTED, "Joe-Kuo direction numbers unavailable");
+  std::memcpy(out, v, size_t(dim_bound) * 32 * sizeof(uint32_t));
+  return PT_OK;
+}
+
+// ------------------------------ renderer --------------------------------
+
+int pt_renderer_create(int device, pt_renderer **out) {
+  PT_GUARD_BEGIN
+  if (!out)
+    return fail(PT_ERR_INVALID, "null output pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(PT_ERR_NO_DEVICE, "no GPU device visible");
+  if (device < 0 || device >= n)
+    return fail(PT_ERR_INVALID, "device index out of range");
+  auto r = std::make_unique<pt_renderer>();
+  std::string err;
+  if (!r->init(device, &err))
+    return fail(PT_ERR_HIP, err);
+  *out = r.release();
+  return PT_OK;
+  PT_GUARD_END
+}
+
+void pt_renderer_destroy(pt_renderer *r) { delete r; }
+
+int pt_render(pt_renderer *r, const pt_sample_spec *spec, const pt_output *out,
+              const pt_scene *scene, uint32_t samples_per,
+              const pt_settings *settings, int show_progress, int show_times,
+              double *seconds) {
+  PT_GUARD_BEGIN
+  if (!r || !spec || !out || !scene || !settings)
+    return fail(PT_ERR_INVALID, "null argument");
+  return r->render(*spec, *out, *scene, samples_per, *settings, show_progress != 0,
+                   show_times != 0, seconds);
+  PT_GUARD_END
+}
+
+int pt_intersect(pt_renderer *r, const pt_scene *scene, const float *rays,
+                 uint32_t n, float *t, uint32_t *idx, uint32_t *back) {
+  PT_GUARD_BEGIN
+  if (!r || !scene || (n && (!rays || !t || !idx || !back)))
+    return fail(PT_ERR_INVALID, "null argument");
+  return r->intersect(*scene, rays, n, t, idx, back);
+  PT_GUARD_END
+}
+
+int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
+                   float *out) {
+  PT_GUARD_BEGIN
+  if (!r || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  return r->sobol_table_to_host(dim_bound, count, out);
+  PT_GUARD_END
+}
+
+int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out) {
+  if (!r || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  *out = r->stats;
+  return PT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,61 @@
+// Kernel parameter blocks and host-side launchers of the GPU path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/shading.h"
+#include "../core/work_division.h"
+
+namespace pt {
+namespace gpu {
+
+// Entries of the per-lane traversal stack (bounds the BVH depth accepted).
+constexpr int kTraceStackSize = 64;
+
+// Device copy of one flattened scene.
+struct DeviceScene {
+  const float4 *nodes = nullptr; // GpuNode array (4 float4 each)
+  const float4 *tris = nullptr;  // GpuTri array in leaf order (3 float4 each)
+  SceneView<float> view{};       // shading data in flattened order
+  uint32_t n_tris = 0;
+  uint32_t n_nodes = 0;
+};
+
+struct RngParams {
+  uint32_t type; // kRngSobel / kRngUniform
+  const float *vals;
+  uint32_t dim_bound;
+  uint32_t initial_dim_bound;
+  uint32_t samples_per;
+};
+
+struct MegaParams {
+  DeviceScene scene;
+  IntegratorSettings settings;
+  WorkDivision wd;
+  RngParams rng;
+  float film_to_world[12];
+  const float *initial_rays; // InitialRays sample spec (6 floats per ray) or null
+  float *out_rgb;            // num_sample_blocks * x_dim * y_dim * 3
+  uint32_t divide_by_spp;    // 1 when num_sample_blocks == 1
+  unsigned long long *ray_counter;
+};
+
+// Launchers (return hipError_t of the launch).
+hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
+                              uint32_t count, float *out, hipStream_t s);
+hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
+                            uint32_t n, float *t, uint32_t *idx, uint32_t *back,
+                            hipStream_t s);
+hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
+                             uint32_t end_block, hipStream_t s);
+// Sum of the num_sample_blocks partials per pixel (in sample-block order),
+// divided by samples_per.
+hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
+                                       uint32_t n_pixels, uint32_t samples_per,
+                                       float *out, hipStream_t s);
+hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
+                          hipStream_t s);
+
+} // namespace gpu
+} // namespace pt

@@ -1,0 +1,263 @@
+// Sobol table, bulk closest-hit and the mega-kernel integrator.
+#include "kernels.h"
+
+#include "../core/camera.h"
+#include "../core/rng.h"
+#include "trace.h"
+#include "wave.h"
+
+namespace pt {
+namespace gpu {
+
+// ---------------------------------------------------------------------------
+// Sobol table: one wave per (dimension, 64 consecutive samples). Lane 0 of
+// the wave evaluates its point directly; every other lane contributes the
+// Gray-code step v[ctz(s)], and an inclusive XOR scan across the wave (DPP)
+// turns the steps into the 64 points.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sobol_table(const uint32_t *__restrict__ dirs,
+                                                     uint32_t dim_bound,
+                                                     uint32_t count,
+                                                     float *__restrict__ out) {
+  uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t dim = wave % dim_bound;
+  uint32_t chunk = wave / dim_bound;
+  uint32_t n_chunks = (count + 63u) / 64u;
+  if (chunk >= n_chunks)
+    return; // whole wave exits together
+  const uint32_t *d = dirs + 32u * dim;
+  uint32_t s0 = chunk * 64u;
+  uint32_t s = s0 + lane;
+  uint32_t delta = lane == 0 ? sobol_u32(d, s0) : d[__builtin_ctz(s)];
+  uint32_t x = wave_xor_scan(delta);
+  if (s < count)
+    out[size_t(s) * dim_bound + dim] = u32_to_unit_float(x ^ sobol_scramble(dim));
+}
+
+hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
+                              uint32_t count, float *out, hipStream_t s) {
+  uint64_t waves = uint64_t(dim_bound) * ((count + 63u) / 64u);
+  uint32_t blocks = uint32_t((waves + 3) / 4);
+  if (blocks == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_sobol_table, dim3(blocks), dim3(256), 0, s, dirs,
+                     dim_bound, count, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Bulk closest hit for a ray array (parity tests, tools).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_intersect(DeviceScene sc,
+                                                   const float *__restrict__ rays,
+                                                   uint32_t n, float *t,
+                                                   uint32_t *idx, uint32_t *back) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float *r = rays + 6 * size_t(i);
+  Ray<float> ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+  Hit<float> h = {0.f, kNoHit, 0};
+  if (sc.n_nodes != 0)
+    h = trace_closest<false>(sc.nodes, sc.tris, ray, 3.4028235e38f, -1.0f,
+                             nullptr);
+  t[i] = h.t;
+  idx[i] = h.idx;
+  back[i] = h.back;
+}
+
+hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
+                            uint32_t n, float *t, uint32_t *idx, uint32_t *back,
+                            hipStream_t s) {
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_intersect, dim3((n + 255) / 256), dim3(256), 0, s, sc,
+                     rays, n, t, idx, back);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Mega-kernel: every thread runs whole paths for its pixel and sample range,
+// then the partial sums of a pixel's threads are reduced across lanes.
+// reference: include/render/detail/integrate_image/mega_kernel/run_impl.h:140-206,
+// include/integrate/rendering_equation.h:27-129
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ V3<float>
+integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location,
+                  uint32_t *rays_traced) {
+  V3<float> total = {0.f, 0.f, 0.f};
+  uint32_t n_rays_traced = 0;
+  for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
+    R rng;
+    if constexpr (sizeof(R) == sizeof(TableRng)) {
+      rng = TableRng{p.rng.vals, sample * p.rng.dim_bound,
+                     initial_dimension(location, p.rng.initial_dim_bound),
+                     p.rng.dim_bound};
+    } else {
+      rng = make_hash_rng(sample + location * p.rng.samples_per);
+    }
+    Ray<float> rays[2];
+    if (p.initial_rays) {
+      const float *r = p.initial_rays + 6 * size_t(info.x);
+      rays[0] = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+    } else {
+      float xo = rng.next();
+      float yo = rng.next();
+      rays[0] = camera_ray(p.film_to_world, float(info.x) + xo,
+                           float(info.y) + yo, p.wd.x_dim, p.wd.y_dim);
+    }
+    uint32_t n_rays = 1;
+    PathState<float> st = initial_path_state<float>();
+    for (;;) {
+      Hit<float> hits[2];
+      for (uint32_t i = 0; i < n_rays; ++i) {
+        bool light = i < st.n_light;
+        float limit = light ? st.light_target + 2e-6f : 3.4028235e38f;
+        float occl = light ? st.light_target - 2e-6f : -1.0f;
+        hits[i] = trace_closest<false>(p.scene.nodes, p.scene.tris, rays[i],
+                                       limit, occl, nullptr);
+      }
+      n_rays_traced += n_rays;
+      PathState<float> ns;
+      Ray<float> new_rays[2];
+      uint32_t n_new;
+      bool done = rendering_equation_iteration(
+          st, rng, rays[n_rays - 1], hits, p.settings, p.scene.view, &ns,
+          new_rays, &n_new);
+      if (done) {
+        total += ns.total;
+        break;
+      }
+      st = ns;
+      rays[0] = new_rays[0];
+      rays[1] = new_rays[1];
+      n_rays = n_new;
+    }
+  }
+  *rays_traced = n_rays_traced;
+  return total;
+}
+
+__global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
+                                                    uint32_t start_block) {
+  uint32_t block = blockIdx.x + start_block;
+  ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
+  V3<float> total = {0.f, 0.f, 0.f};
+  uint32_t traced = 0;
+  if (!info.exit) {
+    uint32_t location = info.x + info.y * p.wd.x_dim;
+    if (p.rng.type == kRngSobel)
+      total = integrate_samples<TableRng>(p, info, location, &traced);
+    else
+      total = integrate_samples<HashRng>(p, info, location, &traced);
+  }
+  // Per-pixel reduction over the sample block (lanes of one pixel are
+  // consecutive; groups larger than a wave go through LDS).
+  uint32_t sbs = p.wd.sample_block_size;
+  uint32_t group = sbs < 64u ? sbs : 64u;
+  total.x = group_reduce_sum(total.x, group);
+  total.y = group_reduce_sum(total.y, group);
+  total.z = group_reduce_sum(total.z, group);
+  bool writer = (threadIdx.x % sbs) == 0 && !info.exit;
+  if (sbs > 64u) {
+    __shared__ float partial[4][3];
+    uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+      partial[w][0] = total.x;
+      partial[w][1] = total.y;
+      partial[w][2] = total.z;
+    }
+    __syncthreads();
+    if (writer) {
+      uint32_t ng = sbs / 64u, w0 = w;
+      float v[4][3];
+      for (uint32_t g = 0; g < ng; ++g)
+        for (int c = 0; c < 3; ++c)
+          v[g][c] = partial[w0 + g][c];
+      for (uint32_t off = ng / 2; off >= 1; off /= 2)
+        for (uint32_t i = 0; i < off; ++i)
+          for (int c = 0; c < 3; ++c)
+            v[i][c] = v[i][c] + v[i + off][c];
+      total = {v[0][0], v[0][1], v[0][2]};
+    }
+  }
+  if (writer) {
+    if (p.divide_by_spp) {
+      float spp = float(p.rng.samples_per);
+      total = {total.x / spp, total.y / spp, total.z / spp};
+    }
+    size_t idx = size_t(p.wd.sample_block_idx(block)) +
+                 size_t(p.wd.num_sample_blocks) *
+                     (size_t(info.x) + size_t(info.y) * p.wd.x_dim);
+    p.out_rgb[3 * idx + 0] = total.x;
+    p.out_rgb[3 * idx + 1] = total.y;
+    p.out_rgb[3 * idx + 2] = total.z;
+  }
+  if (p.ray_counter) {
+    uint32_t lane_total = traced;
+    for (int off = 32; off >= 1; off >>= 1)
+      lane_total += __shfl_xor(lane_total, off, 64);
+    if ((threadIdx.x & 63u) == 0)
+      atomicAdd(p.ray_counter, (unsigned long long)lane_total);
+  }
+}
+
+hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
+                             uint32_t end_block, hipStream_t s) {
+  if (end_block <= start_block)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_megakernel, dim3(end_block - start_block),
+                     dim3(p.wd.block_size), 0, s, p, start_block);
+  return hipGetLastError();
+}
+
+__global__ void k_reduce_sample_blocks(const float *__restrict__ in, uint32_t nsb,
+                                       uint32_t n_pixels, uint32_t spp,
+                                       float *__restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pixels)
+    return;
+  float r = 0.f, g = 0.f, b = 0.f;
+  for (uint32_t k = 0; k < nsb; ++k) {
+    const float *v = in + 3 * (size_t(i) * nsb + k);
+    r = r + v[0];
+    g = g + v[1];
+    b = b + v[2];
+  }
+  float f = float(spp);
+  out[3 * size_t(i) + 0] = r / f;
+  out[3 * size_t(i) + 1] = g / f;
+  out[3 * size_t(i) + 2] = b / f;
+}
+
+hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
+                                       uint32_t n_pixels, uint32_t samples_per,
+                                       float *out, hipStream_t s) {
+  if (n_pixels == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_reduce_sample_blocks, dim3((n_pixels + 255) / 256),
+                     dim3(256), 0, s, in, nsb, n_pixels, samples_per, out);
+  return hipGetLastError();
+}
+
+__global__ void k_to_bgra(const float *__restrict__ rgb, uint32_t n,
+                          uint32_t *__restrict__ bgra) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  bgra[i] = float_rgb_to_bgra32(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+}
+
+hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
+                          hipStream_t s) {
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_to_bgra, dim3((n + 255) / 256), dim3(256), 0, s, rgb, n,
+                     bgra);
+  return hipGetLastError();
+}
+
+} // namespace gpu
+} // namespace pt

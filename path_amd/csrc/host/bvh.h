@@ -1,0 +1,77 @@
+// Host SAH BVH builder and its repack into the GPU node layout.
+#pragma once
+
+#include "../core/geometry.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace pt {
+namespace host {
+
+// Binary BVH in the reference's layout (include/intersect/accel/detail/bvh/node.h:
+// 22-126): node 0 is the root, children are allocated in pairs, a leaf covers
+// the contiguous range [start, end) of the permuted triangle array.
+struct BvhNode {
+  V3<float> lo, hi;
+  uint32_t is_leaf;
+  uint32_t a; // inner: left child; leaf: start
+  uint32_t b; // inner: right child; leaf: end
+};
+
+struct Bvh {
+  std::vector<BvhNode> nodes;
+  std::vector<uint32_t> perm; // perm[i] = flattened index of the i-th leaf-ordered triangle
+  uint32_t max_depth = 0;
+  uint32_t max_leaf = 0;
+};
+
+struct BvhSettings {
+  float traversal_per_intersect_cost = 0.5f; // reference default
+};
+
+// SAH object-split builder following src/intersect/accel/sbvh/detail/generator.cu:
+// 16-438 (spatial splits off, the reference default): per node and axis, sort
+// by bound centroid, sweep all split points with cost i*SA_L + (n-i)*SA_R, make
+// a leaf when cost/SA + traversal_cost >= n. Centroid ties keep input order
+// (the reference's std::sort leaves them unspecified).
+Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
+                  const BvhSettings &settings = {});
+
+// GPU layout: each inner node stores both child boxes (64 bytes):
+//   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y), b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y),
+//   c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z), d = (L ref, R ref, L count, R count)
+// A child ref >= 0 is an inner-node index, a ref < 0 is ~(first triangle) of a
+// leaf with `count` triangles. Boxes are padded outward by `pad` so that the
+// box test is conservative with respect to the float triangle test (the
+// closest hit then does not depend on traversal order).
+struct GpuNode {
+  float a[4], b[4], c[4];
+  int32_t d[4];
+};
+
+// Triangle in leaf order: (v0, idx bits), (e1, 0), (e2, 0).
+struct GpuTri {
+  float v0[3];
+  uint32_t idx;
+  float e1[3];
+  uint32_t pad0;
+  float e2[3];
+  uint32_t pad1;
+};
+
+struct GpuBvh {
+  std::vector<GpuNode> nodes;
+  std::vector<GpuTri> tris;
+  float pad = 0.f;
+  uint32_t max_depth = 0;
+};
+
+GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris);
+
+// Conservative padding for a scene: a relative 2^-13 of its largest
+// coordinate magnitude.
+float bvh_padding(const std::vector<TriGeom<float>> &tris);
+
+} // namespace host
+} // namespace pt

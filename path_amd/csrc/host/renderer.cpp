@@ -1,0 +1,292 @@
+#include "renderer.h"
+
+#include "settings.h"
+
+#include <rocrand/rocrand.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace pt;
+using namespace pt::host;
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return set_error(PT_ERR_HIP, std::string(#expr) + ": " +                 \
+                                       hipGetErrorString(e_));                 \
+  } while (0)
+
+const FlatScene &pt_scene::flat() const {
+  if (flat_gen_ != generation) {
+    flat_ = flatten(scene);
+    flat_gen_ = generation;
+  }
+  return flat_;
+}
+
+const Bvh &pt_scene::bvh() const {
+  if (bvh_gen_ != generation) {
+    bvh_ = build_sah_bvh(flat().tris);
+    bvh_gen_ = generation;
+  }
+  return bvh_;
+}
+
+hipError_t DeviceSceneStore::upload(const FlatScene &f, const Bvh &bvh,
+                                    hipStream_t s) {
+  GpuBvh g = pack_gpu_bvh(bvh, f.tris);
+  std::vector<Material<float>> mats;
+  for (const auto &m : f.materials)
+    mats.push_back(m.to<float>());
+  hipError_t e;
+  if ((e = nodes.upload(g.nodes.data(), g.nodes.size(), s)) != hipSuccess ||
+      (e = tris.upload(g.tris.data(), g.tris.size(), s)) != hipSuccess ||
+      (e = shade_tris.upload(f.tris.data(), f.tris.size(), s)) != hipSuccess ||
+      (e = normals.upload(f.normals.data(), f.normals.size(), s)) != hipSuccess ||
+      (e = tri_mat.upload(f.tri_mat.data(), f.tri_mat.size(), s)) != hipSuccess ||
+      (e = materials.upload(mats.data(), mats.size(), s)) != hipSuccess ||
+      (e = light_tris.upload(f.light_tris.data(), f.light_tris.size(), s)) !=
+          hipSuccess ||
+      (e = light_cdf.upload(f.light_cdf.data(), f.light_cdf.size(), s)) !=
+          hipSuccess)
+    return e;
+  dev.nodes = reinterpret_cast<const float4 *>(nodes.ptr);
+  dev.tris = reinterpret_cast<const float4 *>(tris.ptr);
+  dev.n_nodes = uint32_t(g.nodes.size());
+  dev.n_tris = uint32_t(g.tris.size());
+  dev.view.tris = shade_tris.ptr;
+  dev.view.normals = normals.ptr;
+  dev.view.tri_mat = tri_mat.ptr;
+  dev.view.materials = materials.ptr;
+  dev.view.light_tris = light_tris.ptr;
+  dev.view.light_cdf = light_cdf.ptr;
+  dev.view.n_lights = uint32_t(f.light_tris.size());
+  max_depth = g.max_depth;
+  return hipStreamSynchronize(s);
+}
+
+pt_renderer::~pt_renderer() {
+  if (ev_start)
+    (void)hipEventDestroy(ev_start);
+  if (ev_stop)
+    (void)hipEventDestroy(ev_stop);
+  if (stream)
+    (void)hipStreamDestroy(stream);
+}
+
+bool pt_renderer::init(int dev, std::string *err) {
+  device = dev;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipEventCreate(&ev_start);
+  if (e == hipSuccess)
+    e = hipEventCreate(&ev_stop);
+  if (e != hipSuccess) {
+    *err = std::string("HIP init failed: ") + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+int pt_renderer::ensure_scene(const pt_scene &scene) {
+  if (cached_scene == &scene && cached_gen == scene.generation)
+    return PT_OK;
+  const FlatScene &f = scene.flat();
+  const Bvh &b = scene.bvh();
+  if (b.max_depth + 1 >= uint32_t(gpu_stack_size()))
+    return set_error(PT_ERR_UNSUPPORTED,
+                     "BVH depth " + std::to_string(b.max_depth) +
+                         " exceeds the traversal stack");
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(scene_store.upload(f, b, stream));
+  cached_scene = &scene;
+  cached_gen = scene.generation;
+  return PT_OK;
+}
+
+int pt_renderer::ensure_sobol(uint32_t dim_bound, uint32_t count) {
+  if (sobol_dim_bound == dim_bound && sobol_count == count)
+    return PT_OK;
+  if (dim_bound == 0 || dim_bound > 20000)
+    return set_error(PT_ERR_INVALID, "max_sample_size must be in [1, 20000]");
+  const unsigned int *v = nullptr;
+  if (rocrand_get_direction_vectors32(&v, ROCRAND_DIRECTION_VECTORS_32_JOEKUO6) !=
+          ROCRAND_STATUS_SUCCESS ||
+      !v)
+    return set_error(PT_ERR_UNSUPPORTED, "Joe-Kuo direction numbers unavailable");
+  HIP_TRY(sobol_dirs.upload(v, size_t(dim_bound) * 32, stream));
+  HIP_TRY(sobol_vals.ensure(size_t(dim_bound) * count));
+  HIP_TRY(gpu::launch_sobol_table(sobol_dirs.ptr, dim_bound, count,
+                                  sobol_vals.ptr, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  sobol_dim_bound = dim_bound;
+  sobol_count = count;
+  return PT_OK;
+}
+
+int pt_renderer::sobol_table_to_host(uint32_t dim_bound, uint32_t count,
+                                     float *out) {
+  HIP_TRY(hipSetDevice(device));
+  int rc = ensure_sobol(dim_bound, count);
+  if (rc != PT_OK)
+    return rc;
+  HIP_TRY(hipMemcpyAsync(out, sobol_vals.ptr, size_t(dim_bound) * count * 4,
+                         hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return PT_OK;
+}
+
+int pt_renderer::intersect(const pt_scene &scene, const float *rays, uint32_t n,
+                           float *t, uint32_t *idx, uint32_t *back) {
+  HIP_TRY(hipSetDevice(device));
+  int rc = ensure_scene(scene);
+  if (rc != PT_OK)
+    return rc;
+  if (n == 0)
+    return PT_OK;
+  HIP_TRY(scratch_rays.upload(rays, size_t(n) * 6, stream));
+  HIP_TRY(scratch_t.ensure(n));
+  HIP_TRY(scratch_idx.ensure(n));
+  HIP_TRY(scratch_back.ensure(n));
+  HIP_TRY(gpu::launch_intersect(scene_store.dev, scratch_rays.ptr, n,
+                                scratch_t.ptr, scratch_idx.ptr,
+                                scratch_back.ptr, stream));
+  HIP_TRY(hipMemcpyAsync(t, scratch_t.ptr, n * 4, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(idx, scratch_idx.ptr, n * 4, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(back, scratch_back.ptr, n * 4, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return PT_OK;
+}
+
+int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
+                        const pt_scene &scene, uint32_t samples_per,
+                        const pt_settings &settings, bool show_progress,
+                        bool show_times, double *seconds) {
+  (void)show_progress;
+  if (samples_per == 0)
+    return set_error(PT_ERR_INVALID, "samples_per must be positive");
+  uint32_t x_dim, y_dim;
+  if (spec.type == PT_SAMPLE_SQUARE_IMAGE) {
+    x_dim = spec.x_dim;
+    y_dim = spec.y_dim;
+  } else if (spec.type == PT_SAMPLE_INITIAL_RAYS) {
+    if (!spec.rays)
+      return set_error(PT_ERR_INVALID, "InitialRays sample spec without rays");
+    x_dim = spec.n_rays;
+    y_dim = 1;
+  } else {
+    return set_error(PT_ERR_UNSUPPORTED,
+                     "InitialIdxAndDir sample spec is not supported by this entry point");
+  }
+  if (x_dim == 0 || y_dim == 0)
+    return set_error(PT_ERR_INVALID, "empty image");
+  if (out.type == PT_OUTPUT_PER_STEP)
+    return set_error(PT_ERR_UNSUPPORTED, "per-step output is not supported yet");
+  if (!out.data)
+    return set_error(PT_ERR_INVALID, "null output buffer");
+  if (settings.rng == PT_RNG_SOBEL &&
+      (settings.max_sample_size == 0 || settings.max_sample_size > 20000))
+    return set_error(PT_ERR_INVALID, "max_sample_size must be in [1, 20000]");
+  if (settings.kernel_approach != PT_MEGA_KERNEL &&
+      settings.kernel_approach != PT_STREAMING)
+    return set_error(PT_ERR_INVALID, "unknown kernel approach");
+
+  HIP_TRY(hipSetDevice(device));
+  int rc = ensure_scene(scene);
+  if (rc != PT_OK)
+    return rc;
+  gpu::RngParams rng{};
+  rng.type = settings.rng;
+  rng.samples_per = samples_per;
+  if (settings.rng == PT_RNG_SOBEL) {
+    rc = ensure_sobol(settings.max_sample_size, samples_per);
+    if (rc != PT_OK)
+      return rc;
+    rng.vals = sobol_vals.ptr;
+    rng.dim_bound = settings.max_sample_size;
+    // reference: src/rng/sobel/sobel.cu:74 (64) and rng_from_sequence_gen.h:101-102
+    rng.initial_dim_bound = settings.max_sample_size < 64 ? settings.max_sample_size : 64;
+  }
+  const uint32_t n_pixels = x_dim * y_dim;
+  HIP_TRY(counter.ensure(1));
+  HIP_TRY(hipMemsetAsync(counter.ptr, 0, sizeof(unsigned long long), stream));
+  if (spec.type == PT_SAMPLE_INITIAL_RAYS)
+    HIP_TRY(scratch_rays.upload(spec.rays, size_t(spec.n_rays) * 6, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipEventRecord(ev_start, stream));
+  const float *final_rgb = nullptr;
+  if (settings.kernel_approach == PT_MEGA_KERNEL) {
+    WorkDivision wd;
+    if (!WorkDivision::make(work_division_settings(settings.render_work_division),
+                            samples_per, x_dim, y_dim, &wd))
+      return set_error(PT_ERR_INVALID, "invalid render work division settings");
+    if (wd.block_size > 1024)
+      return set_error(PT_ERR_INVALID, "block_size above 1024");
+    HIP_TRY(rgb.ensure(size_t(wd.num_sample_blocks) * n_pixels * 3));
+    gpu::MegaParams p{};
+    p.scene = scene_store.dev;
+    p.settings = integrator_settings(settings);
+    p.wd = wd;
+    p.rng = rng;
+    std::memcpy(p.film_to_world, spec.film_to_world, sizeof(p.film_to_world));
+    p.initial_rays = spec.type == PT_SAMPLE_INITIAL_RAYS ? scratch_rays.ptr : nullptr;
+    p.out_rgb = rgb.ptr;
+    p.divide_by_spp = wd.num_sample_blocks == 1;
+    p.ray_counter = counter.ptr;
+    uint32_t total = wd.total_num_blocks();
+    uint32_t chunk = settings.max_blocks_per_launch_gpu ? settings.max_blocks_per_launch_gpu : total;
+    for (uint32_t start = 0; start < total; start += chunk) {
+      uint32_t end = start + chunk < total ? start + chunk : total;
+      HIP_TRY(gpu::launch_megakernel(p, start, end, stream));
+    }
+    final_rgb = rgb.ptr;
+    if (wd.num_sample_blocks != 1) {
+      HIP_TRY(rgb2.ensure(size_t(n_pixels) * 3));
+      HIP_TRY(gpu::launch_reduce_sample_blocks(rgb.ptr, wd.num_sample_blocks,
+                                               n_pixels, samples_per, rgb2.ptr,
+                                               stream));
+      final_rgb = rgb2.ptr;
+    }
+  } else {
+    rc = render_streaming(*this, spec, scene_store.dev, samples_per, settings,
+                          rng, x_dim, y_dim, &final_rgb);
+    if (rc != PT_OK)
+      return rc;
+  }
+  if (out.type == PT_OUTPUT_BGRA) {
+    HIP_TRY(bgra.ensure(n_pixels));
+    HIP_TRY(gpu::launch_to_bgra(final_rgb, n_pixels, bgra.ptr, stream));
+  }
+  HIP_TRY(hipEventRecord(ev_stop, stream));
+  if (out.type == PT_OUTPUT_BGRA)
+    HIP_TRY(hipMemcpyAsync(out.data, bgra.ptr, size_t(n_pixels) * 4,
+                           hipMemcpyDeviceToHost, stream));
+  else
+    HIP_TRY(hipMemcpyAsync(out.data, final_rgb, size_t(n_pixels) * 12,
+                           hipMemcpyDeviceToHost, stream));
+  unsigned long long rays_traced = 0;
+  HIP_TRY(hipMemcpyAsync(&rays_traced, counter.ptr, sizeof(rays_traced),
+                         hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  auto t1 = std::chrono::steady_clock::now();
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  stats.kernel_ms = ms;
+  stats.rays = rays_traced;
+  stats.samples = uint64_t(samples_per) * n_pixels;
+  double secs = std::chrono::duration<double>(t1 - t0).count();
+  if (seconds)
+    *seconds = secs;
+  if (show_times)
+    std::printf("render: %.3f ms kernel, %.3f ms total, %llu rays\n", ms,
+                secs * 1e3, rays_traced);
+  return PT_OK;
+}

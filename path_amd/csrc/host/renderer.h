@@ -1,0 +1,133 @@
+// Renderer state behind the C ABI: scene caches, device buffers, launches.
+#pragma once
+
+#include "../../../include/path_amd.h"
+#include "../gpu/kernels.h"
+#include "bvh.h"
+#include "scene.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+namespace pt {
+namespace host {
+int set_error(int code, const std::string &msg);
+inline int gpu_stack_size() { return gpu::kTraceStackSize; }
+} // namespace host
+} // namespace pt
+
+struct pt_scene {
+  pt::host::Scene scene;
+  uint64_t generation = 0;
+
+  const pt::host::FlatScene &flat() const;
+  const pt::host::Bvh &bvh() const;
+
+private:
+  mutable uint64_t flat_gen_ = ~0ull, bvh_gen_ = ~0ull;
+  mutable pt::host::FlatScene flat_;
+  mutable pt::host::Bvh bvh_;
+};
+
+namespace pt {
+namespace host {
+
+// Grow-only device allocation.
+template <class T> struct DeviceBuf {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DeviceBuf() = default;
+  DeviceBuf(const DeviceBuf &) = delete;
+  DeviceBuf &operator=(const DeviceBuf &) = delete;
+  ~DeviceBuf() {
+    if (ptr)
+      (void)hipFree(ptr);
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= count && ptr)
+      return hipSuccess;
+    if (ptr)
+      (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+    hipError_t e = hipMalloc(&ptr, (n ? n : 1) * sizeof(T));
+    if (e == hipSuccess)
+      count = n;
+    return e;
+  }
+  hipError_t upload(const T *src, size_t n, hipStream_t s) {
+    hipError_t e = ensure(n);
+    if (e != hipSuccess || n == 0)
+      return e;
+    return hipMemcpyAsync(ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+// Device-resident copy of one scene.
+struct DeviceSceneStore {
+  DeviceBuf<GpuNode> nodes;
+  DeviceBuf<GpuTri> tris;
+  DeviceBuf<TriGeom<float>> shade_tris;
+  DeviceBuf<V3<float>> normals;
+  DeviceBuf<uint32_t> tri_mat;
+  DeviceBuf<Material<float>> materials;
+  DeviceBuf<TriGeom<float>> light_tris;
+  DeviceBuf<float> light_cdf;
+  gpu::DeviceScene dev;
+  uint32_t max_depth = 0;
+
+  hipError_t upload(const FlatScene &f, const Bvh &bvh, hipStream_t s);
+};
+
+} // namespace host
+} // namespace pt
+
+struct pt_renderer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  pt_render_stats stats{};
+
+  // scene cache
+  const pt_scene *cached_scene = nullptr;
+  uint64_t cached_gen = ~0ull;
+  pt::host::DeviceSceneStore scene_store;
+
+  // sampler cache
+  uint32_t sobol_dim_bound = 0, sobol_count = 0;
+  pt::host::DeviceBuf<uint32_t> sobol_dirs;
+  pt::host::DeviceBuf<float> sobol_vals;
+
+  // outputs / scratch
+  pt::host::DeviceBuf<float> rgb, rgb2;
+  pt::host::DeviceBuf<uint32_t> bgra;
+  pt::host::DeviceBuf<unsigned long long> counter;
+  pt::host::DeviceBuf<float> scratch_rays;
+  pt::host::DeviceBuf<float> scratch_t;
+  pt::host::DeviceBuf<uint32_t> scratch_idx, scratch_back;
+
+  ~pt_renderer();
+  bool init(int device, std::string *err);
+  int ensure_scene(const pt_scene &scene);
+  int ensure_sobol(uint32_t dim_bound, uint32_t count);
+  int render(const pt_sample_spec &spec, const pt_output &out,
+             const pt_scene &scene, uint32_t samples_per,
+             const pt_settings &settings, bool show_progress, bool show_times,
+             double *seconds);
+  int intersect(const pt_scene &scene, const float *rays, uint32_t n, float *t,
+                uint32_t *idx, uint32_t *back);
+  int sobol_table_to_host(uint32_t dim_bound, uint32_t count, float *out);
+};
+
+namespace pt {
+namespace host {
+// Wavefront ("Streaming") integrator; writes the final per-pixel FloatRGB
+// (already divided by samples_per) into a device buffer owned by the renderer.
+int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
+                     const gpu::DeviceScene &scene, uint32_t samples_per,
+                     const pt_settings &settings, const gpu::RngParams &rng,
+                     uint32_t x_dim, uint32_t y_dim, const float **final_rgb);
+} // namespace host
+} // namespace pt

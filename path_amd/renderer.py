@@ -1,0 +1,274 @@
+"""Python mirror of the reference renderer interface over the C ABI.
+
+reference: include/render/renderer.h:43-70 (Renderer::render),
+include/render/renderer_from_files.h:22-43 (RendererFromFiles),
+src/render/config_io.cpp:14-34 (load_config / print_config).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Settings:
+    """render::Settings. Attribute access reads/writes the POD fields; enum
+    fields accept the reference's enum names (e.g. s.term_prob = "NIters")."""
+
+    _ENUMS = {"kernel_approach": L.KERNEL_APPROACH, "accel": L.ACCEL,
+              "light_sampler": L.LIGHT_SAMPLER, "dir_sampler": L.DIR_SAMPLER,
+              "term_prob": L.TERM_PROB, "rng": L.RNG}
+
+    def __init__(self, raw: L.PtSettings | None = None):
+        object.__setattr__(self, "raw", raw if raw is not None else L.PtSettings())
+        if raw is None:
+            L.lib().pt_settings_default(C.byref(self.raw))
+
+    @classmethod
+    def from_yaml(cls, path: str | Path) -> "Settings":
+        s = cls()
+        L.check(L.lib().pt_settings_load_yaml(str(path).encode(), C.byref(s.raw)))
+        return s
+
+    def to_yaml(self) -> str:
+        need = C.c_size_t()
+        L.check(L.lib().pt_settings_to_yaml(C.byref(self.raw), None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value + 1)
+        L.check(L.lib().pt_settings_to_yaml(C.byref(self.raw), buf, need.value + 1, None))
+        return buf.value.decode()
+
+    def __getattr__(self, name):
+        return getattr(self.raw, name)
+
+    def __setattr__(self, name, value):
+        if name in self._ENUMS and isinstance(value, str):
+            value = self._ENUMS[name][value]
+        setattr(self.raw, name, value)
+
+    def copy(self) -> "Settings":
+        raw = L.PtSettings()
+        C.memmove(C.byref(raw), C.byref(self.raw), C.sizeof(raw))
+        return Settings(raw)
+
+
+def material(kind: str, diffuse=(0, 0, 0), specular=(0, 0, 0), shininess=1.0,
+             ior=1.0, emission=(0, 0, 0), diffuse_weight=1.0,
+             glossy_weight=0.0) -> L.PtMaterial:
+    m = L.PtMaterial()
+    m.type = L.BSDF[kind]
+    m.diffuse[:] = list(map(float, diffuse))
+    m.specular[:] = list(map(float, specular))
+    m.shininess = shininess
+    m.ior = ior
+    m.emission[:] = list(map(float, emission))
+    m.diffuse_weight = diffuse_weight
+    m.glossy_weight = glossy_weight
+    return m
+
+
+class Scene:
+    """scene::Scene held by the native library."""
+
+    def __init__(self, handle=None):
+        if handle is None:
+            handle = C.c_void_p()
+            L.check(L.lib().pt_scene_create(C.byref(handle)))
+        self.handle = handle
+
+    @classmethod
+    def load(cls, path: str | Path, width_height_ratio: float = 1.0, quiet: bool = True):
+        """ScenefileLoader::load_scene -> (Scene, film_to_world[12])."""
+        h = C.c_void_p()
+        ftw = np.zeros(12, np.float32)
+        L.check(L.lib().pt_scene_load(str(path).encode(), width_height_ratio, int(quiet),
+                                      C.byref(h), L.fptr(ftw)))
+        return cls(h), ftw
+
+    def add_mesh(self, verts, tri_mats, materials, normals=None, transform=None):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+        tm = np.ascontiguousarray(tri_mats, np.uint32)
+        mats = (L.PtMaterial * len(materials))(*materials)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32)
+        L.check(L.lib().pt_scene_add_mesh(self.handle, L.fptr(verts), L.fptr(nrm), L.uptr(tm),
+                                          len(tm), mats, len(materials), L.fptr(tr)))
+
+    def add_obj(self, path, transform=None):
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32)
+        L.check(L.lib().pt_scene_add_obj(self.handle, str(path).encode(), L.fptr(tr)))
+
+    def counts(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        L.check(L.lib().pt_scene_counts(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def flat(self) -> dict:
+        """World-space flattened arrays (flat_triangle.h:79-146)."""
+        nt, nm, nl = self.counts()
+        tris = np.zeros((nt, 9), np.float32)
+        normals = np.zeros((nt, 9), np.float32)
+        tri_mat = np.zeros(nt, np.uint32)
+        mats = (L.PtMaterial * max(nm, 1))()
+        lt = np.zeros((nl, 9), np.float32)
+        lc = np.zeros(nl, np.float32)
+        L.check(L.lib().pt_scene_flat(self.handle, L.fptr(tris), L.fptr(normals), L.uptr(tri_mat),
+                                      mats, L.fptr(lt), L.fptr(lc)))
+        return {"tris": tris, "normals": normals, "tri_mat": tri_mat,
+                "materials": mats, "n_materials": nm, "light_tris": lt, "light_cdf": lc}
+
+    def bvh(self) -> dict:
+        n = C.c_uint32()
+        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), None, None, None, None, None,
+                                     None, None))
+        nn = n.value
+        nt = self.counts()[0]
+        lo = np.zeros((nn, 3), np.float32)
+        hi = np.zeros((nn, 3), np.float32)
+        leaf = np.zeros(nn, np.uint32)
+        a = np.zeros(nn, np.uint32)
+        b = np.zeros(nn, np.uint32)
+        perm = np.zeros(nt, np.uint32)
+        depth = C.c_uint32()
+        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), L.fptr(lo), L.fptr(hi),
+                                     L.uptr(leaf), L.uptr(a), L.uptr(b), L.uptr(perm),
+                                     C.byref(depth)))
+        return {"lo": lo, "hi": hi, "is_leaf": leaf, "a": a, "b": b, "perm": perm,
+                "max_depth": depth.value}
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and L._lib is not None:
+            L.lib().pt_scene_destroy(h)
+            self.handle = None
+
+
+def camera_from_look(look, up, pos, height_angle_deg: float, width_height_ratio: float):
+    out = np.zeros(12, np.float32)
+    f = lambda v: L.fptr(np.ascontiguousarray(v, np.float32))
+    look = np.asarray(look, np.float32)
+    up = np.asarray(up, np.float32)
+    pos = np.asarray(pos, np.float32)
+    L.check(L.lib().pt_camera_from_look(f(look), f(up), f(pos), height_angle_deg,
+                                        width_height_ratio, L.fptr(out)))
+    return out
+
+
+@dataclass
+class RenderStats:
+    kernel_ms: float
+    trace_ms: float
+    rays: int
+    samples: int
+    trace_launches: int
+
+
+class Renderer:
+    """render::Renderer on one GPU (ExecutionModel::GPU)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib().pt_renderer_create(device, C.byref(h)))
+        self.handle = h
+
+    def render(self, scene: Scene, film_to_world, x_dim: int, y_dim: int, samples_per: int,
+               settings: Settings | None = None, output: str = "FloatRGB",
+               show_progress: bool = False, show_times: bool = False,
+               rays: np.ndarray | None = None):
+        """Renderer::render with a SquareImage (or InitialRays if `rays` is
+        given, shape (n, 6)) sample spec. Returns (image, seconds)."""
+        settings = settings or Settings()
+        spec = L.PtSampleSpec()
+        if rays is not None:
+            rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+            spec.type = L.SAMPLE_SPEC["InitialRays"]
+            spec.rays = L.fptr(rays)
+            spec.n_rays = rays.shape[0]
+            x_dim, y_dim = rays.shape[0], 1
+        else:
+            spec.type = L.SAMPLE_SPEC["SquareImage"]
+            spec.x_dim, spec.y_dim = x_dim, y_dim
+            spec.film_to_world[:] = [float(v) for v in np.asarray(film_to_world).ravel()]
+        out = L.PtOutput()
+        out.type = L.OUTPUT[output]
+        if output == "BGRA":
+            img = np.zeros((y_dim, x_dim), np.uint32)
+        else:
+            img = np.zeros((y_dim, x_dim, 3), np.float32)
+        out.data = img.ctypes.data
+        secs = C.c_double()
+        L.check(L.lib().pt_render(self.handle, C.byref(spec), C.byref(out), scene.handle,
+                                  samples_per, C.byref(settings.raw), int(show_progress),
+                                  int(show_times), C.byref(secs)))
+        return img, secs.value
+
+    def intersect(self, scene: Scene, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        idx = np.zeros(n, np.uint32)
+        back = np.zeros(n, np.uint32)
+        L.check(L.lib().pt_intersect(self.handle, scene.handle, L.fptr(rays), n, L.fptr(t),
+                                     L.uptr(idx), L.uptr(back)))
+        return t, idx, back
+
+    def sobol_table(self, dim_bound: int, count: int) -> np.ndarray:
+        out = np.zeros((count, dim_bound), np.float32)
+        L.check(L.lib().pt_sobol_table(self.handle, dim_bound, count, L.fptr(out)))
+        return out
+
+    def last_stats(self) -> RenderStats:
+        s = L.PtRenderStats()
+        L.check(L.lib().pt_last_render_stats(self.handle, C.byref(s)))
+        return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and L._lib is not None:
+            L.lib().pt_renderer_destroy(h)
+            self.handle = None
+
+
+def sobol_direction_numbers(dim_bound: int) -> np.ndarray:
+    out = np.zeros((dim_bound, 32), np.uint32)
+    L.check(L.lib().pt_sobol_direction_numbers(dim_bound, L.uptr(out)))
+    return out
+
+
+def work_division(settings: L.PtWorkDivisionSettings, samples_per: int, x_dim: int, y_dim: int):
+    out = np.zeros(11, np.uint32)
+    L.check(L.lib().pt_work_division(C.byref(settings), samples_per, x_dim, y_dim, L.uptr(out)))
+    keys = ["block_size", "base_samples_per_thread", "n_threads_per_unit_extra",
+            "sample_block_size", "x_block_size", "y_block_size", "num_sample_blocks",
+            "num_x_blocks", "num_y_blocks", "x_dim", "y_dim"]
+    return dict(zip(keys, map(int, out)))
+
+
+class RendererFromFiles:
+    """RendererFromFiles (include/render/renderer_from_files.h:22-43)."""
+
+    def __init__(self, device: int = 0):
+        self.renderer = None
+        self.device = device
+        self.settings = Settings()
+        self.scene = None
+        self.film_to_world = None
+
+    def load_scene(self, path, width_height_ratio: float = 1.0, quiet: bool = False):
+        self.scene, self.film_to_world = Scene.load(path, width_height_ratio, quiet)
+
+    def load_config(self, path):
+        self.settings = Settings.from_yaml(path)
+
+    def print_config(self):
+        print(self.settings.to_yaml())
+
+    def render(self, x_dim: int, y_dim: int, samples_per: int, output: str = "BGRA",
+               show_progress: bool = False, show_times: bool = False):
+        if self.renderer is None:
+            self.renderer = Renderer(self.device)
+        return self.renderer.render(self.scene, self.film_to_world, x_dim, y_dim, samples_per,
+                                    self.settings, output, show_progress, show_times)
