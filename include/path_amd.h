@@ -247,15 +247,25 @@ int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
                  uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
                  uint32_t *max_depth);
 
-/* Timing of the last pt_render: kernel milliseconds measured with HIP events
- * around the launches (stream-accurate), rays traced, samples. */
+/* Statistics of the last pt_render. kernel_ms spans the launches with HIP
+ * events on the renderer's stream. The per-kernel fields are filled only when
+ * profiling is enabled (pt_set_profiling), which brackets each traversal or
+ * mega-kernel launch with its own events. */
 typedef struct pt_render_stats {
   double kernel_ms;
-  double trace_ms;     /* intersection kernels only (wavefront) */
-  uint64_t rays;       /* camera + extension + shadow rays traced */
+  double trace_ms;         /* ext_ms + shadow_ms */
+  uint64_t rays;           /* camera + extension + shadow rays traced */
   uint64_t samples;
   uint64_t trace_launches;
+  double ext_ms;           /* extension-ray traversal kernels */
+  double shadow_ms;        /* shadow-ray traversal kernels */
+  uint64_t ext_rays, shadow_rays;
+  uint64_t ext_launches, shadow_launches;
+  double mega_ms;          /* mega-kernel launches */
+  uint64_t mega_launches;
+  uint32_t profiled;
 } pt_render_stats;
+int pt_set_profiling(pt_renderer *r, int enable);
 int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out);
 
 #ifdef __cplusplus

@@ -86,6 +86,13 @@ struct Counts {
   uint64_t nodes = 0, tris = 0;
 };
 
+// Traversal counts split by ray kind: [0] continuation/camera rays,
+// [1] light (shadow) rays.
+struct KindCounts {
+  Counts c[2];
+  uint64_t rays[2] = {0, 0};
+};
+
 template <class T>
 Hit<T> loop_all(const std::vector<TriGeom<T>> &tris, const Ray<T> &ray) {
   Hit<T> best{T(0), kNoHit, 0};
@@ -174,7 +181,7 @@ template <class T> struct HashRngT {
 
 template <class T, class R>
 V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
-                       uint32_t location, Counts *cnt, uint64_t *rays) {
+                       uint32_t location, KindCounts *cnt) {
   V3<T> total = {T(0), T(0), T(0)};
   const SceneView<T> view = a.scene->view();
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
@@ -193,9 +200,11 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
     PathState<T> st = initial_path_state<T>();
     for (;;) {
       Hit<T> hits[2];
-      for (uint32_t i = 0; i < n_rays; ++i)
-        hits[i] = bvh_closest(*a.scene, rays_[i], cnt);
-      *rays += n_rays;
+      for (uint32_t i = 0; i < n_rays; ++i) {
+        int kind = i < st.n_light ? 1 : 0;
+        hits[i] = bvh_closest(*a.scene, rays_[i], &cnt->c[kind]);
+        cnt->rays[kind] += 1;
+      }
       PathState<T> ns;
       Ray<T> nr[2];
       uint32_t n_new;
@@ -218,14 +227,15 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
 // single sample block, else the sum over sample blocks divided by spp).
 template <class T>
 int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end,
-                int n_threads, T *out, uint64_t *n_rays, uint64_t *n_nodes,
-                uint64_t *n_tris) {
+                int n_threads, T *out, uint64_t *counts /* 6 */) {
   const WorkDivision &wd = a.wd;
   const uint32_t n_pixels = wd.x_dim * wd.y_dim;
   if (pixel_end == 0 || pixel_end > n_pixels)
     pixel_end = n_pixels;
   std::vector<T> partial(size_t(n_pixels) * wd.num_sample_blocks * 3, T(0));
-  std::atomic<uint64_t> rays{0}, nodes{0}, tris{0};
+  std::atomic<uint64_t> acc[6];
+  for (auto &x : acc)
+    x = 0;
   const int64_t total_blocks = wd.total_num_blocks();
   if (n_threads > 0)
     omp_set_num_threads(n_threads);
@@ -233,8 +243,7 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
   for (int64_t block = 0; block < total_blocks; ++block) {
     std::vector<V3<T>> thread_tot(wd.block_size);
     bool any = false;
-    Counts c;
-    uint64_t r = 0;
+    KindCounts kc;
     for (uint32_t th = 0; th < wd.block_size; ++th) {
       ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
       thread_tot[th] = {T(0), T(0), T(0)};
@@ -245,9 +254,9 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
         continue;
       any = true;
       if (a.rng_type == kRngSobel)
-        thread_tot[th] = integrate_thread<T, TableRngT<T>>(a, info, loc, &c, &r);
+        thread_tot[th] = integrate_thread<T, TableRngT<T>>(a, info, loc, &kc);
       else
-        thread_tot[th] = integrate_thread<T, HashRngT<T>>(a, info, loc, &c, &r);
+        thread_tot[th] = integrate_thread<T, HashRngT<T>>(a, info, loc, &kc);
     }
     if (!any)
       continue;
@@ -263,9 +272,11 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
       partial[3 * idx + 1] = sum.y;
       partial[3 * idx + 2] = sum.z;
     }
-    rays += r;
-    nodes += c.nodes;
-    tris += c.tris;
+    for (int k = 0; k < 2; ++k) {
+      acc[3 * k] += kc.rays[k];
+      acc[3 * k + 1] += kc.c[k].nodes;
+      acc[3 * k + 2] += kc.c[k].tris;
+    }
   }
   T spp = T(float(a.samples_per));
   for (uint32_t p = pixel_begin; p < pixel_end; ++p) {
@@ -278,12 +289,9 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
     for (int ch = 0; ch < 3; ++ch)
       out[3 * size_t(p) + ch] = s[ch] / spp;
   }
-  if (n_rays)
-    *n_rays = rays;
-  if (n_nodes)
-    *n_nodes = nodes;
-  if (n_tris)
-    *n_tris = tris;
+  if (counts)
+    for (int k = 0; k < 6; ++k)
+      counts[k] = acc[k];
   return 0;
 }
 
@@ -410,7 +418,9 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
                   uint32_t x_dim, uint32_t y_dim, const float *sobol,
                   int precision, int n_threads, uint32_t pixel_begin,
                   uint32_t pixel_end, float *out32, double *out64,
-                  uint64_t *counts /* rays, nodes, tris */) {
+                  uint64_t *counts /* 6: per kind (camera/continuation,
+                                      light): rays, nodes, tris */,
+                  double *seconds /* render time, scene setup excluded */) {
   WorkDivisionSettings ws;
   const pt_work_division_settings &w = settings->render_work_division;
   ws.block_size = w.block_size;
@@ -436,8 +446,9 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
   is.normalize_min_prob = settings->normalize_min_prob;
   uint32_t dim_bound = settings->max_sample_size;
   uint32_t init_bound = dim_bound < 64 ? dim_bound : 64;
-  uint64_t c[3] = {0, 0, 0};
+  uint64_t c[6] = {0, 0, 0, 0, 0, 0};
   int rc;
+  double t0 = 0, t1 = 0;
   if (precision == 64) {
     OScene<double> s;
     fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
@@ -446,7 +457,9 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
                          samples_per, settings->rng};
     for (int i = 0; i < 12; ++i)
       a.film_to_world[i] = film_to_world[i];
-    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out64, &c[0], &c[1], &c[2]);
+    t0 = omp_get_wtime();
+    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out64, c);
+    t1 = omp_get_wtime();
   } else {
     OScene<float> s;
     fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
@@ -455,12 +468,18 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
                         samples_per, settings->rng};
     for (int i = 0; i < 12; ++i)
       a.film_to_world[i] = film_to_world[i];
-    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out32, &c[0], &c[1], &c[2]);
+    t0 = omp_get_wtime();
+    rc = render_impl(a, pixel_begin, pixel_end, n_threads, out32, c);
+    t1 = omp_get_wtime();
   }
   if (counts)
     std::memcpy(counts, c, sizeof(c));
+  if (seconds)
+    *seconds = t1 - t0;
   return rc;
 }
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 // Deterministic math exposed for accuracy tests against libm.
 void oracle_detmath(int fn, const float *x, const float *y, float *out,

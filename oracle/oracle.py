@@ -36,7 +36,8 @@ def lib():
         l.oracle_render.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
                                     _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                     C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
-                                    C.c_uint32, _F, _D, _U64]
+                                    C.c_uint32, _F, _D, _U64, _D]
+        l.oracle_max_threads.restype = C.c_int
         l.oracle_detmath.argtypes = [C.c_int, _F, _F, _F, C.c_uint32]
         l.oracle_detmath.restype = None
         _lib = l
@@ -95,11 +96,14 @@ def thread_infos(wds: L.PtWorkDivisionSettings, spp: int, x: int, y: int, max_th
 def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
            sobol: np.ndarray | None, precision: int = 32, threads: int = 0,
            pixel_begin: int = 0, pixel_end: int = 0):
-    """Returns (image (y, x, 3), counts[rays, nodes, tris])."""
+    """Returns (image (y, x, 3), info) with info = {"counts": [[rays, nodes,
+    tris] for continuation/camera rays, same for light rays], "seconds":
+    render time without scene setup}."""
     raw = settings.raw if hasattr(settings, "raw") else settings
     ftw = np.ascontiguousarray(film_to_world, np.float32)
     sob = np.zeros(1, np.float32) if sobol is None else np.ascontiguousarray(sobol, np.float32)
-    counts = np.zeros(3, np.uint64)
+    counts = np.zeros(6, np.uint64)
+    secs = C.c_double()
     out32 = np.zeros((y_dim, x_dim, 3), np.float32)
     out64 = np.zeros((y_dim, x_dim, 3), np.float64)
     rc = lib().oracle_render(
@@ -107,10 +111,15 @@ def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_
         flat["materials"], flat["n_materials"], _f(flat["light_tris"]), _f(flat["light_cdf"]),
         flat["light_tris"].shape[0], _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
         precision, threads, pixel_begin, pixel_end, _f(out32), out64.ctypes.data_as(_D),
-        counts.ctypes.data_as(_U64))
+        counts.ctypes.data_as(_U64), C.byref(secs))
     if rc != 0:
         raise ValueError("oracle render failed")
-    return (out64 if precision == 64 else out32), counts
+    info = {"counts": counts.reshape(2, 3), "seconds": secs.value}
+    return (out64 if precision == 64 else out32), info
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
 
 
 def detmath(fn: str, x, y=None):

@@ -94,7 +94,11 @@ class PtOutput(C.Structure):
 class PtRenderStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double),
                 ("rays", C.c_uint64), ("samples", C.c_uint64),
-                ("trace_launches", C.c_uint64)]
+                ("trace_launches", C.c_uint64), ("ext_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("ext_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("ext_launches", C.c_uint64),
+                ("shadow_launches", C.c_uint64), ("mega_ms", C.c_double),
+                ("mega_launches", C.c_uint64), ("profiled", C.c_uint32)]
 
 
 # Every exported symbol with its (restype, argtypes).
@@ -124,6 +128,7 @@ SIGNATURES = {
     "pt_work_division": (C.c_int, [C.POINTER(PtWorkDivisionSettings), C.c_uint32, C.c_uint32, C.c_uint32, _U]),
     "pt_scene_bvh": (C.c_int, [_P, _U, _F, _F, _U, _U, _U, _U, _U]),
     "pt_last_render_stats": (C.c_int, [_P, C.POINTER(PtRenderStats)]),
+    "pt_set_profiling": (C.c_int, [_P, C.c_int]),
 }
 
 _lib = None

@@ -354,6 +354,13 @@ int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
   PT_GUARD_END
 }
 
+int pt_set_profiling(pt_renderer *r, int enable) {
+  if (!r)
+    return fail(PT_ERR_INVALID, "null renderer");
+  r->profiling = enable != 0;
+  return PT_OK;
+}
+
 int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out) {
   if (!r || !out)
     return fail(PT_ERR_INVALID, "null argument");

@@ -74,6 +74,9 @@ pt_renderer::~pt_renderer() {
     (void)hipEventDestroy(ev_start);
   if (ev_stop)
     (void)hipEventDestroy(ev_stop);
+  for (hipEvent_t e : ev_prof)
+    if (e)
+      (void)hipEventDestroy(e);
   if (stream)
     (void)hipStreamDestroy(stream);
 }
@@ -87,6 +90,10 @@ bool pt_renderer::init(int dev, std::string *err) {
     e = hipEventCreate(&ev_start);
   if (e == hipSuccess)
     e = hipEventCreate(&ev_stop);
+  for (int i = 0; i < 3 && e == hipSuccess; ++i)
+    e = hipEventCreate(&ev_prof[i]);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&n_cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) {
     *err = std::string("HIP init failed: ") + hipGetErrorString(e);
     return false;
@@ -220,6 +227,8 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
     HIP_TRY(scratch_rays.upload(spec.rays, size_t(spec.n_rays) * 6, stream));
   HIP_TRY(hipStreamSynchronize(stream));
 
+  stats = pt_render_stats{};
+  stats.profiled = profiling ? 1u : 0u;
   auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipEventRecord(ev_start, stream));
   const float *final_rgb = nullptr;
@@ -245,7 +254,17 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
     uint32_t chunk = settings.max_blocks_per_launch_gpu ? settings.max_blocks_per_launch_gpu : total;
     for (uint32_t start = 0; start < total; start += chunk) {
       uint32_t end = start + chunk < total ? start + chunk : total;
+      if (profiling)
+        HIP_TRY(hipEventRecord(ev_prof[0], stream));
       HIP_TRY(gpu::launch_megakernel(p, start, end, stream));
+      if (profiling) {
+        HIP_TRY(hipEventRecord(ev_prof[1], stream));
+        HIP_TRY(hipEventSynchronize(ev_prof[1]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev_prof[0], ev_prof[1]));
+        stats.mega_ms += ms;
+      }
+      stats.mega_launches++;
     }
     final_rgb = rgb.ptr;
     if (wd.num_sample_blocks != 1) {
@@ -280,6 +299,7 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
   stats.kernel_ms = ms;
+  stats.trace_ms = stats.ext_ms + stats.shadow_ms;
   stats.rays = rays_traced;
   stats.samples = uint64_t(samples_per) * n_pixels;
   double secs = std::chrono::duration<double>(t1 - t0).count();

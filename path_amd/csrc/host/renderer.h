@@ -89,6 +89,8 @@ struct pt_renderer {
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   pt_render_stats stats{};
+  bool profiling = false;
+  hipEvent_t ev_prof[3] = {nullptr, nullptr, nullptr};
 
   // scene cache
   const pt_scene *cached_scene = nullptr;
@@ -107,6 +109,15 @@ struct pt_renderer {
   pt::host::DeviceBuf<float> scratch_rays;
   pt::host::DeviceBuf<float> scratch_t;
   pt::host::DeviceBuf<uint32_t> scratch_idx, scratch_back;
+
+  // wavefront integrator buffers
+  pt::host::DeviceBuf<uint4> wf_meta, wf_hit_ext, wf_hit_light;
+  pt::host::DeviceBuf<float4> wf_mult, wf_light, wf_total, wf_origin, wf_dir;
+  pt::host::DeviceBuf<float4> wf_q_ext, wf_q_shadow;
+  pt::host::DeviceBuf<uint32_t> wf_counters;
+  pt::host::DeviceBuf<unsigned long long> wf_sample_counter;
+  pt::host::DeviceBuf<float> wf_samples;
+  int n_cus = 256;
 
   ~pt_renderer();
   bool init(int device, std::string *err);

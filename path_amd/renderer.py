@@ -164,6 +164,15 @@ class RenderStats:
     rays: int
     samples: int
     trace_launches: int
+    ext_ms: float
+    shadow_ms: float
+    ext_rays: int
+    shadow_rays: int
+    ext_launches: int
+    shadow_launches: int
+    mega_ms: float
+    mega_launches: int
+    profiled: bool
 
 
 class Renderer:
@@ -223,7 +232,13 @@ class Renderer:
     def last_stats(self) -> RenderStats:
         s = L.PtRenderStats()
         L.check(L.lib().pt_last_render_stats(self.handle, C.byref(s)))
-        return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches)
+        return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches,
+                           s.ext_ms, s.shadow_ms, s.ext_rays, s.shadow_rays, s.ext_launches,
+                           s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled))
+
+    def set_profiling(self, enable: bool) -> None:
+        """Bracket every traversal / mega-kernel launch with HIP events."""
+        L.check(L.lib().pt_set_profiling(self.handle, int(enable)))
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -47,10 +47,41 @@ def test_megakernel_image_bit_exact(renderer, request, which, spp, size):
     img, _ = renderer.render(scene, ftw, size, size, spp, s)
     dirs = O.joe_kuo_directions(256)
     tab = O.sobol_table(dirs, 256, spp)
-    ref32, _ = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=32)
-    ref64, _ = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=64)
+    ref32, _info = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=32)
+    ref64, _info64 = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=64)
     diff = np.abs(img.astype(np.float64) - ref32)
     assert np.array_equal(img.view(np.uint32), ref32.view(np.uint32)), (
         f"max |diff| {diff.max()} at {np.unravel_index(diff.argmax(), diff.shape)}")
     # FP64 oracle on the same samples: per-pixel relative RMSE bound
     assert rel_rmse(img, ref64) < 2e-2
+
+
+@pytest.mark.parametrize("which,spp,size", [("cornell", 16, 64), ("cornell_sphere", 8, 48)])
+def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
+    """The wavefront integrator produces the mega-kernel's image bit for bit
+    (same paths, same per-pixel summation order)."""
+    scene, ftw = request.getfixturevalue(which)
+    mk, _ = renderer.render(scene, ftw, size, size, spp, _settings("MegaKernel"))
+    s = _settings("Streaming")
+    wf, _ = renderer.render(scene, ftw, size, size, spp, s)
+    assert np.array_equal(mk.view(np.uint32), wf.view(np.uint32))
+    # a small path pool forces many regenerations per slot
+    s.max_num_samples_per_launch = 1000
+    wf2, _ = renderer.render(scene, ftw, size, size, spp, s)
+    assert np.array_equal(mk.view(np.uint32), wf2.view(np.uint32))
+
+
+def test_streaming_default_settings_multiplier_func(renderer, cornell):
+    """Reference default settings (MultiplierFunc Russian roulette): wavefront
+    == mega-kernel == CPU checker."""
+    from oracle import oracle as O
+    from path_amd.renderer import Settings
+    scene, ftw = cornell
+    s = Settings()
+    mk, _ = renderer.render(scene, ftw, 40, 40, 16, s)
+    s.kernel_approach = "Streaming"
+    wf, _ = renderer.render(scene, ftw, 40, 40, 16, s)
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, 16)
+    ref, _ = O.render(scene.flat(), ftw, s, 16, 40, 40, tab, precision=32)
+    assert np.array_equal(mk.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(wf.view(np.uint32), ref.view(np.uint32))
