@@ -481,6 +481,59 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
+namespace {
+struct ArrayRng {
+  const double *v;
+  uint32_t n, used;
+  double next() { return used < n ? v[used++] : 0.0; }
+};
+Material<double> to_material(const pt_material &m) {
+  return make_material<double>(
+      m.type, V3<double>{m.diffuse[0], m.diffuse[1], m.diffuse[2]},
+      V3<double>{m.specular[0], m.specular[1], m.specular[2]}, m.shininess,
+      m.ior, m.diffuse_weight, m.glossy_weight,
+      V3<double>{m.emission[0], m.emission[1], m.emission[2]});
+}
+} // namespace
+
+// One BSDF direction sample in double precision, driven by the given uniform
+// values. out: dir[3], multiplier[3], is_discrete, values consumed.
+void oracle_bsdf_sample(const pt_material *mat, const double *in, const double *n,
+                        const double *u, uint32_t n_u, int dir_sampler, double *out) {
+  IntegratorSettings s{};
+  s.dir_sampler = uint32_t(dir_sampler);
+  ArrayRng r{u, n_u, 0};
+  DirSample<double> d = sample_direction(s, to_material(*mat), V3<double>{in[0], in[1], in[2]},
+                                         V3<double>{n[0], n[1], n[2]}, r);
+  out[0] = d.dir.x, out[1] = d.dir.y, out[2] = d.dir.z;
+  out[3] = d.multiplier.x, out[4] = d.multiplier.y, out[5] = d.multiplier.z;
+  out[6] = d.is_discrete ? 1.0 : 0.0;
+  out[7] = double(r.used);
+}
+
+// BSDF evaluation f(in, out) in double precision (continuous part) and the
+// probability of the continuous lobe.
+void oracle_bsdf_eval(const pt_material *mat, const double *in, const double *o,
+                      const double *n, double *out) {
+  Material<double> m = to_material(*mat);
+  V3<double> e = continuous_eval(m, V3<double>{in[0], in[1], in[2]},
+                                 V3<double>{o[0], o[1], o[2]}, V3<double>{n[0], n[1], n[2]});
+  out[0] = e.x, out[1] = e.y, out[2] = e.z;
+  out[3] = prob_continuous(m);
+}
+
+// Termination probability (double).
+double oracle_term_prob(const pt_settings *st, uint32_t iters, const double *mult) {
+  IntegratorSettings s{};
+  s.term_prob = st->term_prob;
+  s.n_iters = st->n_iters;
+  s.constant_prob = st->constant_prob;
+  s.mf_exp = st->multiplier_func_exp;
+  s.mf_min_prob = st->multiplier_func_min_prob;
+  s.normalize_min_prob = st->normalize_min_prob;
+  return term_prob(s, iters, V3<double>{mult[0], mult[1], mult[2]});
+}
+
 // Deterministic math exposed for accuracy tests against libm.
 void oracle_detmath(int fn, const float *x, const float *y, float *out,
                     uint32_t n) {

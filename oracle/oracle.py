@@ -38,6 +38,13 @@ def lib():
                                     C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
                                     C.c_uint32, _F, _D, _U64, _D]
         l.oracle_max_threads.restype = C.c_int
+        l.oracle_bsdf_sample.argtypes = [C.POINTER(L.PtMaterial), _D, _D, _D, C.c_uint32,
+                                         C.c_int, _D]
+        l.oracle_bsdf_sample.restype = None
+        l.oracle_bsdf_eval.argtypes = [C.POINTER(L.PtMaterial), _D, _D, _D, _D]
+        l.oracle_bsdf_eval.restype = None
+        l.oracle_term_prob.argtypes = [C.POINTER(L.PtSettings), C.c_uint32, _D]
+        l.oracle_term_prob.restype = C.c_double
         l.oracle_detmath.argtypes = [C.c_int, _F, _F, _F, C.c_uint32]
         l.oracle_detmath.restype = None
         _lib = l
@@ -139,3 +146,28 @@ def joe_kuo_directions(dim_bound: int) -> np.ndarray:
     if rr.rocrand_get_direction_vectors32(C.byref(ptr), 101) != 0:
         raise RuntimeError("rocrand direction numbers unavailable")
     return np.ctypeslib.as_array(ptr, shape=(dim_bound * 32,)).reshape(dim_bound, 32).copy()
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(_D)
+
+
+def bsdf_sample(mat, incoming, normal, u, dir_sampler=1):
+    """-> (dir, multiplier, is_discrete, n_values_consumed)."""
+    out = np.zeros(8)
+    u = np.ascontiguousarray(u, np.float64)
+    lib().oracle_bsdf_sample(C.byref(mat), _d(incoming), _d(normal), u.ctypes.data_as(_D),
+                             u.size, dir_sampler, out.ctypes.data_as(_D))
+    return out[:3], out[3:6], bool(out[6]), int(out[7])
+
+
+def bsdf_eval(mat, incoming, outgoing, normal):
+    out = np.zeros(4)
+    lib().oracle_bsdf_eval(C.byref(mat), _d(incoming), _d(outgoing), _d(normal),
+                           out.ctypes.data_as(_D))
+    return out[:3], out[3]
+
+
+def term_prob(settings, iters, mult):
+    raw = settings.raw if hasattr(settings, "raw") else settings
+    return float(lib().oracle_term_prob(C.byref(raw), iters, _d(mult)))

@@ -1,0 +1,136 @@
+"""Scene loader (CS123 XML + OBJ/MTL), MTL -> BSDF conversion, camera,
+YAML settings and the C ABI surface. CPU only."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from path_amd import _lib as L
+from path_amd.renderer import Scene, Settings, camera_from_look
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_cornell_diffuse_loads_like_the_reference():
+    sc, ftw = Scene.load(GOLDEN / "scenes" / "cornell-diffuse.xml", 1.0)
+    n_tris, n_mats, n_light = sc.counts()
+    # 18 faces (17 quads + 1 quad light), all triangulated into 2 triangles
+    assert n_tris == 36 and n_mats == 8 and n_light == 2
+    flat = sc.flat()
+    mats = flat["materials"]
+    light = [m for m in mats[:n_mats] if m.emission[0] > 0]
+    assert len(light) == 1 and list(light[0].emission) == [51, 36, 12]
+    assert all(m.type == L.BSDF["Diffuse"] for m in mats[:n_mats])
+    # the light quad (-0.24..0.23 x -0.22..0.16 at y=1.98): two equal-area triangles
+    assert np.allclose(flat["light_cdf"], [0.5, 1.0])
+    # quad (0,1,2,3) -> (0,1,2), (0,2,3): first floor triangle's vertices
+    t = flat["tris"][0]
+    v0, v1, v2 = t[:3], t[:3] + t[3:6], t[:3] + t[6:9]
+    assert np.allclose([v0, v1, v2], [[-1.01, 0, 0.99], [1.0, 0, 0.99], [1.0, 0, -1.04]], atol=1e-6)
+    # face normals for OBJ files without vn; floor normal +y
+    assert np.allclose(flat["normals"][0, :3], [0, 1, 0], atol=1e-6)
+
+
+def test_glass_sphere_materials():
+    sc, _ = Scene.load(GOLDEN / "scenes" / "cornell-glass-sphere.xml", 1.0)
+    flat = sc.flat()
+    types = {m.type for m in flat["materials"][:flat["n_materials"]]}
+    # leftSphere Ks 0.95 Ns 1000 -> Mirror; rightSphere illum 5 Ni 2.5 -> glass
+    assert L.BSDF["Mirror"] in types and L.BSDF["DielectricRefractive"] in types
+    glass = [m for m in flat["materials"][:flat["n_materials"]]
+             if m.type == L.BSDF["DielectricRefractive"]][0]
+    assert abs(glass.ior - 2.5) < 1e-6 and list(glass.specular) == [1, 1, 1]
+
+
+def test_glossy_floor_material():
+    sc, _ = Scene.load(GOLDEN / "scenes" / "cornell-glossy.xml", 1.0)
+    flat = sc.flat()
+    glossy = [m for m in flat["materials"][:flat["n_materials"]] if m.type == L.BSDF["Glossy"]]
+    assert len(glossy) == 1 and glossy[0].shininess == 40
+
+
+def test_camera_transform_closed_form():
+    # pos (0,1,3.6) looking at (0,1,0), 45 degree height angle, square image
+    ftw = camera_from_look([0, 0, -3.6], [0, 1, 0], [0, 1, 3.6], 45.0, 1.0).reshape(3, 4)
+    assert np.allclose(ftw[:, 3], [0, 1, 3.6])
+    t = np.tan(np.pi * 45 / 360)
+    # film point (1, 1, -1) maps to pos + (t, t, -1) in world space
+    p = ftw[:, :3] @ np.array([1, 1, -1]) + ftw[:, 3]
+    assert np.allclose(p, [t, 1 + t, 2.6], atol=1e-6)
+
+
+def test_missing_scene_is_an_error():
+    with pytest.raises(L.PathError) as e:
+        Scene.load("/nonexistent/scene.xml")
+    assert e.value.code == L.PT_ERR_IO
+
+
+def test_unsupported_illum_is_an_error(tmp_path):
+    (tmp_path / "m.mtl").write_text("newmtl a\n  illum 7\n  Kd 0.5 0.5 0.5\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\n")
+    sc = Scene()
+    with pytest.raises(L.PathError) as e:
+        sc.add_obj(tmp_path / "m.obj")
+    assert "illum" in str(e.value)
+
+
+def test_ear_clipping_of_a_concave_polygon(tmp_path):
+    (tmp_path / "m.mtl").write_text("newmtl a\n  illum 2\n  Kd 0.5 0.5 0.5\n")
+    # L-shaped hexagon in the z=0 plane
+    pts = [(0, 0), (2, 0), (2, 1), (1, 1), (1, 2), (0, 2)]
+    obj = "mtllib m.mtl\n" + "".join(f"v {x} {y} 0\n" for x, y in pts) + "usemtl a\nf 1 2 3 4 5 6\n"
+    (tmp_path / "m.obj").write_text(obj)
+    sc = Scene()
+    sc.add_obj(tmp_path / "m.obj")
+    flat = sc.flat()
+    assert flat["tris"].shape[0] == 4
+    area = sum(0.5 * abs(np.cross(t[3:6], t[6:9])[2]) for t in flat["tris"])
+    assert abs(area - 3.0) < 1e-6
+
+
+def test_yaml_reference_config():
+    s = Settings.from_yaml(GOLDEN / "configs" / "benchmark.yaml")
+    assert s.kernel_approach == L.KERNEL_APPROACH["MegaKernel"]
+    assert s.max_blocks_per_launch_gpu == 4096 and s.max_blocks_per_launch_cpu == 128
+    assert s.reduce_work_division.target_x_block_size == 256
+    assert s.accel == L.ACCEL["SBVH"] and s.target_objects == 4
+    assert s.light_sampler == L.LIGHT_SAMPLER["RandomTriangle"]
+    assert s.binary_search_threshold == 1073741823
+    assert s.term_prob == L.TERM_PROB["MultiplierFunc"] and s.multiplier_func_exp == 50.0
+    assert s.rng == L.RNG["Uniform"] and s.max_sample_size == 256
+    assert s.back_cull_emission == 1
+
+
+def test_yaml_round_trip(tmp_path):
+    s = Settings()
+    s.kernel_approach = "Streaming"
+    s.term_prob = "NIters"
+    s.n_iters = 7
+    s.dir_sampler = "Uniform"
+    s.max_num_samples_per_launch = 12345
+    p = tmp_path / "c.yaml"
+    p.write_text(s.to_yaml())
+    t = Settings.from_yaml(p)
+    for f, _ in L.PtSettings._fields_:
+        a, b = getattr(s.raw, f), getattr(t.raw, f)
+        if isinstance(a, L.PtWorkDivisionSettings):
+            a, b = bytes(a), bytes(b)
+        assert a == b, f
+
+
+def test_yaml_unknown_enum_is_an_error(tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("settings:\n  term_prob:\n    type: Bogus\n    value: {}\n")
+    with pytest.raises(L.PathError):
+        Settings.from_yaml(p)
+
+
+def test_library_exports_every_declared_entry_point():
+    header = (Path(__file__).resolve().parent.parent / "include" / "path_amd.h").read_text()
+    declared = set(re.findall(r"^(?:int|void|const char \*)\s*(pt_\w+)\(", header, re.M))
+    assert len(declared) >= 20
+    lib = L.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared <= set(L.SIGNATURES)
