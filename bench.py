@@ -99,6 +99,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--integrator", choices=["Streaming", "MegaKernel"], default=None,
+                    help="override the configuration's integrator")
+    ap.add_argument("--max-paths", type=int, default=0,
+                    help="wavefront path-pool size (max_num_samples_per_launch)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU only: the path shards by sample range with a single "
@@ -106,10 +110,14 @@ def main(argv=None):
         return 2
 
     from path_amd.renderer import Renderer
-    cfg = CONFIGS[a.config]
+    cfg = dict(CONFIGS[a.config])
+    if a.integrator:
+        cfg["approach"] = a.integrator
     with tempfile.TemporaryDirectory() as d:
         scene, ftw = build_scene(cfg, d)
     settings = make_settings(cfg)
+    if a.max_paths:
+        settings.max_num_samples_per_launch = a.max_paths
     n_tris = scene.counts()[0]
     r = Renderer(0)
     size, spp = cfg["size"], cfg["spp"]

@@ -43,16 +43,15 @@ namespace {
 
 template <class T> struct OScene {
   std::vector<TriGeom<T>> tris;
-  std::vector<V3<T>> normals;
-  std::vector<uint32_t> tri_mat;
+  std::vector<ShadeTri<T>> shade;
   std::vector<Material<T>> mats;
   std::vector<TriGeom<T>> light_tris;
   std::vector<T> light_cdf;
   host::Bvh bvh;
   std::vector<V3<T>> lo, hi; // padded node boxes
   SceneView<T> view() const {
-    return {tris.data(), normals.data(), tri_mat.data(), mats.data(),
-            light_tris.data(), light_cdf.data(), uint32_t(light_tris.size())};
+    return {shade.data(), mats.data(), light_tris.data(), light_cdf.data(),
+            uint32_t(light_tris.size())};
   }
 };
 
@@ -193,30 +192,30 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
       rng = {make_hash_rng(sample + location * a.samples_per)};
     T xo = rng.next();
     T yo = rng.next();
-    Ray<T> rays_[2];
-    rays_[0] = camera_ray(a.film_to_world, T(float(info.x)) + xo,
-                          T(float(info.y)) + yo, a.wd.x_dim, a.wd.y_dim);
-    uint32_t n_rays = 1;
+    Ray<T> next_ray = camera_ray(a.film_to_world, T(float(info.x)) + xo,
+                                 T(float(info.y)) + yo, a.wd.x_dim, a.wd.y_dim);
+    Ray<T> light_ray = next_ray;
     PathState<T> st = initial_path_state<T>();
     for (;;) {
-      Hit<T> hits[2];
-      for (uint32_t i = 0; i < n_rays; ++i) {
-        int kind = i < st.n_light ? 1 : 0;
-        hits[i] = bvh_closest(*a.scene, rays_[i], &cnt->c[kind]);
-        cnt->rays[kind] += 1;
+      Hit<T> light_hit{T(0), kNoHit, 0}, next_hit{T(0), kNoHit, 0};
+      if (st.n_light) {
+        light_hit = bvh_closest(*a.scene, light_ray, &cnt->c[1]);
+        cnt->rays[1] += 1;
+      }
+      if (st.has_next) {
+        next_hit = bvh_closest(*a.scene, next_ray, &cnt->c[0]);
+        cnt->rays[0] += 1;
       }
       PathState<T> ns;
-      Ray<T> nr[2];
-      uint32_t n_new;
-      if (rendering_equation_iteration(st, rng, rays_[n_rays - 1], hits,
-                                       a.settings, view, &ns, nr, &n_new)) {
+      Ray<T> nl = light_ray, nn = next_ray;
+      if (rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
+                                       a.settings, view, &ns, &nl, &nn)) {
         total += ns.total;
         break;
       }
       st = ns;
-      rays_[0] = nr[0];
-      rays_[1] = nr[1];
-      n_rays = n_new;
+      light_ray = nl;
+      next_ray = nn;
     }
   }
   return total;
@@ -301,13 +300,14 @@ void fill_scene(OScene<T> &s, const float *tris, const float *normals,
                 uint32_t n_mats, const float *light_tris, const float *light_cdf,
                 uint32_t n_lights) {
   s.tris.resize(n_tris);
-  s.normals.resize(size_t(n_tris) * 3);
-  s.tri_mat.assign(tri_mat, tri_mat + n_tris);
+  s.shade.resize(n_tris);
   for (uint32_t i = 0; i < n_tris; ++i) {
     s.tris[i] = load_tri<T>(tris + 9 * i);
+    V3<T> n[3];
     for (int k = 0; k < 3; ++k)
-      s.normals[3 * i + k] = {T(normals[9 * i + 3 * k]), T(normals[9 * i + 3 * k + 1]),
-                              T(normals[9 * i + 3 * k + 2])};
+      n[k] = {T(normals[9 * i + 3 * k]), T(normals[9 * i + 3 * k + 1]),
+              T(normals[9 * i + 3 * k + 2])};
+    s.shade[i] = make_shade_tri(s.tris[i], n, tri_mat[i]);
   }
   for (uint32_t i = 0; i < n_mats; ++i) {
     const pt_material &m = mats[i];

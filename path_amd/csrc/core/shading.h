@@ -317,10 +317,40 @@ PT_INLINE T term_prob(const IntegratorSettings &s, uint32_t iters,
 // Scene view used during shading. Arrays are indexed by the flattened
 // (world-space) triangle index carried in Hit::idx.
 // ---------------------------------------------------------------------------
+
+// Everything shading needs about one triangle in one record (six 16-byte
+// vectors for float), so a hit costs a single dependent fetch.
+template <class T> struct ShadeTri {
+  V3<T> v0;
+  uint32_t material;
+  V3<T> e1;
+  uint32_t pad0;
+  V3<T> e2;
+  uint32_t pad1;
+  V3<T> n0;
+  T pad2;
+  V3<T> n1;
+  T pad3;
+  V3<T> n2;
+  T pad4;
+};
+
+template <class T>
+PT_HD inline ShadeTri<T> make_shade_tri(const TriGeom<T> &g, const V3<T> *n,
+                                        uint32_t material) {
+  ShadeTri<T> r{};
+  r.v0 = g.v0;
+  r.e1 = g.e1;
+  r.e2 = g.e2;
+  r.material = material;
+  r.n0 = n[0];
+  r.n1 = n[1];
+  r.n2 = n[2];
+  return r;
+}
+
 template <class T> struct SceneView {
-  const TriGeom<T> *tris;      // [n_tris]
-  const V3<T> *normals;        // [3 * n_tris] per-vertex shading normals
-  const uint32_t *tri_mat;     // [n_tris] material index
+  const ShadeTri<T> *tris; // [n_tris]
   const Material<T> *materials;
   const TriGeom<T> *light_tris; // emissive triangles, world space
   const T *light_cdf;           // inclusive cumulative weights, last = 1
@@ -407,16 +437,18 @@ template <class T> PT_INLINE bool within_target(T target, T t) {
   return abs_(target - t) <= T(1e-6f);
 }
 
-// One vertex of the rendering equation. `hits` holds the closest hits of the
-// rays emitted by the previous step (light ray first, then the continuation),
-// with Hit::idx == kNoHit for a miss. Returns true when the path is finished
-// (radiance in out->total). Otherwise writes the new state and up to two rays
-// (light ray first).
+// One vertex of the rendering equation. `light_hit` / `next_hit` are the
+// closest hits of the light ray (when st.n_light) and of the continuation ray
+// (when st.has_next) emitted by the previous step, with Hit::idx == kNoHit for
+// a miss. Returns true when the path is finished (radiance in out->total).
+// Otherwise writes the new state; out->n_light / out->has_next say which of
+// *light_ray / *next_ray were emitted (both start at the new vertex).
 template <class T, class R>
 PT_INLINE bool rendering_equation_iteration(
-    const PathState<T> &st, R &rng, const Ray<T> &last_ray, const Hit<T> *hits,
+    const PathState<T> &st, R &rng, const Ray<T> &last_ray,
+    const Hit<T> &light_hit, const Hit<T> &next_hit,
     const IntegratorSettings &s, const SceneView<T> &sc, PathState<T> *out,
-    Ray<T> *new_rays, uint32_t *n_new_rays) {
+    Ray<T> *light_ray, Ray<T> *next_ray) {
   PathState<T> ns;
   ns.iters = st.iters + 1;
   ns.total = st.total;
@@ -428,24 +460,19 @@ PT_INLINE bool rendering_equation_iteration(
   ns.light_target = T(0);
   ns.count_emission = 0;
   ns.has_next = 0;
-  uint32_t n_new = 0;
-  *n_new_rays = 0;
 
-  for (uint32_t i = 0; i < st.n_light; ++i) {
-    const Hit<T> &h = hits[i];
-    if (h.idx == kNoHit || !within_target(st.light_target, h.t))
-      continue;
-    if (s.back_cull_emission && h.back)
-      continue;
-    ns.total += sc.materials[sc.tri_mat[h.idx]].emission * st.light_mult;
-  }
+  if (st.n_light && light_hit.idx != kNoHit &&
+      within_target(st.light_target, light_hit.t) &&
+      !(s.back_cull_emission && light_hit.back))
+    ns.total += sc.materials[sc.tris[light_hit.idx].material].emission *
+                st.light_mult;
 
   if (!st.has_next) {
     *out = ns;
     return true;
   }
 
-  const Hit<T> &h = hits[st.n_light];
+  const Hit<T> &h = next_hit;
   if (h.idx == kNoHit ||
       (st.ray_has_target && !within_target(st.ray_target, h.t))) {
     *out = ns;
@@ -454,16 +481,16 @@ PT_INLINE bool rendering_equation_iteration(
 
   V3<T> p = hit_point(last_ray, h.t);
   V3<T> mult = st.ray_mult;
-  const Material<T> mat = sc.materials[sc.tri_mat[h.idx]];
+  const ShadeTri<T> tri = sc.tris[h.idx];
+  const Material<T> mat = sc.materials[tri.material];
   bool performs_samples = s.light_sampler != kLightNone;
   if ((!performs_samples || st.count_emission) &&
       !(s.back_cull_emission && h.back))
     ns.total += mult * mat.emission;
 
-  const TriGeom<T> tri = sc.tris[h.idx];
-  V3<T> vn[3] = {sc.normals[3 * h.idx], sc.normals[3 * h.idx + 1],
-                 sc.normals[3 * h.idx + 2]};
-  V3<T> normal = shading_normal(tri, vn, p);
+  const TriGeom<T> g = {tri.v0, tri.e1, tri.e2};
+  const V3<T> vn[3] = {tri.n0, tri.n1, tri.n2};
+  V3<T> normal = shading_normal(g, vn, p);
   V3<T> in = last_ray.d;
 
   if (performs_samples) {
@@ -476,7 +503,7 @@ PT_INLINE bool rendering_equation_iteration(
             ls.multiplier;
         ns.light_target = ls.target_distance;
         ns.n_light = 1;
-        new_rays[n_new++] = {p, ls.dir};
+        *light_ray = {p, ls.dir};
       }
     }
   }
@@ -490,13 +517,12 @@ PT_INLINE bool rendering_equation_iteration(
     T denom = T(1) - tp;
     nm = {nm.x / denom, nm.y / denom, nm.z / denom};
     ns.ray_mult = nm;
-    new_rays[n_new++] = {p, ds.dir};
-  } else if (n_new == 0) {
+    *next_ray = {p, ds.dir};
+  } else if (!ns.n_light) {
     *out = ns;
     return true;
   }
   *out = ns;
-  *n_new_rays = n_new;
   return false;
 }
 

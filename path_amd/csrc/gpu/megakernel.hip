@@ -59,9 +59,10 @@ __global__ __launch_bounds__(256) void k_intersect(DeviceScene sc,
   const float *r = rays + 6 * size_t(i);
   Ray<float> ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
   Hit<float> h = {0.f, kNoHit, 0};
-  if (sc.n_nodes != 0)
-    h = trace_closest<false>(sc.nodes, sc.tris, ray, 3.4028235e38f, -1.0f,
-                             nullptr);
+  if (sc.n_nodes != 0) {
+    PrivateStack stack;
+    h = trace_closest(sc.nodes, sc.tris, ray, 3.4028235e38f, -1.0f, stack);
+  }
   t[i] = h.t;
   idx[i] = h.idx;
   back[i] = h.back;
@@ -98,42 +99,45 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
     } else {
       rng = make_hash_rng(sample + location * p.rng.samples_per);
     }
-    Ray<float> rays[2];
+    Ray<float> next_ray, light_ray;
     if (p.initial_rays) {
       const float *r = p.initial_rays + 6 * size_t(info.x);
-      rays[0] = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+      next_ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
     } else {
       float xo = rng.next();
       float yo = rng.next();
-      rays[0] = camera_ray(p.film_to_world, float(info.x) + xo,
-                           float(info.y) + yo, p.wd.x_dim, p.wd.y_dim);
+      next_ray = camera_ray(p.film_to_world, float(info.x) + xo,
+                            float(info.y) + yo, p.wd.x_dim, p.wd.y_dim);
     }
-    uint32_t n_rays = 1;
+    light_ray = next_ray;
     PathState<float> st = initial_path_state<float>();
     for (;;) {
-      Hit<float> hits[2];
-      for (uint32_t i = 0; i < n_rays; ++i) {
-        bool light = i < st.n_light;
-        float limit = light ? st.light_target + 2e-6f : 3.4028235e38f;
-        float occl = light ? st.light_target - 2e-6f : -1.0f;
-        hits[i] = trace_closest<false>(p.scene.nodes, p.scene.tris, rays[i],
-                                       limit, occl, nullptr);
+      Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
+      if (st.n_light) {
+        PrivateStack stack;
+        light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
+                                  st.light_target + 2e-6f,
+                                  st.light_target - 2e-6f, stack);
+        ++n_rays_traced;
       }
-      n_rays_traced += n_rays;
+      if (st.has_next) {
+        PrivateStack stack;
+        next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
+                                 3.4028235e38f, -1.0f, stack);
+        ++n_rays_traced;
+      }
       PathState<float> ns;
-      Ray<float> new_rays[2];
-      uint32_t n_new;
-      bool done = rendering_equation_iteration(
-          st, rng, rays[n_rays - 1], hits, p.settings, p.scene.view, &ns,
-          new_rays, &n_new);
+      Ray<float> nl = light_ray, nn = next_ray;
+      bool done = rendering_equation_iteration(st, rng, next_ray, light_hit,
+                                               next_hit, p.settings,
+                                               p.scene.view, &ns, &nl, &nn);
       if (done) {
         total += ns.total;
         break;
       }
       st = ns;
-      rays[0] = new_rays[0];
-      rays[1] = new_rays[1];
-      n_rays = n_new;
+      light_ray = nl;
+      next_ray = nn;
     }
   }
   *rays_traced = n_rays_traced;
@@ -162,7 +166,7 @@ __global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
   total.z = group_reduce_sum(total.z, group);
   bool writer = (threadIdx.x % sbs) == 0 && !info.exit;
   if (sbs > 64u) {
-    __shared__ float partial[4][3];
+    __shared__ float partial[16][3];
     uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0) {
       partial[w][0] = total.x;
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
     __syncthreads();
     if (writer) {
       uint32_t ng = sbs / 64u, w0 = w;
-      float v[4][3];
+      float v[16][3];
       for (uint32_t g = 0; g < ng; ++g)
         for (int c = 0; c < 3; ++c)
           v[g][c] = partial[w0 + g][c];

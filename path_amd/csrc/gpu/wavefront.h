@@ -25,13 +25,15 @@ struct WfState {
   float4 *total;    // accumulated radiance xyz
   float4 *origin;   // origin of the pending rays (last vertex / camera)
   float4 *dir;      // direction of the pending continuation ray
-  uint4 *hit_ext;   // (t bits, idx, back) of the continuation ray
-  uint4 *hit_light; // (t bits, idx, back) of the light ray
+  float4 *light_dir; // direction of the pending light ray, target distance
+  uint2 *hit_ext;   // packed closest hit of the continuation ray
+  uint2 *hit_light; // packed closest hit of the light ray
 };
 
+// Ray queues hold slot indices; the rays themselves are read from the state.
 struct WfQueues {
-  float4 *ext;    // 2 float4 per entry: (o.xyz, slot bits), (d.xyz, 0)
-  float4 *shadow; // 2 float4 per entry: (o.xyz, slot bits), (d.xyz, target)
+  uint32_t *ext;
+  uint32_t *shadow;
 };
 
 struct WfParams {
@@ -48,6 +50,8 @@ struct WfParams {
   unsigned long long *sample_counter; // next sample to start
   unsigned long long *ray_counter;    // rays traced
   float *out_samples;                 // [location * spp + sample] * 3
+  int *stack_overflow;                // traversal stack spill area
+  uint32_t overflow_stride;           // threads of a traversal launch
 };
 
 hipError_t launch_wf_reset(uint32_t *counters, uint32_t mask, hipStream_t s);

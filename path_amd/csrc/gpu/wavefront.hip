@@ -111,22 +111,17 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     st.light_target = m.w;
     st.light_mult = {lm.x, lm.y, lm.z};
     st.total = {tot.x, tot.y, tot.z};
-    Hit<float> hits[2];
-    uint32_t k = 0;
-    if (st.n_light) {
-      uint4 h = p.st.hit_light[slot];
-      hits[k++] = {__uint_as_float(h.x), h.y, h.z};
-    }
-    if (st.has_next) {
-      uint4 h = p.st.hit_ext[slot];
-      hits[k++] = {__uint_as_float(h.x), h.y, h.z};
-    }
+    Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
+    if (st.n_light)
+      light_hit = unpack_hit(p.st.hit_light[slot]);
+    if (st.has_next)
+      next_hit = unpack_hit(p.st.hit_ext[slot]);
     Ray<float> last = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
     R rng = make_rng<R>(p.rng, location, sample, counter);
-    Ray<float> nr[2];
-    uint32_t n_new = 0;
-    finished = rendering_equation_iteration(st, rng, last, hits, p.settings,
-                                            p.scene.view, &ns, nr, &n_new);
+    Ray<float> nl = last, nn = last;
+    finished = rendering_equation_iteration(st, rng, last, light_hit, next_hit,
+                                            p.settings, p.scene.view, &ns, &nl,
+                                            &nn);
     counter = rng_counter(rng);
     if (finished) {
       float *out = p.out_samples +
@@ -135,35 +130,44 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
       out[1] = ns.total.y;
       out[2] = ns.total.z;
     } else {
-      uint32_t r = 0;
       if (ns.n_light) {
         push_light = true;
-        light_d = nr[r].d;
+        light_d = nl.d;
         light_target = ns.light_target;
-        ext_o = nr[r].o;
-        ++r;
+        ext_o = nl.o;
       }
       if (ns.has_next) {
         push_ext = true;
-        ext_o = nr[r].o;
-        ext_d = nr[r].d;
+        ext_o = nn.o;
+        ext_d = nn.d;
       }
     }
   }
 
   // ---- start a new sample in an empty slot ----
+  // Block-aggregated counters: one atomic per block for the sample counter and
+  // one per queue (wave ballots + an LDS prefix over the block's 4 waves).
+  __shared__ uint32_t s_counts[3][4];
+  __shared__ unsigned long long s_sample_base;
+  __shared__ uint32_t s_queue_base[2];
+  const uint32_t wave = threadIdx.x >> 6;
   bool want_new = in_range && (!active || finished);
   uint32_t n_want;
   uint32_t prefix = ballot_prefix(want_new, &n_want);
-  unsigned long long base = 0;
-  if (lane_id() == 0 && n_want)
-    base = atomicAdd(p.sample_counter, (unsigned long long)n_want);
-  base = (unsigned long long)__shfl((long long)base, 0, 64);
-  bool fresh = false;
+  if (lane_id() == 0)
+    s_counts[0][wave] = n_want;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = s_counts[0][0] + s_counts[0][1] + s_counts[0][2] + s_counts[0][3];
+    s_sample_base = tot ? atomicAdd(p.sample_counter, (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
   if (want_new) {
-    unsigned long long s = base + prefix;
+    unsigned long long before = 0;
+    for (uint32_t w = 0; w < wave; ++w)
+      before += s_counts[0][w];
+    unsigned long long s = s_sample_base + before + prefix;
     if (s < p.total_samples) {
-      fresh = true;
       location = uint32_t(s / p.rng.samples_per);
       sample = uint32_t(s % p.rng.samples_per);
       R rng = make_rng<R>(p.rng, location, sample, rng_start<R>(p.rng, location));
@@ -195,37 +199,40 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     p.st.total[slot] = make_float4(ns.total.x, ns.total.y, ns.total.z, 0.f);
     p.st.origin[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.f);
     p.st.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
+    if (push_light)
+      p.st.light_dir[slot] = make_float4(light_d.x, light_d.y, light_d.z, light_target);
   } else if (in_range && (active || want_new)) {
     p.st.meta[slot] = make_uint4(0, 0, 0, 0); // empty slot
   }
-  (void)fresh;
 
-  // ---- append rays to the queues ----
-  uint32_t qe = queue_append(&p.counters[kExtCount], push_ext);
-  uint32_t qs = queue_append(&p.counters[kShadowCount], push_light);
-  if (push_ext) {
-    p.q.ext[2 * size_t(qe)] = make_float4(ext_o.x, ext_o.y, ext_o.z, __uint_as_float(slot));
-    p.q.ext[2 * size_t(qe) + 1] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
-  }
-  if (push_light) {
-    p.q.shadow[2 * size_t(qs)] =
-        make_float4(ext_o.x, ext_o.y, ext_o.z, __uint_as_float(slot));
-    p.q.shadow[2 * size_t(qs) + 1] =
-        make_float4(light_d.x, light_d.y, light_d.z, light_target);
-  }
-  uint32_t n_active;
-  (void)ballot_prefix(holds_path, &n_active);
+  // ---- append rays to the queues (order within the block: wave, lane) ----
   uint32_t n_e, n_s;
-  (void)ballot_prefix(push_ext, &n_e);
-  (void)ballot_prefix(push_light, &n_s);
+  uint32_t pe = ballot_prefix(push_ext, &n_e);
+  uint32_t ps = ballot_prefix(push_light, &n_s);
   if (lane_id() == 0) {
-    if (n_active)
-      atomicAdd(&p.counters[kActive], n_active);
-    if (n_e + n_s)
-      atomicAdd(p.ray_counter, (unsigned long long)(n_e + n_s));
+    s_counts[1][wave] = n_e;
+    s_counts[2][wave] = n_s;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t te = s_counts[1][0] + s_counts[1][1] + s_counts[1][2] + s_counts[1][3];
+    uint32_t ts = s_counts[2][0] + s_counts[2][1] + s_counts[2][2] + s_counts[2][3];
+    s_queue_base[0] = te ? atomicAdd(&p.counters[kExtCount], te) : 0u;
+    s_queue_base[1] = ts ? atomicAdd(&p.counters[kShadowCount], ts) : 0u;
+  }
+  __syncthreads();
+  uint32_t be = s_queue_base[0], bs = s_queue_base[1];
+  for (uint32_t w = 0; w < wave; ++w) {
+    be += s_counts[1][w];
+    bs += s_counts[2][w];
+  }
+  if (push_ext)
+    p.q.ext[be + pe] = slot;
+  if (push_light)
+    p.q.shadow[bs + ps] = slot;
 }
 
+// Requires blockDim.x == 256 (block-level counter aggregation).
 __global__ __launch_bounds__(256) void k_wf_shade_sobol(WfParams p) {
   wf_shade<TableRng>(p);
 }
@@ -248,10 +255,12 @@ hipError_t launch_wf_shade(const WfParams &p, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <bool kShadow>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
+  __shared__ int lds_stack[kLdsStack * 256];
   const uint32_t count = p.counters[kShadow ? kShadowCount : kExtCount];
   uint32_t *head = &p.counters[kShadow ? kShadowHead : kExtHead];
-  const float4 *q = kShadow ? p.q.shadow : p.q.ext;
-  uint4 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
+  const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
+  uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   for (;;) {
     uint32_t base = 0;
     if (lane_id() == 0)
@@ -261,14 +270,17 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
       break;
     uint32_t i = base + lane_id();
     if (i < count) {
-      float4 a = q[2 * size_t(i)];
-      float4 b = q[2 * size_t(i) + 1];
-      Ray<float> ray = {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
-      float limit = kShadow ? b.w + 2e-6f : 3.4028235e38f;
-      float occl = kShadow ? b.w - 2e-6f : -1.0f;
-      Hit<float> h = trace_closest<false>(p.scene.nodes, p.scene.tris, ray,
-                                          limit, occl, nullptr);
-      hits[__float_as_uint(a.w)] = make_uint4(__float_as_uint(h.t), h.idx, h.back, 0);
+      uint32_t slot = q[i];
+      float4 o = p.st.origin[slot];
+      float4 d = kShadow ? p.st.light_dir[slot] : p.st.dir[slot];
+      Ray<float> ray = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
+      float limit = kShadow ? d.w + 2e-6f : 3.4028235e38f;
+      float occl = kShadow ? d.w - 2e-6f : -1.0f;
+      LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid,
+                     p.overflow_stride};
+      Hit<float> h = trace_closest(p.scene.nodes, p.scene.tris, ray, limit,
+                                   occl, stack);
+      hits[slot] = pack_hit(h);
     }
   }
 }

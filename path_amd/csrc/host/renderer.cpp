@@ -42,12 +42,13 @@ hipError_t DeviceSceneStore::upload(const FlatScene &f, const Bvh &bvh,
   std::vector<Material<float>> mats;
   for (const auto &m : f.materials)
     mats.push_back(m.to<float>());
+  std::vector<ShadeTri<float>> st(f.tris.size());
+  for (size_t i = 0; i < f.tris.size(); ++i)
+    st[i] = make_shade_tri(f.tris[i], &f.normals[3 * i], f.tri_mat[i]);
   hipError_t e;
   if ((e = nodes.upload(g.nodes.data(), g.nodes.size(), s)) != hipSuccess ||
       (e = tris.upload(g.tris.data(), g.tris.size(), s)) != hipSuccess ||
-      (e = shade_tris.upload(f.tris.data(), f.tris.size(), s)) != hipSuccess ||
-      (e = normals.upload(f.normals.data(), f.normals.size(), s)) != hipSuccess ||
-      (e = tri_mat.upload(f.tri_mat.data(), f.tri_mat.size(), s)) != hipSuccess ||
+      (e = shade_tris.upload(st.data(), st.size(), s)) != hipSuccess ||
       (e = materials.upload(mats.data(), mats.size(), s)) != hipSuccess ||
       (e = light_tris.upload(f.light_tris.data(), f.light_tris.size(), s)) !=
           hipSuccess ||
@@ -59,8 +60,6 @@ hipError_t DeviceSceneStore::upload(const FlatScene &f, const Bvh &bvh,
   dev.n_nodes = uint32_t(g.nodes.size());
   dev.n_tris = uint32_t(g.tris.size());
   dev.view.tris = shade_tris.ptr;
-  dev.view.normals = normals.ptr;
-  dev.view.tri_mat = tri_mat.ptr;
   dev.view.materials = materials.ptr;
   dev.view.light_tris = light_tris.ptr;
   dev.view.light_cdf = light_cdf.ptr;
@@ -275,8 +274,9 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
       final_rgb = rgb2.ptr;
     }
   } else {
+    uint64_t wf_rays = 0;
     rc = render_streaming(*this, spec, scene_store.dev, samples_per, settings,
-                          rng, x_dim, y_dim, &final_rgb);
+                          rng, x_dim, y_dim, &final_rgb, &wf_rays);
     if (rc != PT_OK)
       return rc;
   }
@@ -300,7 +300,9 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
   stats.kernel_ms = ms;
   stats.trace_ms = stats.ext_ms + stats.shadow_ms;
-  stats.rays = rays_traced;
+  stats.rays = settings.kernel_approach == PT_STREAMING
+                   ? stats.ext_rays + stats.shadow_rays
+                   : rays_traced;
   stats.samples = uint64_t(samples_per) * n_pixels;
   double secs = std::chrono::duration<double>(t1 - t0).count();
   if (seconds)

@@ -69,9 +69,7 @@ template <class T> struct DeviceBuf {
 struct DeviceSceneStore {
   DeviceBuf<GpuNode> nodes;
   DeviceBuf<GpuTri> tris;
-  DeviceBuf<TriGeom<float>> shade_tris;
-  DeviceBuf<V3<float>> normals;
-  DeviceBuf<uint32_t> tri_mat;
+  DeviceBuf<ShadeTri<float>> shade_tris;
   DeviceBuf<Material<float>> materials;
   DeviceBuf<TriGeom<float>> light_tris;
   DeviceBuf<float> light_cdf;
@@ -111,9 +109,11 @@ struct pt_renderer {
   pt::host::DeviceBuf<uint32_t> scratch_idx, scratch_back;
 
   // wavefront integrator buffers
-  pt::host::DeviceBuf<uint4> wf_meta, wf_hit_ext, wf_hit_light;
-  pt::host::DeviceBuf<float4> wf_mult, wf_light, wf_total, wf_origin, wf_dir;
-  pt::host::DeviceBuf<float4> wf_q_ext, wf_q_shadow;
+  pt::host::DeviceBuf<uint4> wf_meta;
+  pt::host::DeviceBuf<uint2> wf_hit_ext, wf_hit_light;
+  pt::host::DeviceBuf<float4> wf_mult, wf_light, wf_total, wf_origin, wf_dir, wf_light_dir;
+  pt::host::DeviceBuf<uint32_t> wf_q_ext, wf_q_shadow;
+  pt::host::DeviceBuf<int> wf_stack_overflow;
   pt::host::DeviceBuf<uint32_t> wf_counters;
   pt::host::DeviceBuf<unsigned long long> wf_sample_counter;
   pt::host::DeviceBuf<float> wf_samples;
@@ -139,6 +139,7 @@ namespace host {
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
                      const gpu::DeviceScene &scene, uint32_t samples_per,
                      const pt_settings &settings, const gpu::RngParams &rng,
-                     uint32_t x_dim, uint32_t y_dim, const float **final_rgb);
+                     uint32_t x_dim, uint32_t y_dim, const float **final_rgb,
+                     uint64_t *rays_out);
 } // namespace host
 } // namespace pt

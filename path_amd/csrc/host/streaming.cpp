@@ -3,6 +3,7 @@
 #include "renderer.h"
 #include "settings.h"
 
+#include "../gpu/trace.h"
 #include "../gpu/wavefront.h"
 
 #include <cstring>
@@ -24,7 +25,8 @@ namespace host {
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
                      const gpu::DeviceScene &scene, uint32_t samples_per,
                      const pt_settings &settings, const gpu::RngParams &rng,
-                     uint32_t x_dim, uint32_t y_dim, const float **final_rgb) {
+                     uint32_t x_dim, uint32_t y_dim, const float **final_rgb,
+                     uint64_t *rays_out) {
   if (spec.type != PT_SAMPLE_SQUARE_IMAGE)
     return set_error(PT_ERR_UNSUPPORTED,
                      "the Streaming integrator takes SquareImage sample specs");
@@ -50,8 +52,9 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
   HIP_TRY(r.wf_total.ensure(n_slots));
   HIP_TRY(r.wf_origin.ensure(n_slots));
   HIP_TRY(r.wf_dir.ensure(n_slots));
-  HIP_TRY(r.wf_q_ext.ensure(size_t(n_slots) * 2));
-  HIP_TRY(r.wf_q_shadow.ensure(size_t(n_slots) * 2));
+  HIP_TRY(r.wf_light_dir.ensure(n_slots));
+  HIP_TRY(r.wf_q_ext.ensure(n_slots));
+  HIP_TRY(r.wf_q_shadow.ensure(n_slots));
   HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
   HIP_TRY(r.wf_sample_counter.ensure(1));
   HIP_TRY(r.wf_samples.ensure(size_t(total) * 3));
@@ -66,18 +69,23 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
   p.y_dim = y_dim;
   p.total_samples = total;
   p.n_slots = n_slots;
-  p.st = {r.wf_meta.ptr,  r.wf_mult.ptr,   r.wf_light.ptr,   r.wf_total.ptr,
-          r.wf_origin.ptr, r.wf_dir.ptr,  r.wf_hit_ext.ptr, r.wf_hit_light.ptr};
+  p.st = {r.wf_meta.ptr,   r.wf_mult.ptr, r.wf_light.ptr,
+          r.wf_total.ptr,  r.wf_origin.ptr, r.wf_dir.ptr,
+          r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr};
   p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr};
   p.counters = r.wf_counters.ptr;
   p.sample_counter = r.wf_sample_counter.ptr;
   p.ray_counter = r.counter.ptr;
   p.out_samples = r.wf_samples.ptr;
 
+  const uint32_t trace_threads = uint32_t(r.n_cus) * 8 * 256;
+  HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
+  p.stack_overflow = r.wf_stack_overflow.ptr;
+  p.overflow_stride = trace_threads;
   HIP_TRY(hipMemsetAsync(r.wf_meta.ptr, 0, size_t(n_slots) * sizeof(uint4), s));
   HIP_TRY(hipMemsetAsync(r.wf_sample_counter.ptr, 0, sizeof(unsigned long long), s));
-  const uint32_t trace_blocks = uint32_t(r.n_cus) * 8;
-  uint64_t launches = 0;
+  const uint32_t trace_blocks = trace_threads / 256;
+  uint64_t launches = 0, rays_total = 0;
   bool pending = false;
   uint32_t words[gpu::kNumCounters] = {};
   for (;;) {
@@ -96,10 +104,12 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
       r.stats.shadow_ms += b;
       pending = false;
     }
-    if (words[gpu::kActive] == 0)
+    // a slot holds a path iff it queued at least one ray
+    if (words[gpu::kExtCount] + words[gpu::kShadowCount] == 0)
       break;
     r.stats.ext_rays += words[gpu::kExtCount];
     r.stats.shadow_rays += words[gpu::kShadowCount];
+    rays_total += uint64_t(words[gpu::kExtCount]) + words[gpu::kShadowCount];
     HIP_TRY(gpu::launch_wf_reset(
         p.counters, (1u << gpu::kExtHead) | (1u << gpu::kShadowHead), s));
     if (r.profiling)
@@ -126,6 +136,8 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
     out = r.rgb2.ptr;
   }
   r.stats.trace_launches = launches;
+  r.stats.rays = rays_total; // overwritten below by the renderer's counter
+  *rays_out = rays_total;
   *final_rgb = out;
   return PT_OK;
 }
