@@ -261,13 +261,11 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
   const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
   uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0)
-      base = atomicAdd(head, 64u);
-    base = wave_broadcast_first(base);
-    if (base >= count)
-      break;
+  // Static schedule: wave w takes 64-entry batches w, w + W, w + 2W, ...
+  // (same-address atomics for dynamic fetching cost more than the imbalance).
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  (void)head;
+  for (uint32_t base = (gtid >> 6) * 64u; base < count; base += n_waves * 64u) {
     uint32_t i = base + lane_id();
     if (i < count) {
       uint32_t slot = q[i];
