@@ -247,6 +247,49 @@ int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
                  uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
                  uint32_t *max_depth);
 
+/* ------------------------------------------------------------------------ */
+/* Batched rendering for training-data generation. No reference counterpart */
+/* renders several scenes per launch: the reference calls Renderer::render  */
+/* once per scene (src/generate_data/full_scene/generate_data.cpp:280-294). */
+/* ------------------------------------------------------------------------ */
+/* Seeded random scene: 2-40 boxes/spheres, triangle budget log-uniform in
+ * [20, 20000], diffuse albedos U[0.05, 0.95], 1-3 quad emitters of radiance
+ * U[5, 30], square resolution from {32, 64, 96, 128}. */
+int pt_scene_generate(uint64_t seed, pt_scene **out, float film_to_world[12],
+                      uint32_t *resolution);
+
+typedef struct pt_batch_item {
+  const pt_scene *scene;
+  uint32_t x_dim, y_dim;
+  float film_to_world[12];
+  float *out_rgb;     /* x_dim*y_dim*3 floats (host) */
+  float *out_gbuffer; /* NULL or x_dim*y_dim*12 floats (host): first hit of
+                         sample 0 per pixel - position, shading normal, albedo,
+                         emission; zeros where the camera ray misses */
+} pt_batch_item;
+
+/* Render every item with the wavefront integrator in one launch sequence
+ * (all scenes share the path pool). Each image is bit-identical to
+ * pt_render of that scene alone with kernel_approach = Streaming. *seconds
+ * covers scene packing, upload, render and download (BVH builds run on a
+ * host thread pool and are included). */
+int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
+                    uint32_t samples_per, const pt_settings *settings,
+                    double *seconds);
+
+/* Launch parameters of the wavefront integrator. Results never depend on
+ * them. trace_blocks_per_cu: traversal grid = CUs x this x 256 threads;
+ * iters_per_sync: iterations enqueued between host checks; use_graphs:
+ * replay those iterations as one captured hipGraph. */
+int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
+                         uint32_t iters_per_sync, int use_graphs);
+/* Time candidate (path pool, traversal grid, iterations per sync) triples on
+ * the given items, keep the fastest on the renderer; out[4] = path pool,
+ * blocks per CU, iterations per sync, graphs used. */
+int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
+                      uint32_t samples_per, pt_settings *settings,
+                      uint32_t out[4]);
+
 /* Statistics of the last pt_render. kernel_ms spans the launches with HIP
  * events on the renderer's stream. The per-kernel fields are filled only when
  * profiling is enabled (pt_set_profiling), which brackets each traversal or

@@ -101,6 +101,12 @@ class PtRenderStats(C.Structure):
                 ("mega_launches", C.c_uint64), ("profiled", C.c_uint32)]
 
 
+class PtBatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("x_dim", C.c_uint32), ("y_dim", C.c_uint32),
+                ("film_to_world", C.c_float * 12), ("out_rgb", C.POINTER(C.c_float)),
+                ("out_gbuffer", C.POINTER(C.c_float))]
+
+
 # Every exported symbol with its (restype, argtypes).
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
@@ -129,6 +135,12 @@ SIGNATURES = {
     "pt_scene_bvh": (C.c_int, [_P, _U, _F, _F, _U, _U, _U, _U, _U]),
     "pt_last_render_stats": (C.c_int, [_P, C.POINTER(PtRenderStats)]),
     "pt_set_profiling": (C.c_int, [_P, C.c_int]),
+    "pt_scene_generate": (C.c_int, [C.c_uint64, C.POINTER(_P), _F, _U]),
+    "pt_render_batch": (C.c_int, [_P, C.POINTER(PtBatchItem), C.c_uint32, C.c_uint32,
+                                  C.POINTER(PtSettings), C.POINTER(C.c_double)]),
+    "pt_set_launch_params": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_int]),
+    "pt_autotune_batch": (C.c_int, [_P, C.POINTER(PtBatchItem), C.c_uint32, C.c_uint32,
+                                    C.POINTER(PtSettings), _U]),
 }
 
 _lib = None

@@ -43,6 +43,8 @@ SOURCES = [
     "host/settings.cpp",
     "host/renderer.cpp",
     "host/streaming.cpp",
+    "host/batch.cpp",
+    "host/scene_gen.cpp",
     "gpu/megakernel.hip",
     "gpu/wavefront.hip",
 ]

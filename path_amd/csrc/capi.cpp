@@ -6,6 +6,7 @@
 #include "gpu/kernels.h"
 #include "host/bvh.h"
 #include "host/renderer.h"
+#include "host/scene_gen.h"
 #include "host/scene.h"
 #include "host/settings.h"
 
@@ -351,6 +352,55 @@ int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
   if (!r || !out)
     return fail(PT_ERR_INVALID, "null argument");
   return r->sobol_table_to_host(dim_bound, count, out);
+  PT_GUARD_END
+}
+
+int pt_scene_generate(uint64_t seed, pt_scene **out, float film_to_world[12],
+                      uint32_t *resolution) {
+  PT_GUARD_BEGIN
+  if (!out)
+    return fail(PT_ERR_INVALID, "null output pointer");
+  GeneratedScene g = generate_scene(seed);
+  auto sc = std::make_unique<pt_scene>();
+  sc->scene = std::move(g.scene);
+  sc->generation++;
+  if (film_to_world)
+    std::memcpy(film_to_world, g.camera.film_to_world.m, 12 * sizeof(float));
+  if (resolution)
+    *resolution = g.resolution;
+  *out = sc.release();
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
+                    uint32_t samples_per, const pt_settings *settings,
+                    double *seconds) {
+  PT_GUARD_BEGIN
+  if (!r || (n && !items) || !settings)
+    return fail(PT_ERR_INVALID, "null argument");
+  return render_batch(*r, items, n, samples_per, *settings, seconds);
+  PT_GUARD_END
+}
+
+int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
+                         uint32_t iters_per_sync, int use_graphs) {
+  if (!r || trace_blocks_per_cu == 0 || trace_blocks_per_cu > 64 ||
+      iters_per_sync == 0 || iters_per_sync > 64)
+    return fail(PT_ERR_INVALID, "invalid launch parameters");
+  r->trace_blocks_per_cu = trace_blocks_per_cu;
+  r->iters_per_sync = iters_per_sync;
+  r->use_graphs = use_graphs != 0;
+  return PT_OK;
+}
+
+int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
+                      uint32_t samples_per, pt_settings *settings,
+                      uint32_t out[4]) {
+  PT_GUARD_BEGIN
+  if (!r || !items || !n || !settings || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  return autotune_batch(*r, items, n, samples_per, *settings, out);
   PT_GUARD_END
 }
 

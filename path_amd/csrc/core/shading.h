@@ -437,6 +437,28 @@ template <class T> PT_INLINE bool within_target(T target, T t) {
   return abs_(target - t) <= T(1e-6f);
 }
 
+// Surface data of the vertex shaded by one iteration (G-buffer output).
+template <class T> struct VertexInfo {
+  V3<T> p;
+  V3<T> normal;
+  uint32_t material;
+  uint32_t valid;
+};
+
+// Reflectance summary of a material for G-buffers: the diffuse albedo, the
+// specular coefficient, or their lobe-weighted sum for Combined BSDFs.
+template <class T> PT_INLINE V3<T> material_albedo(const Material<T> &m) {
+  switch (m.type) {
+  case kDiffuse:
+    return m.diffuse;
+  case kDiffuseGlossy:
+  case kDiffuseMirror:
+    return m.diffuse * m.weight0 + m.specular * (T(1) - m.weight0);
+  default:
+    return m.specular;
+  }
+}
+
 // One vertex of the rendering equation. `light_hit` / `next_hit` are the
 // closest hits of the light ray (when st.n_light) and of the continuation ray
 // (when st.has_next) emitted by the previous step, with Hit::idx == kNoHit for
@@ -448,7 +470,7 @@ PT_INLINE bool rendering_equation_iteration(
     const PathState<T> &st, R &rng, const Ray<T> &last_ray,
     const Hit<T> &light_hit, const Hit<T> &next_hit,
     const IntegratorSettings &s, const SceneView<T> &sc, PathState<T> *out,
-    Ray<T> *light_ray, Ray<T> *next_ray) {
+    Ray<T> *light_ray, Ray<T> *next_ray, VertexInfo<T> *vertex = nullptr) {
   PathState<T> ns;
   ns.iters = st.iters + 1;
   ns.total = st.total;
@@ -492,6 +514,8 @@ PT_INLINE bool rendering_equation_iteration(
   const V3<T> vn[3] = {tri.n0, tri.n1, tri.n2};
   V3<T> normal = shading_normal(g, vn, p);
   V3<T> in = last_ray.d;
+  if (vertex)
+    *vertex = {p, normal, tri.material, 1u};
 
   if (performs_samples) {
     T pc = prob_continuous(mat);

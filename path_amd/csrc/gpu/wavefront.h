@@ -8,26 +8,27 @@ namespace gpu {
 
 // Counter words used by one render (device memory, zeroed by k_wf_reset).
 enum WfCounter : uint32_t {
-  kExtCount = 0,    // entries in the extension-ray queue
-  kShadowCount = 1, // entries in the shadow-ray queue
-  kExtHead = 2,     // next extension-ray entry to fetch
-  kShadowHead = 3,  // next shadow-ray entry to fetch
-  kActive = 4,      // slots holding a path after the last shade pass
+  kExtCount = 0,      // entries in the extension-ray queue
+  kShadowCount = 1,   // entries in the shadow-ray queue
+  kExtTotal = 2,      // 64-bit running sum of kExtCount (words 2, 3)
+  kShadowTotal = 4,   // 64-bit running sum of kShadowCount (words 4, 5)
+  kBusyIters = 6,     // iterations that queued at least one ray
   kNumCounters = 8,
 };
 
 // Per-slot path state, structure of arrays (one 16-byte vector per slot and
 // array so a wave's accesses are contiguous).
 struct WfState {
-  uint4 *meta;      // location, sample, rng counter, flags
-  float4 *mult;     // ray multiplier xyz, light target distance
-  float4 *light;    // light-sample multiplier xyz
-  float4 *total;    // accumulated radiance xyz
-  float4 *origin;   // origin of the pending rays (last vertex / camera)
-  float4 *dir;      // direction of the pending continuation ray
+  uint4 *meta;       // location, sample, rng counter, flags
+  float4 *mult;      // ray multiplier xyz, light target distance
+  float4 *light;     // light-sample multiplier xyz
+  float4 *total;     // accumulated radiance xyz
+  float4 *origin;    // origin of the pending rays (last vertex / camera)
+  float4 *dir;       // direction of the pending continuation ray
   float4 *light_dir; // direction of the pending light ray, target distance
-  uint2 *hit_ext;   // packed closest hit of the continuation ray
-  uint2 *hit_light; // packed closest hit of the light ray
+  uint2 *hit_ext;    // packed closest hit of the continuation ray
+  uint2 *hit_light;  // packed closest hit of the light ray
+  uint32_t *scene;   // scene of the slot (batched renders only)
 };
 
 // Ray queues hold slot indices; the rays themselves are read from the state.
@@ -36,8 +37,29 @@ struct WfQueues {
   uint32_t *shadow;
 };
 
+// One scene of a batched render. Offsets index the concatenated arrays of all
+// scenes (nodes in GpuNode units, triangles in GpuTri units, etc.).
+struct SceneDesc {
+  uint32_t node_off, tri_off, shade_off, mat_off;
+  uint32_t light_off, n_lights, x_dim, y_dim;
+  uint32_t pixel_off, has_nodes, pad0, pad1;
+  float film_to_world[12];
+};
+
+struct WfBatch {
+  const SceneDesc *scenes;
+  const unsigned long long *sample_begin; // n_scenes + 1 prefix of pixels*spp
+  uint32_t n_scenes;
+  const float4 *nodes, *tris;             // concatenated BVHs
+  const ShadeTri<float> *shade;
+  const Material<float> *materials;
+  const TriGeom<float> *light_tris;
+  const float *light_cdf;
+};
+
 struct WfParams {
-  DeviceScene scene;
+  DeviceScene scene; // single-scene renders
+  WfBatch batch;     // batched renders
   IntegratorSettings settings;
   RngParams rng;
   float film_to_world[12];
@@ -48,18 +70,21 @@ struct WfParams {
   WfQueues q;
   uint32_t *counters;                 // WfCounter words
   unsigned long long *sample_counter; // next sample to start
-  unsigned long long *ray_counter;    // rays traced
-  float *out_samples;                 // [location * spp + sample] * 3
+  float *out_samples;                 // [pixel * spp + sample] * 3
+  float *gbuffer; // first hit of sample 0 per pixel: position, normal,
+                  // albedo, emission (12 floats) or null
   int *stack_overflow;                // traversal stack spill area
   uint32_t overflow_stride;           // threads of a traversal launch
 };
 
-hipError_t launch_wf_reset(uint32_t *counters, uint32_t mask, hipStream_t s);
-hipError_t launch_wf_shade(const WfParams &p, hipStream_t s);
-hipError_t launch_wf_trace_ext(const WfParams &p, uint32_t grid_blocks,
-                               hipStream_t s);
-hipError_t launch_wf_trace_shadow(const WfParams &p, uint32_t grid_blocks,
-                                  hipStream_t s);
+// Fold the queue counts of the previous iteration into the running totals and
+// zero them (one thread; safe inside a captured graph).
+hipError_t launch_wf_reset(uint32_t *counters, hipStream_t s);
+hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s);
+hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
+                               uint32_t grid_blocks, hipStream_t s);
+hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,
+                                  uint32_t grid_blocks, hipStream_t s);
 // Per-pixel sum of the per-sample radiance in the mega-kernel's order
 // (sequential per thread range, then the lane tree), divided by spp.
 hipError_t launch_wf_reduce(const float *samples, const WorkDivision &wd,

@@ -3,8 +3,15 @@
 // traversal over the queues, and the final per-pixel reduction.
 //
 // Each path slot keeps its state between iterations; a slot whose path ends
-// immediately starts the next unstarted sample (one atomic per wave). Rays are
-// appended to compact queues, so the traversal kernels run full waves.
+// immediately starts the next unstarted sample. Rays are appended to compact
+// queues of slot indices, so the traversal kernels run full waves. Counters
+// are aggregated per block (one atomic per block and counter): same-address
+// device-scope atomics are the expensive operation on this chip.
+//
+// Batched renders (many scenes in one launch sequence) are the same kernels
+// instantiated with kBatch: every slot also carries its scene, whose BVH,
+// shading records, materials and lights are addressed through offsets into
+// concatenated arrays.
 // reference semantics: include/render/detail/integrate_image/streaming/run_impl.h:155-460
 // (init / intersect / shade loop) with the per-vertex step of
 // include/integrate/rendering_equation_iteration.h:9-153.
@@ -24,32 +31,22 @@ constexpr uint32_t kFlagCountEmission = 1u << 25;
 constexpr uint32_t kFlagHasNext = 1u << 26;
 constexpr uint32_t kFlagLight = 1u << 27;
 constexpr uint32_t kItersMask = 0x00ffffffu;
-
-__device__ __forceinline__ uint32_t wave_broadcast_first(uint32_t v) {
-  return __builtin_amdgcn_readfirstlane(v);
-}
-
-// Wave-aggregated append: returns this lane's slot in the queue (valid where
-// pred is set). One atomic per wave.
-__device__ __forceinline__ uint32_t queue_append(uint32_t *count, bool pred) {
-  uint32_t total;
-  uint32_t prefix = ballot_prefix(pred, &total);
-  uint32_t base = 0;
-  if (lane_id() == 0 && total)
-    base = atomicAdd(count, total);
-  base = wave_broadcast_first(base);
-  return base + prefix;
-}
 } // namespace
 
-__global__ void k_wf_reset(uint32_t *counters, uint32_t mask) {
-  uint32_t i = threadIdx.x;
-  if (i < kNumCounters && (mask >> i) & 1u)
-    counters[i] = 0;
+__global__ void k_wf_reset(uint32_t *counters) {
+  if (threadIdx.x != 0)
+    return;
+  const uint32_t ne = counters[kExtCount], ns = counters[kShadowCount];
+  auto *tot = reinterpret_cast<unsigned long long *>(counters);
+  tot[kExtTotal / 2] += ne;
+  tot[kShadowTotal / 2] += ns;
+  counters[kBusyIters] += (ne + ns) != 0u;
+  counters[kExtCount] = 0;
+  counters[kShadowCount] = 0;
 }
 
-hipError_t launch_wf_reset(uint32_t *counters, uint32_t mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, counters, mask);
+hipError_t launch_wf_reset(uint32_t *counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, counters);
   return hipGetLastError();
 }
 
@@ -81,20 +78,72 @@ __device__ __forceinline__ uint32_t rng_start(const RngParams &rp,
     return 0;
 }
 
-template <class R>
+// Scene-dependent addressing of one slot.
+struct SlotScene {
+  SceneView<float> view;
+  const float *film_to_world;
+  uint32_t x_dim, y_dim, pixel_off;
+};
+
+template <bool kBatch>
+__device__ __forceinline__ SlotScene slot_scene(const WfParams &p,
+                                                uint32_t scene) {
+  SlotScene s;
+  if constexpr (kBatch) {
+    const SceneDesc &d = p.batch.scenes[scene];
+    s.view.tris = p.batch.shade + d.shade_off;
+    s.view.materials = p.batch.materials + d.mat_off;
+    s.view.light_tris = p.batch.light_tris + d.light_off;
+    s.view.light_cdf = p.batch.light_cdf + d.light_off;
+    s.view.n_lights = d.n_lights;
+    s.film_to_world = d.film_to_world;
+    s.x_dim = d.x_dim;
+    s.y_dim = d.y_dim;
+    s.pixel_off = d.pixel_off;
+  } else {
+    (void)scene;
+    s.view = p.scene.view;
+    s.film_to_world = p.film_to_world;
+    s.x_dim = p.x_dim;
+    s.y_dim = p.y_dim;
+    s.pixel_off = 0;
+  }
+  return s;
+}
+
+// Scene of global sample s (batched): largest i with sample_begin[i] <= s.
+__device__ __forceinline__ uint32_t find_scene(const WfBatch &b,
+                                               unsigned long long s) {
+  uint32_t lo = 0, hi = b.n_scenes; // invariant: begin[lo] <= s < begin[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (b.sample_begin[mid] <= s)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class R, bool kBatch>
 __device__ __forceinline__ void wf_shade(const WfParams &p) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in_range = slot < p.n_slots;
   uint4 meta = in_range ? p.st.meta[slot] : make_uint4(0, 0, 0, 0);
   bool active = in_range && (meta.w & kFlagActive);
+  uint32_t scene = 0;
+  if (kBatch && active)
+    scene = p.st.scene[slot];
 
   // ---- continue the path in this slot ----
   bool push_ext = false, push_light = false, finished = false;
-  V3<float> ext_o = {0.f, 0.f, 0.f}, ext_d = {0.f, 0.f, 0.f}, light_d = {0.f, 0.f, 0.f};
+  V3<float> ext_o = {0.f, 0.f, 0.f}, ext_d = {0.f, 0.f, 0.f},
+            light_d = {0.f, 0.f, 0.f};
   float light_target = 0.f;
   PathState<float> ns;
   uint32_t location = meta.x, sample = meta.y, counter = meta.z;
   if (active) {
+    SlotScene sc = slot_scene<kBatch>(p, scene);
     float4 m = p.st.mult[slot];
     float4 lm = p.st.light[slot];
     float4 tot = p.st.total[slot];
@@ -119,13 +168,24 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     Ray<float> last = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
     R rng = make_rng<R>(p.rng, location, sample, counter);
     Ray<float> nl = last, nn = last;
+    VertexInfo<float> vi{};
+    const bool want_gbuffer = p.gbuffer && st.iters == 0 && sample == 0;
     finished = rendering_equation_iteration(st, rng, last, light_hit, next_hit,
-                                            p.settings, p.scene.view, &ns, &nl,
-                                            &nn);
+                                            p.settings, sc.view, &ns, &nl, &nn,
+                                            want_gbuffer ? &vi : nullptr);
+    if (want_gbuffer && vi.valid) {
+      const Material<float> &mat = sc.view.materials[vi.material];
+      V3<float> alb = material_albedo(mat);
+      float *g = p.gbuffer + 12 * size_t(sc.pixel_off + location);
+      g[0] = vi.p.x, g[1] = vi.p.y, g[2] = vi.p.z;
+      g[3] = vi.normal.x, g[4] = vi.normal.y, g[5] = vi.normal.z;
+      g[6] = alb.x, g[7] = alb.y, g[8] = alb.z;
+      g[9] = mat.emission.x, g[10] = mat.emission.y, g[11] = mat.emission.z;
+    }
     counter = rng_counter(rng);
     if (finished) {
       float *out = p.out_samples +
-                   3 * (size_t(location) * p.rng.samples_per + sample);
+                   3 * (size_t(sc.pixel_off + location) * p.rng.samples_per + sample);
       out[0] = ns.total.x;
       out[1] = ns.total.y;
       out[2] = ns.total.z;
@@ -168,14 +228,20 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
       before += s_counts[0][w];
     unsigned long long s = s_sample_base + before + prefix;
     if (s < p.total_samples) {
-      location = uint32_t(s / p.rng.samples_per);
-      sample = uint32_t(s % p.rng.samples_per);
+      unsigned long long local = s;
+      if constexpr (kBatch) {
+        scene = find_scene(p.batch, s);
+        local = s - p.batch.sample_begin[scene];
+      }
+      SlotScene sc = slot_scene<kBatch>(p, scene);
+      location = uint32_t(local / p.rng.samples_per);
+      sample = uint32_t(local % p.rng.samples_per);
       R rng = make_rng<R>(p.rng, location, sample, rng_start<R>(p.rng, location));
       float xo = rng.next();
       float yo = rng.next();
-      uint32_t px = location % p.x_dim, py = location / p.x_dim;
-      Ray<float> cam = camera_ray(p.film_to_world, float(px) + xo,
-                                  float(py) + yo, p.x_dim, p.y_dim);
+      uint32_t px = location % sc.x_dim, py = location / sc.x_dim;
+      Ray<float> cam = camera_ray(sc.film_to_world, float(px) + xo,
+                                  float(py) + yo, sc.x_dim, sc.y_dim);
       counter = rng_counter(rng);
       ns = initial_path_state<float>();
       push_ext = true;
@@ -192,6 +258,8 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
                      (ns.has_next ? kFlagHasNext : 0u) |
                      (ns.n_light ? kFlagLight : 0u);
     p.st.meta[slot] = make_uint4(location, sample, counter, flags);
+    if constexpr (kBatch)
+      p.st.scene[slot] = scene;
     p.st.mult[slot] = make_float4(ns.ray_mult.x, ns.ray_mult.y, ns.ray_mult.z,
                                   ns.light_target);
     p.st.light[slot] = make_float4(ns.light_mult.x, ns.light_mult.y,
@@ -200,7 +268,8 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     p.st.origin[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.f);
     p.st.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
     if (push_light)
-      p.st.light_dir[slot] = make_float4(light_d.x, light_d.y, light_d.z, light_target);
+      p.st.light_dir[slot] =
+          make_float4(light_d.x, light_d.y, light_d.z, light_target);
   } else if (in_range && (active || want_new)) {
     p.st.meta[slot] = make_uint4(0, 0, 0, 0); // empty slot
   }
@@ -232,39 +301,41 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     p.q.shadow[bs + ps] = slot;
 }
 
-// Requires blockDim.x == 256 (block-level counter aggregation).
-__global__ __launch_bounds__(256) void k_wf_shade_sobol(WfParams p) {
-  wf_shade<TableRng>(p);
-}
-__global__ __launch_bounds__(256) void k_wf_shade_uniform(WfParams p) {
-  wf_shade<HashRng>(p);
+// All shade kernels require blockDim.x == 256 (block-level aggregation).
+template <class R, bool kBatch>
+__global__ __launch_bounds__(256) void k_wf_shade(WfParams p) {
+  wf_shade<R, kBatch>(p);
 }
 
-hipError_t launch_wf_shade(const WfParams &p, hipStream_t s) {
+hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
   uint32_t blocks = (p.n_slots + 255) / 256;
-  if (p.rng.type == kRngSobel)
-    hipLaunchKernelGGL(k_wf_shade_sobol, dim3(blocks), dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL(k_wf_shade_uniform, dim3(blocks), dim3(256), 0, s, p);
+  bool sobol = p.rng.type == kRngSobel;
+  if (batched) {
+    if (sobol)
+      hipLaunchKernelGGL((k_wf_shade<TableRng, true>), dim3(blocks), dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL((k_wf_shade<HashRng, true>), dim3(blocks), dim3(256), 0, s, p);
+  } else {
+    if (sobol)
+      hipLaunchKernelGGL((k_wf_shade<TableRng, false>), dim3(blocks), dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL((k_wf_shade<HashRng, false>), dim3(blocks), dim3(256), 0, s, p);
+  }
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Persistent traversal: each wave takes 64 queue entries at a time until the
-// queue is exhausted (every wave reaches the exit test).
+// Persistent traversal. Static schedule: wave w takes the 64-entry batches
+// w, w + W, w + 2W, ... of its queue (no same-address atomics).
 // ---------------------------------------------------------------------------
-template <bool kShadow>
+template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStack * 256];
   const uint32_t count = p.counters[kShadow ? kShadowCount : kExtCount];
-  uint32_t *head = &p.counters[kShadow ? kShadowHead : kExtHead];
   const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
   uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  // Static schedule: wave w takes 64-entry batches w, w + W, w + 2W, ...
-  // (same-address atomics for dynamic fetching cost more than the imbalance).
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  (void)head;
   for (uint32_t base = (gtid >> 6) * 64u; base < count; base += n_waves * 64u) {
     uint32_t i = base + lane_id();
     if (i < count) {
@@ -274,31 +345,45 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
       Ray<float> ray = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
       float limit = kShadow ? d.w + 2e-6f : 3.4028235e38f;
       float occl = kShadow ? d.w - 2e-6f : -1.0f;
-      LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid,
-                     p.overflow_stride};
-      Hit<float> h = trace_closest(p.scene.nodes, p.scene.tris, ray, limit,
-                                   occl, stack);
+      const float4 *nodes = p.scene.nodes, *tris = p.scene.tris;
+      bool has_nodes = true;
+      if constexpr (kBatch) {
+        const SceneDesc &sd = p.batch.scenes[p.st.scene[slot]];
+        nodes = p.batch.nodes + 4 * size_t(sd.node_off);
+        tris = p.batch.tris + 3 * size_t(sd.tri_off);
+        has_nodes = sd.has_nodes != 0;
+      }
+      Hit<float> h = {0.f, kNoHit, 0};
+      if (has_nodes) {
+        LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid,
+                       p.overflow_stride};
+        h = trace_closest(nodes, tris, ray, limit, occl, stack);
+      }
       hits[slot] = pack_hit(h);
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_wf_trace_ext(WfParams p) {
-  wf_trace<false>(p);
-}
-__global__ __launch_bounds__(256) void k_wf_trace_shadow(WfParams p) {
-  wf_trace<true>(p);
+template <bool kShadow, bool kBatch>
+__global__ __launch_bounds__(256) void k_wf_trace(WfParams p) {
+  wf_trace<kShadow, kBatch>(p);
 }
 
-hipError_t launch_wf_trace_ext(const WfParams &p, uint32_t grid_blocks,
-                               hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_trace_ext, dim3(grid_blocks), dim3(256), 0, s, p);
+hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
+                               uint32_t grid_blocks, hipStream_t s) {
+  if (batched)
+    hipLaunchKernelGGL((k_wf_trace<false, true>), dim3(grid_blocks), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_wf_trace<false, false>), dim3(grid_blocks), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_wf_trace_shadow(const WfParams &p, uint32_t grid_blocks,
-                                  hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_trace_shadow, dim3(grid_blocks), dim3(256), 0, s, p);
+hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,
+                                  uint32_t grid_blocks, hipStream_t s) {
+  if (batched)
+    hipLaunchKernelGGL((k_wf_trace<true, true>), dim3(grid_blocks), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_wf_trace<true, false>), dim3(grid_blocks), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 

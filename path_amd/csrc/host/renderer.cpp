@@ -76,6 +76,8 @@ pt_renderer::~pt_renderer() {
   for (hipEvent_t e : ev_prof)
     if (e)
       (void)hipEventDestroy(e);
+  if (wf_host_counts)
+    (void)hipHostFree(wf_host_counts);
   if (stream)
     (void)hipStreamDestroy(stream);
 }

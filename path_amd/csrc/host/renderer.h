@@ -3,6 +3,7 @@
 
 #include "../../../include/path_amd.h"
 #include "../gpu/kernels.h"
+#include "../gpu/wavefront.h"
 #include "bvh.h"
 #include "scene.h"
 
@@ -117,7 +118,24 @@ struct pt_renderer {
   pt::host::DeviceBuf<uint32_t> wf_counters;
   pt::host::DeviceBuf<unsigned long long> wf_sample_counter;
   pt::host::DeviceBuf<float> wf_samples;
+  pt::host::DeviceBuf<uint32_t> wf_scene;
+  uint32_t *wf_host_counts = nullptr; // pinned
+
+  // batched-render buffers (concatenated scenes)
+  pt::host::DeviceBuf<pt::host::GpuNode> b_nodes;
+  pt::host::DeviceBuf<pt::host::GpuTri> b_tris;
+  pt::host::DeviceBuf<pt::ShadeTri<float>> b_shade;
+  pt::host::DeviceBuf<pt::Material<float>> b_mats;
+  pt::host::DeviceBuf<pt::TriGeom<float>> b_light_tris;
+  pt::host::DeviceBuf<float> b_light_cdf;
+  pt::host::DeviceBuf<pt::gpu::SceneDesc> b_desc;
+  pt::host::DeviceBuf<unsigned long long> b_sample_begin;
+  pt::host::DeviceBuf<float> b_gbuffer;
   int n_cus = 256;
+  // wavefront launch parameters (tunable; results do not depend on them)
+  uint32_t trace_blocks_per_cu = 8;
+  uint32_t iters_per_sync = 1;
+  bool use_graphs = false;
 
   ~pt_renderer();
   bool init(int device, std::string *err);
@@ -134,6 +152,14 @@ struct pt_renderer {
 
 namespace pt {
 namespace host {
+int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
+                    uint64_t total_samples, bool batched, uint32_t trace_blocks);
+int wavefront_run(pt_renderer &r, gpu::WfParams &p, bool batched,
+                  uint32_t trace_blocks, uint32_t iters_per_sync);
+int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
+                 uint32_t spp, const pt_settings &settings, double *seconds);
+int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
+                   uint32_t spp, pt_settings &settings, uint32_t out[4]);
 // Wavefront ("Streaming") integrator; writes the final per-pixel FloatRGB
 // (already divided by samples_per) into a device buffer owned by the renderer.
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,

@@ -22,6 +22,166 @@ using namespace pt::host;
 namespace pt {
 namespace host {
 
+int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
+                    uint64_t total_samples, bool batched, uint32_t trace_blocks) {
+  HIP_TRY(r.wf_meta.ensure(n_slots));
+  HIP_TRY(r.wf_hit_ext.ensure(n_slots));
+  HIP_TRY(r.wf_hit_light.ensure(n_slots));
+  HIP_TRY(r.wf_mult.ensure(n_slots));
+  HIP_TRY(r.wf_light.ensure(n_slots));
+  HIP_TRY(r.wf_total.ensure(n_slots));
+  HIP_TRY(r.wf_origin.ensure(n_slots));
+  HIP_TRY(r.wf_dir.ensure(n_slots));
+  HIP_TRY(r.wf_light_dir.ensure(n_slots));
+  if (batched)
+    HIP_TRY(r.wf_scene.ensure(n_slots));
+  HIP_TRY(r.wf_q_ext.ensure(n_slots));
+  HIP_TRY(r.wf_q_shadow.ensure(n_slots));
+  HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
+  HIP_TRY(r.wf_sample_counter.ensure(1));
+  HIP_TRY(r.wf_samples.ensure(size_t(total_samples) * 3));
+  const uint32_t trace_threads = trace_blocks * 256;
+  HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
+  p.total_samples = total_samples;
+  p.n_slots = n_slots;
+  p.st = {r.wf_meta.ptr,     r.wf_mult.ptr,   r.wf_light.ptr,
+          r.wf_total.ptr,    r.wf_origin.ptr, r.wf_dir.ptr,
+          r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr,
+          batched ? r.wf_scene.ptr : nullptr};
+  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr};
+  p.counters = r.wf_counters.ptr;
+  p.sample_counter = r.wf_sample_counter.ptr;
+  p.out_samples = r.wf_samples.ptr;
+  p.stack_overflow = r.wf_stack_overflow.ptr;
+  p.overflow_stride = trace_threads;
+  return PT_OK;
+}
+
+// Enqueue one iteration: fold the previous queue counts into the totals and
+// zero them, shade, copy the counts to pinned host memory, trace extension
+// and shadow rays.
+static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
+                                    bool batched, uint32_t trace_blocks,
+                                    bool profile_events) {
+  hipStream_t s = r.stream;
+  hipError_t e;
+  if ((e = gpu::launch_wf_reset(p.counters, s)) != hipSuccess)
+    return e;
+  if ((e = gpu::launch_wf_shade(p, batched, s)) != hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(r.wf_host_counts, p.counters,
+                          sizeof(uint32_t) * gpu::kNumCounters,
+                          hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if (profile_events && (e = hipEventRecord(r.ev_prof[0], s)) != hipSuccess)
+    return e;
+  if ((e = gpu::launch_wf_trace_ext(p, batched, trace_blocks, s)) != hipSuccess)
+    return e;
+  if (profile_events && (e = hipEventRecord(r.ev_prof[1], s)) != hipSuccess)
+    return e;
+  if ((e = gpu::launch_wf_trace_shadow(p, batched, trace_blocks, s)) != hipSuccess)
+    return e;
+  if (profile_events && (e = hipEventRecord(r.ev_prof[2], s)) != hipSuccess)
+    return e;
+  return hipSuccess;
+}
+
+int wavefront_run(pt_renderer &r, gpu::WfParams &p, bool batched,
+                  uint32_t trace_blocks, uint32_t iters_per_sync) {
+  hipStream_t s = r.stream;
+  HIP_TRY(hipMemsetAsync(p.st.meta, 0, size_t(p.n_slots) * sizeof(uint4), s));
+  HIP_TRY(hipMemsetAsync(p.sample_counter, 0, sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(p.counters, 0, sizeof(uint32_t) * gpu::kNumCounters, s));
+  if (!r.wf_host_counts)
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&r.wf_host_counts),
+                          sizeof(uint32_t) * gpu::kNumCounters, hipHostMallocDefault));
+  const bool prof = r.profiling;
+  if (iters_per_sync == 0)
+    iters_per_sync = 1;
+  // Profiling needs per-iteration event reads, so it syncs every iteration.
+  if (prof)
+    iters_per_sync = 1;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = r.use_graphs && !prof;
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (uint32_t k = 0; k < iters_per_sync && e == hipSuccess; ++k)
+      e = enqueue_iteration(r, p, batched, trace_blocks, false);
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph)
+        (void)hipGraphDestroy(graph);
+      return set_error(PT_ERR_HIP, std::string("graph capture failed: ") +
+                                       hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+      return set_error(PT_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                       hipGetErrorString(e));
+  }
+  uint64_t launches = 0;
+  int rc = PT_OK;
+  for (;;) {
+    if (use_graph) {
+      hipError_t e = hipGraphLaunch(exec, s);
+      if (e != hipSuccess) {
+        rc = set_error(PT_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        break;
+      }
+      launches += 2 * iters_per_sync;
+    } else {
+      for (uint32_t k = 0; k < iters_per_sync; ++k) {
+        hipError_t e = enqueue_iteration(r, p, batched, trace_blocks, prof);
+        if (e != hipSuccess) {
+          rc = set_error(PT_ERR_HIP, hipGetErrorString(e));
+          break;
+        }
+        launches += 2;
+      }
+      if (rc != PT_OK)
+        break;
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      rc = set_error(PT_ERR_HIP, std::string("sync: ") + hipGetErrorString(e));
+      break;
+    }
+    // counts of the last iteration's queues (pinned host copy in the stream)
+    uint32_t ne = r.wf_host_counts[gpu::kExtCount];
+    uint32_t ns = r.wf_host_counts[gpu::kShadowCount];
+    if (prof && ne + ns) {
+      float a = 0.f, b = 0.f;
+      (void)hipEventElapsedTime(&a, r.ev_prof[0], r.ev_prof[1]);
+      (void)hipEventElapsedTime(&b, r.ev_prof[1], r.ev_prof[2]);
+      r.stats.ext_ms += a;
+      r.stats.shadow_ms += b;
+    }
+    // a slot holds a path iff it queued at least one ray
+    if (ne + ns == 0)
+      break;
+  }
+  if (exec)
+    (void)hipGraphExecDestroy(exec);
+  if (rc != PT_OK)
+    return rc;
+  // the last iteration queued nothing, so the totals are complete
+  uint32_t c[gpu::kNumCounters];
+  HIP_TRY(hipMemcpyAsync(c, p.counters, sizeof(c), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint64_t ext = uint64_t(c[gpu::kExtTotal]) | (uint64_t(c[gpu::kExtTotal + 1]) << 32);
+  uint64_t shadow =
+      uint64_t(c[gpu::kShadowTotal]) | (uint64_t(c[gpu::kShadowTotal + 1]) << 32);
+  r.stats.ext_rays = ext;
+  r.stats.shadow_rays = shadow;
+  r.stats.ext_launches = c[gpu::kBusyIters];
+  r.stats.shadow_launches = c[gpu::kBusyIters];
+  r.stats.trace_launches = launches;
+  return rc;
+}
+
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
                      const gpu::DeviceScene &scene, uint32_t samples_per,
                      const pt_settings &settings, const gpu::RngParams &rng,
@@ -42,24 +202,7 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
     return set_error(PT_ERR_INVALID, "max_num_samples_per_launch must be positive");
   if (slots64 > total)
     slots64 = total;
-  const uint32_t n_slots = uint32_t(slots64);
-  hipStream_t s = r.stream;
-  HIP_TRY(r.wf_meta.ensure(n_slots));
-  HIP_TRY(r.wf_hit_ext.ensure(n_slots));
-  HIP_TRY(r.wf_hit_light.ensure(n_slots));
-  HIP_TRY(r.wf_mult.ensure(n_slots));
-  HIP_TRY(r.wf_light.ensure(n_slots));
-  HIP_TRY(r.wf_total.ensure(n_slots));
-  HIP_TRY(r.wf_origin.ensure(n_slots));
-  HIP_TRY(r.wf_dir.ensure(n_slots));
-  HIP_TRY(r.wf_light_dir.ensure(n_slots));
-  HIP_TRY(r.wf_q_ext.ensure(n_slots));
-  HIP_TRY(r.wf_q_shadow.ensure(n_slots));
-  HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
-  HIP_TRY(r.wf_sample_counter.ensure(1));
-  HIP_TRY(r.wf_samples.ensure(size_t(total) * 3));
-  HIP_TRY(r.rgb.ensure(size_t(wd.num_sample_blocks) * n_pixels * 3));
-
+  const uint32_t trace_blocks = uint32_t(r.n_cus) * r.trace_blocks_per_cu;
   gpu::WfParams p{};
   p.scene = scene;
   p.settings = integrator_settings(settings);
@@ -67,65 +210,14 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
   std::memcpy(p.film_to_world, spec.film_to_world, sizeof(p.film_to_world));
   p.x_dim = x_dim;
   p.y_dim = y_dim;
-  p.total_samples = total;
-  p.n_slots = n_slots;
-  p.st = {r.wf_meta.ptr,   r.wf_mult.ptr, r.wf_light.ptr,
-          r.wf_total.ptr,  r.wf_origin.ptr, r.wf_dir.ptr,
-          r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr};
-  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr};
-  p.counters = r.wf_counters.ptr;
-  p.sample_counter = r.wf_sample_counter.ptr;
-  p.ray_counter = r.counter.ptr;
-  p.out_samples = r.wf_samples.ptr;
-
-  const uint32_t trace_threads = uint32_t(r.n_cus) * 8 * 256;
-  HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
-  p.stack_overflow = r.wf_stack_overflow.ptr;
-  p.overflow_stride = trace_threads;
-  HIP_TRY(hipMemsetAsync(r.wf_meta.ptr, 0, size_t(n_slots) * sizeof(uint4), s));
-  HIP_TRY(hipMemsetAsync(r.wf_sample_counter.ptr, 0, sizeof(unsigned long long), s));
-  const uint32_t trace_blocks = trace_threads / 256;
-  uint64_t launches = 0, rays_total = 0;
-  bool pending = false;
-  uint32_t words[gpu::kNumCounters] = {};
-  for (;;) {
-    HIP_TRY(gpu::launch_wf_reset(p.counters,
-                                 (1u << gpu::kExtCount) | (1u << gpu::kShadowCount) |
-                                     (1u << gpu::kActive),
-                                 s));
-    HIP_TRY(gpu::launch_wf_shade(p, s));
-    HIP_TRY(hipMemcpyAsync(words, p.counters, sizeof(words), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (pending) { // the traversal launches of the previous iteration are done
-      float a = 0.f, b = 0.f;
-      HIP_TRY(hipEventElapsedTime(&a, r.ev_prof[0], r.ev_prof[1]));
-      HIP_TRY(hipEventElapsedTime(&b, r.ev_prof[1], r.ev_prof[2]));
-      r.stats.ext_ms += a;
-      r.stats.shadow_ms += b;
-      pending = false;
-    }
-    // a slot holds a path iff it queued at least one ray
-    if (words[gpu::kExtCount] + words[gpu::kShadowCount] == 0)
-      break;
-    r.stats.ext_rays += words[gpu::kExtCount];
-    r.stats.shadow_rays += words[gpu::kShadowCount];
-    rays_total += uint64_t(words[gpu::kExtCount]) + words[gpu::kShadowCount];
-    HIP_TRY(gpu::launch_wf_reset(
-        p.counters, (1u << gpu::kExtHead) | (1u << gpu::kShadowHead), s));
-    if (r.profiling)
-      HIP_TRY(hipEventRecord(r.ev_prof[0], s));
-    HIP_TRY(gpu::launch_wf_trace_ext(p, trace_blocks, s));
-    if (r.profiling)
-      HIP_TRY(hipEventRecord(r.ev_prof[1], s));
-    HIP_TRY(gpu::launch_wf_trace_shadow(p, trace_blocks, s));
-    if (r.profiling) {
-      HIP_TRY(hipEventRecord(r.ev_prof[2], s));
-      pending = true;
-    }
-    r.stats.ext_launches++;
-    r.stats.shadow_launches++;
-    launches += 2;
-  }
+  int rc = wavefront_alloc(r, p, uint32_t(slots64), total, false, trace_blocks);
+  if (rc != PT_OK)
+    return rc;
+  HIP_TRY(r.rgb.ensure(size_t(wd.num_sample_blocks) * n_pixels * 3));
+  rc = wavefront_run(r, p, false, trace_blocks, r.iters_per_sync);
+  if (rc != PT_OK)
+    return rc;
+  hipStream_t s = r.stream;
   HIP_TRY(gpu::launch_wf_reduce(r.wf_samples.ptr, wd, samples_per, r.rgb.ptr, s));
   const float *out = r.rgb.ptr;
   if (wd.num_sample_blocks != 1) {
@@ -135,10 +227,8 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
                                              r.rgb2.ptr, s));
     out = r.rgb2.ptr;
   }
-  r.stats.trace_launches = launches;
-  r.stats.rays = rays_total; // overwritten below by the renderer's counter
-  *rays_out = rays_total;
   *final_rgb = out;
+  *rays_out = r.stats.ext_rays + r.stats.shadow_rays;
   return PT_OK;
 }
 

@@ -1,0 +1,90 @@
+"""Training-data generation: seeded random scenes rendered many per launch.
+
+The reference renders one scene per Renderer::render call
+(src/generate_data/full_scene/generate_data.cpp:280-294) and its Python
+datagen drives that loop scene by scene. Here a batch of scenes shares one
+wavefront path pool: every image is bit-identical to rendering its scene alone
+with the Streaming integrator, and the first-hit G-buffer (position, shading
+normal, albedo, emission of sample 0) is written on the way.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .renderer import Renderer, Scene, Settings
+
+
+@dataclass
+class GeneratedScene:
+    scene: Scene
+    film_to_world: np.ndarray  # (12,) float32
+    resolution: int
+    seed: int
+
+
+def generate_scene(seed: int) -> GeneratedScene:
+    """Deterministic random scene (boxes/spheres, 1-3 quad emitters)."""
+    h = C.c_void_p()
+    ftw = np.zeros(12, np.float32)
+    res = C.c_uint32()
+    L.check(L.lib().pt_scene_generate(C.c_uint64(seed), C.byref(h), L.fptr(ftw),
+                                      C.byref(res)))
+    return GeneratedScene(Scene(h), ftw, int(res.value), seed)
+
+
+def _items(scenes, gbuffer: bool):
+    items = (L.PtBatchItem * len(scenes))()
+    images, gbufs = [], []
+    for i, g in enumerate(scenes):
+        r = g.resolution
+        img = np.zeros((r, r, 3), np.float32)
+        gb = np.zeros((r, r, 12), np.float32) if gbuffer else None
+        items[i].scene = g.scene.handle
+        items[i].x_dim = r
+        items[i].y_dim = r
+        items[i].film_to_world[:] = [float(v) for v in g.film_to_world]
+        items[i].out_rgb = L.fptr(img)
+        items[i].out_gbuffer = L.fptr(gb)
+        images.append(img)
+        gbufs.append(gb)
+    return items, images, gbufs
+
+
+def render_batch(renderer: Renderer, scenes, samples_per: int,
+                 settings: Settings | None = None, gbuffer: bool = True):
+    """Render every scene of `scenes` (GeneratedScene list) in one launch
+    sequence. Returns (images, gbuffers, seconds)."""
+    settings = settings or datagen_settings()
+    items, images, gbufs = _items(scenes, gbuffer)
+    secs = C.c_double()
+    L.check(L.lib().pt_render_batch(renderer.handle, items, len(scenes), samples_per,
+                                    C.byref(settings.raw), C.byref(secs)))
+    return images, gbufs, secs.value
+
+
+def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -> dict:
+    """Pick the fastest (path pool, traversal blocks per CU, iterations per
+    host sync) for this workload; the renderer and `settings` keep it."""
+    items, _, _ = _items(scenes, False)
+    out = (C.c_uint32 * 4)()
+    L.check(L.lib().pt_autotune_batch(renderer.handle, items, len(scenes), samples_per,
+                                      C.byref(settings.raw), out))
+    return {"max_num_samples_per_launch": out[0], "trace_blocks_per_cu": out[1],
+            "iters_per_sync": out[2], "graphs": bool(out[3])}
+
+
+def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 8,
+                      iters_per_sync: int = 1, use_graphs: bool = False) -> None:
+    L.check(L.lib().pt_set_launch_params(renderer.handle, trace_blocks_per_cu,
+                                         iters_per_sync, int(use_graphs)))
+
+
+def datagen_settings() -> Settings:
+    """Streaming integrator, Sobol sampling, BSDF directions, NEE."""
+    s = Settings()
+    s.kernel_approach = "Streaming"
+    return s

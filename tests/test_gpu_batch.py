@@ -1,0 +1,85 @@
+"""Batched multi-scene rendering (training-data generation) on the GPU."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = list(range(6))
+SPP = 4
+
+
+@pytest.fixture(scope="module")
+def gen_scenes():
+    from path_amd.datagen import generate_scene
+    return [generate_scene(s) for s in SEEDS]
+
+
+def _single(renderer, g, spp, settings):
+    img, _ = renderer.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp,
+                             settings=settings, output="FloatRGB")
+    return img
+
+
+def test_batch_equals_single_scene_renders(renderer, gen_scenes):
+    from path_amd.datagen import datagen_settings, render_batch, set_launch_params
+    set_launch_params(renderer, 8, 1, False)
+    st = datagen_settings()
+    images, gbufs, secs = render_batch(renderer, gen_scenes, SPP, st)
+    stats = renderer.last_stats()
+    assert stats.samples == sum(g.resolution ** 2 for g in gen_scenes) * SPP
+    assert stats.ext_rays > 0 and stats.shadow_rays > 0
+    for g, img in zip(gen_scenes, images):
+        ref = _single(renderer, g, SPP, st)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), g.seed
+        assert np.isfinite(img).all() and img.max() > 0
+
+
+def test_batch_small_pool_and_graphs_bit_exact(renderer, gen_scenes):
+    """Results are independent of the path pool, the traversal grid, the
+    number of iterations between host syncs and hipGraph replay."""
+    from path_amd.datagen import datagen_settings, render_batch, set_launch_params
+    st = datagen_settings()
+    set_launch_params(renderer, 8, 1, False)
+    base, _, _ = render_batch(renderer, gen_scenes, SPP, st, gbuffer=False)
+    for blocks, iters, graphs, pool in [(4, 4, True, 3000), (16, 2, True, 1 << 20),
+                                        (8, 3, False, 777)]:
+        s2 = st.copy()
+        s2.max_num_samples_per_launch = pool
+        set_launch_params(renderer, blocks, iters, graphs)
+        imgs, _, _ = render_batch(renderer, gen_scenes, SPP, s2, gbuffer=False)
+        for a, b in zip(base, imgs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (blocks, iters, graphs, pool)
+    set_launch_params(renderer, 8, 1, False)
+
+
+def test_gbuffer_first_hit(renderer, gen_scenes):
+    """The G-buffer position is the closest hit along the camera ray through
+    it, its normal is the hit triangle's (unit, facing the camera side), and
+    albedo/emission are the hit material's."""
+    from path_amd.datagen import datagen_settings, render_batch
+    _, gbufs, _ = render_batch(renderer, gen_scenes[:3], SPP, datagen_settings())
+    for g, gb in zip(gen_scenes[:3], gbufs):
+        flat = g.scene.flat()
+        pos = gb[..., 0:3].reshape(-1, 3)
+        nrm = gb[..., 3:6].reshape(-1, 3)
+        alb = gb[..., 6:9].reshape(-1, 3)
+        emi = gb[..., 9:12].reshape(-1, 3)
+        hit = np.abs(nrm).sum(1) > 0
+        assert hit.mean() > 0.05
+        assert np.allclose(np.linalg.norm(nrm[hit], axis=1), 1.0, atol=1e-4)
+        o = g.film_to_world[[3, 7, 11]].astype(np.float64)
+        d = pos[hit].astype(np.float64) - o
+        dist = np.linalg.norm(d, axis=1)
+        rays = np.concatenate([np.broadcast_to(o, d.shape), d / dist[:, None]], 1)
+        t, idx, back = renderer.intersect(g.scene, rays.astype(np.float32))
+        ok = (idx != 0xFFFFFFFF) & (np.abs(t - dist) <= 1e-3 * np.maximum(dist, 1))
+        assert ok.mean() > 0.97
+        mats = flat["materials"]
+        tri_mat = flat["tri_mat"][idx[ok]]
+        want_alb = np.array([list(mats[m].diffuse) for m in tri_mat], np.float32)
+        want_emi = np.array([list(mats[m].emission) for m in tri_mat], np.float32)
+        # grazing rays may pick a neighbouring triangle of another object
+        agree = np.all(np.abs(alb[hit][ok] - want_alb) < 1e-6, axis=1)
+        assert agree.mean() > 0.97
+        assert np.allclose(emi[hit][ok][agree], want_emi[agree])
+        assert (gb[~hit.reshape(gb.shape[:2])] == 0).all()
