@@ -12,7 +12,13 @@ counted with the reference's AoS sizes on the ray set the CPU checker traces
 for the same pixels) and `cpu_baseline` (the FP32 CPU checker on the host's
 cores, bounded pixel sample of the same workload).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3]
+
+`--config c3` is the batched training-data workload instead: 1024 seeded
+scenes (2-40 boxes/spheres, 20-20k triangles log-uniform, resolutions
+{32,64,96,128}^2), 128 spp, max depth 6, FP32 radiance + G-buffer; one step
+renders the whole batch through the shared path pool and the line reports
+scenes/s next to the same scenes rendered one render call per scene.
 """
 from __future__ import annotations
 
@@ -38,7 +44,172 @@ CONFIGS = {
                level=6, size=512, spp=64, iters=7, approach="Streaming"),
     "c1": dict(workload="cornell_box_36tri_256x256_16spp_depth5_megakernel",
                level=None, size=256, spp=16, iters=4, approach="MegaKernel"),
+    "c3": dict(workload="datagen_1024_seeded_scenes_128spp_depth6_gbuffer",
+               n_scenes=1024, spp=128, iters=5, approach="Streaming"),
 }
+
+
+def _roofline(name, ms, launches, n_rays, bpr, config):
+    avg_s = ms / 1e3 / max(launches, 1)
+    bytes_per_launch = bpr * n_rays / max(launches, 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = None
+    tf = ROOT / "profiles" / f"{name}_{config}_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_dispatch")
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+            "traffic": traffic, "kernel": name,
+            "avg_launch_ms": round(avg_s * 1e3, 4),
+            "algorithmic_bytes_per_ray": round(bpr, 1),
+            "rays_per_launch": round(n_rays / max(launches, 1), 1)}
+
+
+def _bytes_per_ray(counts, kind):
+    n_rays, n_nodes, n_tris_t = (float(x) for x in counts[kind])
+    if n_rays == 0:
+        return 0.0
+    return RAY_BYTES + NODE_BYTES * n_nodes / n_rays + TRI_BYTES * n_tris_t / n_rays
+
+
+def bench_datagen(a, cfg):
+    """Batched datagen: scenes/s of one batched render of every scene vs one
+    render call per scene (same scenes, settings and images)."""
+    import numpy as np
+    from path_amd.datagen import (autotune, datagen_settings, generate_scene,
+                                  render_batch, set_launch_params)
+    from path_amd.renderer import Renderer
+    n, spp = cfg["n_scenes"], cfg["spp"]
+    t0 = time.perf_counter()
+    gens = [generate_scene(seed) for seed in range(n)]
+    gen_s = time.perf_counter() - t0
+    settings = datagen_settings()
+    settings.term_prob = "NIters"
+    settings.n_iters = cfg["iters"]
+    if a.max_paths:
+        settings.max_num_samples_per_launch = a.max_paths
+    r = Renderer(0)
+    # host BVH builds (thread pool inside the first batched call) + warm-up
+    t0 = time.perf_counter()
+    render_batch(r, gens, spp, settings)
+    first_s = time.perf_counter() - t0
+    tuned = None
+    if not a.no_autotune:
+        tuned = autotune(r, gens[: min(n, 128)], spp, settings)
+    for _ in range(max(a.warmup - 1, 0)):
+        render_batch(r, gens, spp, settings)
+    sync = _sync_fn()
+    sync()
+    t0 = time.perf_counter()
+    rays = 0
+    kernel_ms = 0.0
+    for _ in range(a.steps):
+        images, gbufs, _ = render_batch(r, gens, spp, settings)
+        st = r.last_stats()
+        rays += st.rays
+        kernel_ms += st.kernel_ms
+    sync()
+    elapsed = time.perf_counter() - t0
+    value = n * a.steps / elapsed
+    total_px = sum(g.resolution ** 2 for g in gens)
+
+    # per-scene launches: the same scenes, one render call each
+    single = settings.copy()
+    set_launch_params(r, 8, 1, False)
+    for g in gens[:4]:
+        r.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp, single)
+    sync()
+    t0 = time.perf_counter()
+    mismatched = 0
+    for i, g in enumerate(gens):
+        img, _ = r.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp, single)
+        if not np.array_equal(img.view(np.uint32), images[i].view(np.uint32)):
+            mismatched += 1
+    sync()
+    per_scene_s = time.perf_counter() - t0
+    per_scene_value = n / per_scene_s
+
+    # profiled batched render for the per-kernel split
+    if tuned:
+        set_launch_params(r, tuned["trace_blocks_per_cu"], 1, False)
+    r.set_profiling(True)
+    render_batch(r, gens, spp, settings, gbuffer=False)
+    pst = r.last_stats()
+    r.set_profiling(False)
+
+    out = {
+        "metric": "scenes/s (batched datagen, 1 GPU)",
+        "value": round(value, 2),
+        "unit": "scenes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "scenes": n, "spp": spp,
+                   "max_depth": cfg["iters"] + 1, "pixels": total_px,
+                   "triangles_min": min(g.scene.counts()[0] for g in gens),
+                   "triangles_max": max(g.scene.counts()[0] for g in gens),
+                   "integrator": "Streaming", "sampler": "Sobel", "gbuffer": True},
+        "msamples_per_s": round(total_px * spp * a.steps / elapsed / 1e6, 2),
+        "mrays_per_s": round(rays / elapsed / 1e6, 2),
+        "gpu_ms_per_batch": round(kernel_ms / a.steps, 2),
+        "per_scene_launches": {"value": round(per_scene_value, 2), "unit": "scenes/s",
+                               "seconds": round(per_scene_s, 3),
+                               "images_differing_from_batched": mismatched},
+        "speedup_vs_per_scene": round(value / per_scene_value, 2),
+        "autotuned": tuned,
+        "host_seconds": {"generate": round(gen_s, 3),
+                         "first_batch_incl_bvh_builds": round(first_s, 3)},
+    }
+
+    counts = None
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        tab = O.sobol_table(O.joe_kuo_directions(settings.max_sample_size),
+                            settings.max_sample_size, spp)
+        done, secs, counts = 0, 0.0, np.zeros((2, 3), np.uint64)
+        while done < n and secs < 12.0:
+            g = gens[done]
+            _, info = O.render(g.scene.flat(), g.film_to_world, settings, spp, g.resolution,
+                               g.resolution, tab, 32, 0)
+            secs += info["seconds"]
+            counts += info["counts"]
+            done += 1
+        out["cpu_baseline"] = {
+            "value": round(done / secs, 4), "unit": "scenes/s", "cores": O.max_threads(),
+            "kind": "port",
+            "sample": f"first {done} scenes of the batch, full resolution and spp, FP32 CPU "
+                      f"checker, {secs:.1f} s"}
+    roof = None
+    if counts is not None and pst.ext_launches:
+        kernels = {
+            "trace_ext": (pst.ext_ms, pst.ext_launches, pst.ext_rays, _bytes_per_ray(counts, 0)),
+            "trace_shadow": (pst.shadow_ms, pst.shadow_launches, pst.shadow_rays,
+                             _bytes_per_ray(counts, 1)),
+        }
+        name, vals = max(kernels.items(), key=lambda kv: kv[1][0])
+        roof = _roofline(name, *vals, config="c3")
+        if roof["frac"] > 1.0:
+            roof["note"] = ("per-scene BVHs (<= 20k triangles) stay cache resident, so the "
+                            "algorithmic node/triangle bytes exceed the HBM traffic")
+    out["roofline"] = roof
+    print(json.dumps(out))
+    return 0
+
+
+def _sync_fn():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.synchronize
+    except Exception:  # torch is optional plumbing here
+        pass
+    return lambda: None
 
 
 def build_scene(cfg, d):
@@ -103,6 +274,8 @@ def main(argv=None):
                     help="override the configuration's integrator")
     ap.add_argument("--max-paths", type=int, default=0,
                     help="wavefront path-pool size (max_num_samples_per_launch)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c3: keep the default launch parameters")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU only: the path shards by sample range with a single "
@@ -111,6 +284,8 @@ def main(argv=None):
 
     from path_amd.renderer import Renderer
     cfg = dict(CONFIGS[a.config])
+    if a.config == "c3":
+        return bench_datagen(a, cfg)
     if a.integrator:
         cfg["approach"] = a.integrator
     with tempfile.TemporaryDirectory() as d:
@@ -123,11 +298,7 @@ def main(argv=None):
     size, spp = cfg["size"], cfg["spp"]
     for _ in range(a.warmup):
         r.render(scene, ftw, size, size, spp, settings)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is optional plumbing here
-        sync = lambda: None
+    sync = _sync_fn()
     sync()
     t0 = time.perf_counter()
     rays = 0
@@ -173,16 +344,12 @@ def main(argv=None):
     # roofline of the dominant kernel
     roof = None
     if counts is not None:
-        def bytes_per_ray(kind):
-            n_rays, n_nodes, n_tris_t = (float(x) for x in counts[kind])
-            if n_rays == 0:
-                return 0.0
-            return RAY_BYTES + NODE_BYTES * n_nodes / n_rays + TRI_BYTES * n_tris_t / n_rays
         if cfg["approach"] == "Streaming" and st.ext_launches:
             kernels = {
-                "trace_ext": (st.ext_ms, st.ext_launches, st.ext_rays, bytes_per_ray(0)),
+                "trace_ext": (st.ext_ms, st.ext_launches, st.ext_rays,
+                              _bytes_per_ray(counts, 0)),
                 "trace_shadow": (st.shadow_ms, st.shadow_launches, st.shadow_rays,
-                                 bytes_per_ray(1)),
+                                 _bytes_per_ray(counts, 1)),
             }
         else:
             # mega-kernel: every ray of the render
@@ -190,20 +357,8 @@ def main(argv=None):
             bpr = (RAY_BYTES * tot_rays + NODE_BYTES * float(counts[0][1] + counts[1][1]) +
                    TRI_BYTES * float(counts[0][2] + counts[1][2])) / max(tot_rays, 1.0)
             kernels = {"megakernel": (st.mega_ms, st.mega_launches, st.rays, bpr)}
-        name, (ms, launches, n_rays, bpr) = max(kernels.items(), key=lambda kv: kv[1][0])
-        avg_s = ms / 1e3 / max(launches, 1)
-        bytes_per_launch = bpr * n_rays / max(launches, 1)
-        achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-        traffic = None
-        tf = ROOT / "profiles" / f"{name}_{a.config}_traffic.json"
-        if tf.exists():
-            traffic = json.loads(tf.read_text()).get("bytes_per_dispatch")
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                "traffic": traffic, "kernel": name,
-                "avg_launch_ms": round(avg_s * 1e3, 4),
-                "algorithmic_bytes_per_ray": round(bpr, 1),
-                "rays_per_launch": round(n_rays / max(launches, 1), 1)}
+        name, vals = max(kernels.items(), key=lambda kv: kv[1][0])
+        roof = _roofline(name, *vals, config=a.config)
     out["roofline"] = roof
     print(json.dumps(out))
     return 0

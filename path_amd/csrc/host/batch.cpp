@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace pt;
@@ -27,78 +28,122 @@ using namespace pt::host;
 
 namespace {
 
-// Flatten + build BVHs of distinct scenes on a host thread pool.
-void build_all(const pt_batch_item *items, uint32_t n) {
-  std::vector<const pt_scene *> uniq;
-  for (uint32_t i = 0; i < n; ++i)
-    if (std::find(uniq.begin(), uniq.end(), items[i].scene) == uniq.end())
-      uniq.push_back(items[i].scene);
+// Run f(i) for i in [0, n) on up to 16 host threads.
+template <class F> void parallel_for(size_t n, F f) {
   unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::thread> pool;
+  unsigned nt = unsigned(std::min<size_t>(hw, n));
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i)
+      f(i);
+    return;
+  }
   std::atomic<size_t> next{0};
-  for (unsigned t = 0; t < hw; ++t)
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t)
     pool.emplace_back([&] {
-      for (size_t k = next++; k < uniq.size(); k = next++) {
-        uniq[k]->flat();
-        uniq[k]->bvh();
-      }
+      for (size_t k = next++; k < n; k = next++)
+        f(k);
     });
   for (auto &th : pool)
     th.join();
 }
 
-struct Packed {
-  std::vector<GpuNode> nodes;
-  std::vector<GpuTri> tris;
-  std::vector<ShadeTri<float>> shade;
-  std::vector<Material<float>> mats;
-  std::vector<TriGeom<float>> light_tris;
-  std::vector<float> light_cdf;
-  std::vector<gpu::SceneDesc> desc;
-  std::vector<unsigned long long> sample_begin;
-  uint64_t n_pixels = 0;
+// Flatten, build and pack the BVHs of the distinct scenes in parallel.
+void build_all(const pt_batch_item *items, uint32_t n) {
+  std::vector<const pt_scene *> uniq;
+  for (uint32_t i = 0; i < n; ++i)
+    uniq.push_back(items[i].scene);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  parallel_for(uniq.size(), [&](size_t k) { uniq[k]->packed(); });
+}
+
+// Byte layout of the concatenated scene arrays in the pinned staging buffer
+// (each section 256-byte aligned).
+struct Layout {
+  enum { kNodes, kTris, kShade, kMats, kLightTris, kLightCdf, kDesc, kBegin, kN };
+  size_t off[kN + 1];
+  size_t count[kN];
 };
 
-int pack(const pt_batch_item *items, uint32_t n, uint32_t spp, Packed *pk) {
-  pk->sample_begin.push_back(0);
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
+         std::vector<gpu::SceneDesc> *desc, Layout *lay, uint64_t *n_pixels,
+         uint64_t *total_samples) {
+  desc->resize(n);
+  std::vector<unsigned long long> begin(n + 1, 0);
+  size_t c[Layout::kN] = {};
+  uint64_t px_total = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const pt_batch_item &it = items[i];
-    if (!it.scene || it.x_dim == 0 || it.y_dim == 0 || !it.out_rgb)
-      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
-                                           ": null scene/output or empty image");
-    const FlatScene &f = it.scene->flat();
-    const Bvh &b = it.scene->bvh();
-    if (b.max_depth + 1 >= uint32_t(gpu_stack_size()))
+    const pt_scene &sc = *it.scene;
+    const FlatScene &f = sc.flat();
+    const PackedScene &pk = sc.packed();
+    if (pk.bvh.max_depth + 1 >= uint32_t(gpu_stack_size()))
       return set_error(PT_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
                                                ": BVH too deep for the traversal stack");
-    GpuBvh g = pack_gpu_bvh(b, f.tris);
     gpu::SceneDesc d{};
-    d.node_off = uint32_t(pk->nodes.size());
-    d.tri_off = uint32_t(pk->tris.size());
-    d.shade_off = uint32_t(pk->shade.size());
-    d.mat_off = uint32_t(pk->mats.size());
-    d.light_off = uint32_t(pk->light_tris.size());
+    d.node_off = uint32_t(c[Layout::kNodes]);
+    d.tri_off = uint32_t(c[Layout::kTris]);
+    d.shade_off = uint32_t(c[Layout::kShade]);
+    d.mat_off = uint32_t(c[Layout::kMats]);
+    d.light_off = uint32_t(c[Layout::kLightTris]);
     d.n_lights = uint32_t(f.light_tris.size());
     d.x_dim = it.x_dim;
     d.y_dim = it.y_dim;
-    d.pixel_off = uint32_t(pk->n_pixels);
-    d.has_nodes = g.nodes.empty() ? 0u : 1u;
+    d.pixel_off = uint32_t(px_total);
+    d.has_nodes = pk.bvh.nodes.empty() ? 0u : 1u;
     std::memcpy(d.film_to_world, it.film_to_world, sizeof(d.film_to_world));
-    pk->nodes.insert(pk->nodes.end(), g.nodes.begin(), g.nodes.end());
-    pk->tris.insert(pk->tris.end(), g.tris.begin(), g.tris.end());
-    for (size_t t = 0; t < f.tris.size(); ++t)
-      pk->shade.push_back(make_shade_tri(f.tris[t], &f.normals[3 * t], f.tri_mat[t]));
-    for (const auto &m : f.materials)
-      pk->mats.push_back(m.to<float>());
-    pk->light_tris.insert(pk->light_tris.end(), f.light_tris.begin(), f.light_tris.end());
-    pk->light_cdf.insert(pk->light_cdf.end(), f.light_cdf.begin(), f.light_cdf.end());
-    pk->desc.push_back(d);
+    (*desc)[i] = d;
+    c[Layout::kNodes] += pk.bvh.nodes.size();
+    c[Layout::kTris] += pk.bvh.tris.size();
+    c[Layout::kShade] += pk.shade.size();
+    c[Layout::kMats] += pk.mats.size();
+    c[Layout::kLightTris] += f.light_tris.size();
+    c[Layout::kLightCdf] += f.light_cdf.size();
     uint64_t px = uint64_t(it.x_dim) * it.y_dim;
-    pk->n_pixels += px;
-    pk->sample_begin.push_back(pk->sample_begin.back() + px * spp);
+    px_total += px;
+    begin[i + 1] = begin[i] + px * spp;
+    if (px_total >= 0xffffffffull || c[Layout::kNodes] >= 0x7fffffffull ||
+        c[Layout::kTris] >= 0x7fffffffull)
+      return set_error(PT_ERR_INVALID, "batch too large for 32-bit offsets");
   }
-  if (pk->n_pixels >= 0xffffffffull)
-    return set_error(PT_ERR_INVALID, "batch has too many pixels");
+  c[Layout::kDesc] = n;
+  c[Layout::kBegin] = n + 1;
+  const size_t elem[Layout::kN] = {sizeof(GpuNode),         sizeof(GpuTri),
+                                   sizeof(ShadeTri<float>), sizeof(Material<float>),
+                                   sizeof(TriGeom<float>),  sizeof(float),
+                                   sizeof(gpu::SceneDesc),  sizeof(unsigned long long)};
+  lay->off[0] = 0;
+  for (int k = 0; k < Layout::kN; ++k) {
+    lay->count[k] = c[k];
+    lay->off[k + 1] = lay->off[k] + align256(c[k] * elem[k]);
+  }
+  hipError_t e = r.b_stage_in.ensure(lay->off[Layout::kN]);
+  if (e != hipSuccess)
+    return set_error(PT_ERR_HIP, std::string("pinned staging: ") + hipGetErrorString(e));
+  char *base = static_cast<char *>(r.b_stage_in.ptr);
+  std::memcpy(base + lay->off[Layout::kDesc], desc->data(), n * sizeof(gpu::SceneDesc));
+  std::memcpy(base + lay->off[Layout::kBegin], begin.data(), (n + 1) * sizeof(begin[0]));
+  parallel_for(n, [&](size_t i) {
+    const gpu::SceneDesc &d = (*desc)[i];
+    const FlatScene &f = items[i].scene->flat();
+    const PackedScene &pk = items[i].scene->packed();
+    auto put = [&](int sec, size_t first, const void *src, size_t bytes) {
+      if (bytes)
+        std::memcpy(base + lay->off[sec] + first * elem[sec], src, bytes);
+    };
+    put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(), pk.bvh.nodes.size() * sizeof(GpuNode));
+    put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
+    put(Layout::kShade, d.shade_off, pk.shade.data(), pk.shade.size() * sizeof(ShadeTri<float>));
+    put(Layout::kMats, d.mat_off, pk.mats.data(), pk.mats.size() * sizeof(Material<float>));
+    put(Layout::kLightTris, d.light_off, f.light_tris.data(),
+        f.light_tris.size() * sizeof(TriGeom<float>));
+    put(Layout::kLightCdf, d.light_off, f.light_cdf.data(), f.light_cdf.size() * sizeof(float));
+  });
+  *n_pixels = px_total;
+  *total_samples = begin[n];
   return PT_OK;
 }
 
@@ -115,20 +160,33 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     return set_error(PT_ERR_INVALID, "samples_per must be positive");
   auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipSetDevice(r.device));
+  for (uint32_t i = 0; i < n; ++i)
+    if (!items[i].scene || items[i].x_dim == 0 || items[i].y_dim == 0 || !items[i].out_rgb)
+      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                           ": null scene/output or empty image");
   build_all(items, n);
-  Packed pk;
-  int rc = pack(items, n, spp, &pk);
+  std::vector<gpu::SceneDesc> desc;
+  Layout lay;
+  uint64_t n_pixels = 0, total = 0;
+  int rc = pack(r, items, n, spp, &desc, &lay, &n_pixels, &total);
   if (rc != PT_OK)
     return rc;
   hipStream_t s = r.stream;
-  HIP_TRY(r.b_nodes.upload(pk.nodes.data(), pk.nodes.size(), s));
-  HIP_TRY(r.b_tris.upload(pk.tris.data(), pk.tris.size(), s));
-  HIP_TRY(r.b_shade.upload(pk.shade.data(), pk.shade.size(), s));
-  HIP_TRY(r.b_mats.upload(pk.mats.data(), pk.mats.size(), s));
-  HIP_TRY(r.b_light_tris.upload(pk.light_tris.data(), pk.light_tris.size(), s));
-  HIP_TRY(r.b_light_cdf.upload(pk.light_cdf.data(), pk.light_cdf.size(), s));
-  HIP_TRY(r.b_desc.upload(pk.desc.data(), pk.desc.size(), s));
-  HIP_TRY(r.b_sample_begin.upload(pk.sample_begin.data(), pk.sample_begin.size(), s));
+  {
+    const char *base = static_cast<const char *>(r.b_stage_in.ptr);
+    auto up = [&](auto &buf, int sec) {
+      using T = std::remove_pointer_t<decltype(buf.ptr)>;
+      return buf.upload(reinterpret_cast<const T *>(base + lay.off[sec]), lay.count[sec], s);
+    };
+    HIP_TRY(up(r.b_nodes, Layout::kNodes));
+    HIP_TRY(up(r.b_tris, Layout::kTris));
+    HIP_TRY(up(r.b_shade, Layout::kShade));
+    HIP_TRY(up(r.b_mats, Layout::kMats));
+    HIP_TRY(up(r.b_light_tris, Layout::kLightTris));
+    HIP_TRY(up(r.b_light_cdf, Layout::kLightCdf));
+    HIP_TRY(up(r.b_desc, Layout::kDesc));
+    HIP_TRY(up(r.b_sample_begin, Layout::kBegin));
+  }
 
   gpu::RngParams rng{};
   rng.type = settings.rng;
@@ -141,7 +199,6 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     rng.dim_bound = settings.max_sample_size;
     rng.initial_dim_bound = settings.max_sample_size < 64 ? settings.max_sample_size : 64;
   }
-  const uint64_t total = pk.sample_begin.back();
   uint64_t slots = settings.max_num_samples_per_launch;
   if (slots == 0)
     return set_error(PT_ERR_INVALID, "max_num_samples_per_launch must be positive");
@@ -166,11 +223,11 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   for (uint32_t i = 0; i < n; ++i)
     want_gbuffer |= items[i].out_gbuffer != nullptr;
   if (want_gbuffer) {
-    HIP_TRY(r.b_gbuffer.ensure(size_t(pk.n_pixels) * 12));
-    HIP_TRY(hipMemsetAsync(r.b_gbuffer.ptr, 0, size_t(pk.n_pixels) * 12 * sizeof(float), s));
+    HIP_TRY(r.b_gbuffer.ensure(size_t(n_pixels) * 12));
+    HIP_TRY(hipMemsetAsync(r.b_gbuffer.ptr, 0, size_t(n_pixels) * 12 * sizeof(float), s));
     p.gbuffer = r.b_gbuffer.ptr;
   }
-  HIP_TRY(r.rgb.ensure(size_t(pk.n_pixels) * 3));
+  HIP_TRY(r.rgb.ensure(size_t(n_pixels) * 3));
   r.stats = pt_render_stats{};
   r.stats.profiled = r.profiling ? 1u : 0u;
   HIP_TRY(hipEventRecord(r.ev_start, s));
@@ -185,7 +242,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                             it.x_dim, it.y_dim, &wd) ||
         wd.block_size > 1024)
       return set_error(PT_ERR_INVALID, "invalid render work division settings");
-    size_t off = pk.desc[i].pixel_off;
+    size_t off = desc[i].pixel_off;
     size_t px = size_t(it.x_dim) * it.y_dim;
     if (wd.num_sample_blocks == 1) {
       HIP_TRY(gpu::launch_wf_reduce(r.wf_samples.ptr + 3 * off * spp, wd, spp,
@@ -199,16 +256,23 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     }
   }
   HIP_TRY(hipEventRecord(r.ev_stop, s));
-  for (uint32_t i = 0; i < n; ++i) {
-    size_t off = pk.desc[i].pixel_off;
-    size_t px = size_t(items[i].x_dim) * items[i].y_dim;
-    HIP_TRY(hipMemcpyAsync(items[i].out_rgb, r.rgb.ptr + 3 * off, px * 12,
+  // results: one copy each into pinned staging, then a parallel scatter
+  const size_t rgb_bytes = size_t(n_pixels) * 3 * sizeof(float);
+  const size_t gb_bytes = want_gbuffer ? size_t(n_pixels) * 12 * sizeof(float) : 0;
+  HIP_TRY(r.b_stage_out.ensure(rgb_bytes + gb_bytes));
+  char *stage = static_cast<char *>(r.b_stage_out.ptr);
+  HIP_TRY(hipMemcpyAsync(stage, r.rgb.ptr, rgb_bytes, hipMemcpyDeviceToHost, s));
+  if (want_gbuffer)
+    HIP_TRY(hipMemcpyAsync(stage + rgb_bytes, r.b_gbuffer.ptr, gb_bytes,
                            hipMemcpyDeviceToHost, s));
-    if (items[i].out_gbuffer)
-      HIP_TRY(hipMemcpyAsync(items[i].out_gbuffer, r.b_gbuffer.ptr + 12 * off, px * 48,
-                             hipMemcpyDeviceToHost, s));
-  }
   HIP_TRY(hipStreamSynchronize(s));
+  parallel_for(n, [&](size_t i) {
+    size_t off = desc[i].pixel_off;
+    size_t px = size_t(items[i].x_dim) * items[i].y_dim;
+    std::memcpy(items[i].out_rgb, stage + off * 12, px * 12);
+    if (items[i].out_gbuffer)
+      std::memcpy(items[i].out_gbuffer, stage + rgb_bytes + off * 48, px * 48);
+  });
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
   r.stats.kernel_ms = ms;

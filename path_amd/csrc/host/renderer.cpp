@@ -36,15 +36,27 @@ const Bvh &pt_scene::bvh() const {
   return bvh_;
 }
 
-hipError_t DeviceSceneStore::upload(const FlatScene &f, const Bvh &bvh,
-                                    hipStream_t s) {
-  GpuBvh g = pack_gpu_bvh(bvh, f.tris);
-  std::vector<Material<float>> mats;
-  for (const auto &m : f.materials)
-    mats.push_back(m.to<float>());
-  std::vector<ShadeTri<float>> st(f.tris.size());
-  for (size_t i = 0; i < f.tris.size(); ++i)
-    st[i] = make_shade_tri(f.tris[i], &f.normals[3 * i], f.tri_mat[i]);
+const PackedScene &pt_scene::packed() const {
+  if (packed_gen_ != generation) {
+    const FlatScene &f = flat();
+    packed_.bvh = pack_gpu_bvh(bvh(), f.tris);
+    packed_.mats.clear();
+    for (const auto &m : f.materials)
+      packed_.mats.push_back(m.to<float>());
+    packed_.shade.resize(f.tris.size());
+    for (size_t i = 0; i < f.tris.size(); ++i)
+      packed_.shade[i] = make_shade_tri(f.tris[i], &f.normals[3 * i], f.tri_mat[i]);
+    packed_gen_ = generation;
+  }
+  return packed_;
+}
+
+hipError_t DeviceSceneStore::upload(const pt_scene &scene, hipStream_t s) {
+  const FlatScene &f = scene.flat();
+  const PackedScene &pk = scene.packed();
+  const GpuBvh &g = pk.bvh;
+  const auto &mats = pk.mats;
+  const auto &st = pk.shade;
   hipError_t e;
   if ((e = nodes.upload(g.nodes.data(), g.nodes.size(), s)) != hipSuccess ||
       (e = tris.upload(g.tris.data(), g.tris.size(), s)) != hipSuccess ||
@@ -105,14 +117,13 @@ bool pt_renderer::init(int dev, std::string *err) {
 int pt_renderer::ensure_scene(const pt_scene &scene) {
   if (cached_scene == &scene && cached_gen == scene.generation)
     return PT_OK;
-  const FlatScene &f = scene.flat();
   const Bvh &b = scene.bvh();
   if (b.max_depth + 1 >= uint32_t(gpu_stack_size()))
     return set_error(PT_ERR_UNSUPPORTED,
                      "BVH depth " + std::to_string(b.max_depth) +
                          " exceeds the traversal stack");
   HIP_TRY(hipSetDevice(device));
-  HIP_TRY(scene_store.upload(f, b, stream));
+  HIP_TRY(scene_store.upload(scene, stream));
   cached_scene = &scene;
   cached_gen = scene.generation;
   return PT_OK;

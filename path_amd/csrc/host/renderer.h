@@ -19,17 +19,33 @@ inline int gpu_stack_size() { return gpu::kTraceStackSize; }
 } // namespace host
 } // namespace pt
 
+namespace pt {
+namespace host {
+// Device-layout arrays of one scene, ready to copy: wave64 BVH, shading
+// records, FP32 materials.
+struct PackedScene {
+  GpuBvh bvh;
+  std::vector<ShadeTri<float>> shade;
+  std::vector<Material<float>> mats;
+};
+} // namespace host
+} // namespace pt
+
+// Caches are rebuilt when `generation` changes; distinct scenes may be built
+// on different threads at once.
 struct pt_scene {
   pt::host::Scene scene;
   uint64_t generation = 0;
 
   const pt::host::FlatScene &flat() const;
   const pt::host::Bvh &bvh() const;
+  const pt::host::PackedScene &packed() const;
 
 private:
-  mutable uint64_t flat_gen_ = ~0ull, bvh_gen_ = ~0ull;
+  mutable uint64_t flat_gen_ = ~0ull, bvh_gen_ = ~0ull, packed_gen_ = ~0ull;
   mutable pt::host::FlatScene flat_;
   mutable pt::host::Bvh bvh_;
+  mutable pt::host::PackedScene packed_;
 };
 
 namespace pt {
@@ -77,7 +93,32 @@ struct DeviceSceneStore {
   gpu::DeviceScene dev;
   uint32_t max_depth = 0;
 
-  hipError_t upload(const FlatScene &f, const Bvh &bvh, hipStream_t s);
+  hipError_t upload(const pt_scene &scene, hipStream_t s);
+};
+
+// Grow-only pinned host allocation (staging for batched copies).
+struct PinnedBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() {
+    if (ptr)
+      (void)hipHostFree(ptr);
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && ptr)
+      return hipSuccess;
+    if (ptr)
+      (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&ptr, n ? n : 1, hipHostMallocDefault);
+    if (e == hipSuccess)
+      bytes = n;
+    return e;
+  }
 };
 
 } // namespace host
@@ -131,6 +172,7 @@ struct pt_renderer {
   pt::host::DeviceBuf<pt::gpu::SceneDesc> b_desc;
   pt::host::DeviceBuf<unsigned long long> b_sample_begin;
   pt::host::DeviceBuf<float> b_gbuffer;
+  pt::host::PinnedBuf b_stage_in, b_stage_out;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
   uint32_t trace_blocks_per_cu = 8;

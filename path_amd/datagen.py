@@ -69,10 +69,12 @@ def render_batch(renderer: Renderer, scenes, samples_per: int,
 def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -> dict:
     """Pick the fastest (path pool, traversal blocks per CU, iterations per
     host sync) for this workload; the renderer and `settings` keep it."""
-    items, _, _ = _items(scenes, False)
+    # the output arrays must outlive the call: the renders write into them
+    items, images, _ = _items(scenes, False)
     out = (C.c_uint32 * 4)()
     L.check(L.lib().pt_autotune_batch(renderer.handle, items, len(scenes), samples_per,
                                       C.byref(settings.raw), out))
+    del images
     return {"max_num_samples_per_launch": out[0], "trace_blocks_per_cu": out[1],
             "iters_per_sync": out[2], "graphs": bool(out[3])}
 
