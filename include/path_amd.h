@@ -290,6 +290,32 @@ int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                       uint32_t samples_per, pt_settings *settings,
                       uint32_t out[4]);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiable rendering. No reference counterpart (the reference has no  */
+/* gradients); the forward pass is pt_render's mega-kernel estimator.        */
+/* ------------------------------------------------------------------------ */
+/* Replace material `idx` (geometry, BVH and the emitter list are kept; a
+ * material may not switch between emissive and non-emissive). Renderers
+ * re-upload only the material table and the emitter CDF. */
+int pt_scene_set_material(pt_scene *scene, uint32_t idx, const pt_material *m);
+
+/* Path-replay adjoint of a SquareImage render with the same scene, camera,
+ * spp and settings: given d_image = dLoss/dImage (host, y_dim*x_dim*3 floats,
+ * FloatRGB layout), writes dLoss/d(albedo color) and dLoss/d(emission) for
+ * every scene material (host, n_materials*3 floats each). The albedo color is
+ * `diffuse` for Diffuse materials and `specular` for Glossy, Mirror and
+ * DielectricRefractive ones. Paths are recomputed from their Sobol
+ * coordinates; nothing is stored per vertex. Requirements: term_prob NIters or
+ * Constant, no two-lobe BSDFs, at most 40 materials. Emission gradients hold
+ * the emitter list and its selection distribution fixed (exact when one
+ * material emits; for a non-emissive material only the hits its surface gets
+ * from BSDF-sampled rays count). Channels whose albedo is 0 report 0. */
+int pt_render_adjoint(pt_renderer *r, const pt_scene *scene,
+                      const float film_to_world[12], uint32_t x_dim,
+                      uint32_t y_dim, uint32_t samples_per,
+                      const pt_settings *settings, const float *d_image,
+                      float *grad_albedo, float *grad_emission, double *seconds);
+
 /* Statistics of the last pt_render. kernel_ms spans the launches with HIP
  * events on the renderer's stream. The per-kernel fields are filled only when
  * profiling is enabled (pt_set_profiling), which brackets each traversal or

@@ -221,6 +221,132 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
   return total;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 forward-mode derivative of the mega-kernel estimator for fixed samples
+// (the check of the GPU's reverse-mode path replay). The path loop of
+// rendering_equation.h:27-129 / rendering_equation_iteration.h:9-153 is
+// restated here with, next to every multiplier, its derivative with respect
+// to each material's albedo color (one channel at a time: channel c of a
+// multiplier depends only on channel c of the albedos). The albedo-linear
+// factors are evaluated twice, with the material's color and with a unit
+// color, so no quantity is ever divided by an albedo:
+//   vertex of material m:  W = sample weight,  Wu = weight with unit color
+//     d(M * W)[j] = dM[j] * W + [j == m] * M * Wu     (same for the NEE eval)
+//   emission term  M * Le[e]:  dL/dalbedo[j] += dM[j] * Le[e],
+//                              dL/dLe[e]     += M.
+// The emitter-selection CDF is held fixed (as it is exact when one material
+// emits).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOracleMaxMats = 64;
+
+using V3d = V3<double>;
+
+Material<double> unit_color(Material<double> m) {
+  if (m.type == kDiffuse)
+    m.diffuse = {1.0, 1.0, 1.0};
+  else
+    m.specular = {1.0, 1.0, 1.0};
+  return m;
+}
+
+// Returns false if the unit-color replay of a direction sample diverged
+// (the sampler is expected not to depend on the color).
+template <class R>
+bool adjoint_path(const RenderArgs<double> &a, uint32_t px, uint32_t py, R rng,
+                  const double d_i[3], uint32_t n_mats, double *g_alb, double *g_em,
+                  V3d *radiance) {
+  const SceneView<double> view = a.scene->view();
+  const IntegratorSettings &s = a.settings;
+  double xo = rng.next();
+  double yo = rng.next();
+  Ray<double> ray = camera_ray(a.film_to_world, double(float(px)) + xo,
+                               double(float(py)) + yo, a.wd.x_dim, a.wd.y_dim);
+  Ray<double> light_ray = ray;
+  V3d M = {1.0, 1.0, 1.0}, LM = {0.0, 0.0, 0.0}, total = {0.0, 0.0, 0.0};
+  V3d dM[kOracleMaxMats], dLM[kOracleMaxMats];
+  for (uint32_t j = 0; j < n_mats; ++j)
+    dM[j] = dLM[j] = {0.0, 0.0, 0.0};
+  double light_target = 0.0;
+  bool has_light = false, has_next = true, count_emission = true;
+  const bool performs_samples = s.light_sampler != kLightNone;
+  auto add_emission = [&](const V3d &mult, const V3d *dmult, uint32_t e) {
+    const V3d le = view.materials[e].emission;
+    total += mult * le;
+    for (uint32_t j = 0; j < n_mats; ++j) {
+      const V3d d = dmult[j] * le;
+      g_alb[3 * j] += d_i[0] * d.x;
+      g_alb[3 * j + 1] += d_i[1] * d.y;
+      g_alb[3 * j + 2] += d_i[2] * d.z;
+    }
+    g_em[3 * e] += d_i[0] * mult.x;
+    g_em[3 * e + 1] += d_i[1] * mult.y;
+    g_em[3 * e + 2] += d_i[2] * mult.z;
+  };
+  for (uint32_t iters = 0;; ++iters) {
+    Hit<double> light_hit{0.0, kNoHit, 0}, next_hit{0.0, kNoHit, 0};
+    if (has_light)
+      light_hit = bvh_closest(*a.scene, light_ray, nullptr);
+    if (has_next)
+      next_hit = bvh_closest(*a.scene, ray, nullptr);
+    if (has_light && light_hit.idx != kNoHit && within_target(light_target, light_hit.t) &&
+        !(s.back_cull_emission && light_hit.back))
+      add_emission(LM, dLM, view.tris[light_hit.idx].material);
+    if (!has_next || next_hit.idx == kNoHit)
+      break;
+    const V3d p = hit_point(ray, next_hit.t);
+    const ShadeTri<double> tri = view.tris[next_hit.idx];
+    const uint32_t m = tri.material;
+    const Material<double> mat = view.materials[m];
+    const Material<double> unit = unit_color(mat);
+    if ((!performs_samples || count_emission) && !(s.back_cull_emission && next_hit.back))
+      add_emission(M, dM, m);
+    const TriGeom<double> g = {tri.v0, tri.e1, tri.e2};
+    const V3d vn[3] = {tri.n0, tri.n1, tri.n2};
+    const V3d normal = shading_normal(g, vn, p);
+    const V3d in = ray.d;
+    has_light = false;
+    if (performs_samples) {
+      const double pc = prob_continuous(mat);
+      LightSample<double> ls;
+      if (pc > 0.0 && sample_light(view, p, mat, normal, rng, &ls)) {
+        const V3d e = continuous_eval(mat, in, ls.dir, normal);
+        const V3d eu = continuous_eval(unit, in, ls.dir, normal);
+        LM = ((M * e) * pc) / ls.multiplier;
+        for (uint32_t j = 0; j < n_mats; ++j)
+          dLM[j] = ((dM[j] * e) * pc) / ls.multiplier;
+        dLM[m] += ((M * eu) * pc) / ls.multiplier;
+        light_target = ls.target_distance;
+        has_light = true;
+        light_ray = {p, ls.dir};
+      }
+    }
+    R rng_unit = rng;
+    const DirSample<double> ds = sample_direction(s, mat, in, normal, rng);
+    const DirSample<double> du = sample_direction(s, unit, in, normal, rng_unit);
+    if (!(ds.dir.x == du.dir.x && ds.dir.y == du.dir.y && ds.dir.z == du.dir.z))
+      return false;
+    count_emission = ds.is_discrete;
+    V3d nm = M * ds.multiplier;
+    V3d dnm[kOracleMaxMats];
+    for (uint32_t j = 0; j < n_mats; ++j)
+      dnm[j] = dM[j] * ds.multiplier;
+    dnm[m] += M * du.multiplier;
+    const double tp = term_prob(s, iters, nm);
+    has_next = rng.next() > tp;
+    if (has_next) {
+      const double denom = 1.0 - tp;
+      M = {nm.x / denom, nm.y / denom, nm.z / denom};
+      for (uint32_t j = 0; j < n_mats; ++j)
+        dM[j] = {dnm[j].x / denom, dnm[j].y / denom, dnm[j].z / denom};
+      ray = {p, ds.dir};
+    } else if (!has_light) {
+      break;
+    }
+  }
+  *radiance = total;
+  return true;
+}
+
 // Renders the pixels [pixel_begin, pixel_end) (row-major) or all when
 // pixel_end == 0. Output: per pixel RGB (already divided by spp when there is a
 // single sample block, else the sum over sample blocks divided by spp).
@@ -324,6 +450,21 @@ void fill_scene(OScene<T> &s, const float *tris, const float *normals,
   build_accel(s, float_tris(tris, n_tris));
 }
 
+IntegratorSettings integrator_from(const pt_settings &st) {
+  IntegratorSettings is{};
+  is.light_sampler = st.light_sampler;
+  is.dir_sampler = st.dir_sampler;
+  is.term_prob = st.term_prob;
+  is.rng = st.rng;
+  is.back_cull_emission = st.back_cull_emission;
+  is.n_iters = st.n_iters;
+  is.constant_prob = st.constant_prob;
+  is.mf_exp = st.multiplier_func_exp;
+  is.mf_min_prob = st.multiplier_func_min_prob;
+  is.normalize_min_prob = st.normalize_min_prob;
+  return is;
+}
+
 } // namespace
 
 extern "C" {
@@ -407,6 +548,88 @@ int oracle_work_division(const pt_work_division_settings *s, uint32_t spp,
   return int(total);
 }
 
+// FP64 forward-mode gradients of sum(d_image * image) for the mega-kernel
+// estimator with the same samples: grad_albedo / grad_emission per material
+// (n_mats * 3 doubles each; albedo = diffuse for Diffuse materials, specular
+// otherwise) and the FP64 image (x_dim * y_dim * 3). Returns 0, -1 on bad
+// settings, -2 if a direction sampler depended on a color.
+int oracle_adjoint(const float *tris, const float *normals, const uint32_t *tri_mat,
+                   uint32_t n_tris, const pt_material *mats, uint32_t n_mats,
+                   const float *light_tris, const float *light_cdf, uint32_t n_lights,
+                   const float *film_to_world, const pt_settings *settings,
+                   uint32_t samples_per, uint32_t x_dim, uint32_t y_dim,
+                   const float *sobol, const double *d_image, int n_threads,
+                   double *grad_albedo, double *grad_emission, double *image,
+                   double *seconds) {
+  if (n_mats == 0 || n_mats > kOracleMaxMats)
+    return -1;
+  WorkDivision wd{};
+  wd.x_dim = x_dim;
+  wd.y_dim = y_dim;
+  IntegratorSettings is = integrator_from(*settings);
+  uint32_t dim_bound = settings->max_sample_size;
+  uint32_t init_bound = dim_bound < 64 ? dim_bound : 64;
+  OScene<double> sc;
+  fill_scene(sc, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris, light_cdf,
+             n_lights);
+  RenderArgs<double> a{&sc, is, wd, {}, sobol, dim_bound, init_bound, samples_per,
+                       settings->rng};
+  for (int i = 0; i < 12; ++i)
+    a.film_to_world[i] = film_to_world[i];
+  if (n_threads > 0)
+    omp_set_num_threads(n_threads);
+  const uint32_t n_pixels = x_dim * y_dim;
+  const int max_t = omp_get_max_threads();
+  std::vector<double> ga(size_t(max_t) * n_mats * 3, 0.0), ge(size_t(max_t) * n_mats * 3, 0.0);
+  std::atomic<int> bad{0};
+  double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t pix = 0; pix < int64_t(n_pixels); ++pix) {
+    const int th = omp_get_thread_num();
+    double *g_alb = ga.data() + size_t(th) * n_mats * 3;
+    double *g_em = ge.data() + size_t(th) * n_mats * 3;
+    const uint32_t px = uint32_t(pix) % x_dim, py = uint32_t(pix) / x_dim;
+    const uint32_t location = uint32_t(pix);
+    const double d_i[3] = {d_image[3 * pix] / samples_per, d_image[3 * pix + 1] / samples_per,
+                           d_image[3 * pix + 2] / samples_per};
+    V3d sum = {0.0, 0.0, 0.0};
+    for (uint32_t sample = 0; sample < samples_per; ++sample) {
+      V3d rad;
+      bool ok;
+      if (settings->rng == kRngSobel)
+        ok = adjoint_path(a, px, py,
+                          TableRngT<double>{sobol, sample * dim_bound,
+                                            initial_dimension(location, init_bound), dim_bound},
+                          d_i, n_mats, g_alb, g_em, &rad);
+      else
+        ok = adjoint_path(a, px, py,
+                          HashRngT<double>{make_hash_rng(sample + location * samples_per)},
+                          d_i, n_mats, g_alb, g_em, &rad);
+      if (!ok)
+        bad = 1;
+      sum += rad;
+    }
+    if (image) {
+      image[3 * pix] = sum.x / samples_per;
+      image[3 * pix + 1] = sum.y / samples_per;
+      image[3 * pix + 2] = sum.z / samples_per;
+    }
+  }
+  double t1 = omp_get_wtime();
+  for (uint32_t i = 0; i < n_mats * 3; ++i) {
+    double x = 0.0, y = 0.0;
+    for (int t = 0; t < max_t; ++t) {
+      x += ga[size_t(t) * n_mats * 3 + i];
+      y += ge[size_t(t) * n_mats * 3 + i];
+    }
+    grad_albedo[i] = x;
+    grad_emission[i] = y;
+  }
+  if (seconds)
+    *seconds = t1 - t0;
+  return bad ? -2 : 0;
+}
+
 // Mega-kernel-equivalent render. precision: 32 or 64. Output out32 (float) or
 // out64 (double), x_dim*y_dim*3. Counts per render: rays traced, BVH nodes
 // visited and triangles tested by the reference traversal.
@@ -433,17 +656,7 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
   WorkDivision wd;
   if (!WorkDivision::make(ws, samples_per, x_dim, y_dim, &wd))
     return -1;
-  IntegratorSettings is{};
-  is.light_sampler = settings->light_sampler;
-  is.dir_sampler = settings->dir_sampler;
-  is.term_prob = settings->term_prob;
-  is.rng = settings->rng;
-  is.back_cull_emission = settings->back_cull_emission;
-  is.n_iters = settings->n_iters;
-  is.constant_prob = settings->constant_prob;
-  is.mf_exp = settings->multiplier_func_exp;
-  is.mf_min_prob = settings->multiplier_func_min_prob;
-  is.normalize_min_prob = settings->normalize_min_prob;
+  IntegratorSettings is = integrator_from(*settings);
   uint32_t dim_bound = settings->max_sample_size;
   uint32_t init_bound = dim_bound < 64 ? dim_bound : 64;
   uint64_t c[6] = {0, 0, 0, 0, 0, 0};

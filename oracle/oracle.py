@@ -37,6 +37,9 @@ def lib():
                                     _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                     C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
                                     C.c_uint32, _F, _D, _U64, _D]
+        l.oracle_adjoint.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
+                                     _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
+                                     C.c_uint32, C.c_uint32, _F, _D, C.c_int, _D, _D, _D, _D]
         l.oracle_max_threads.restype = C.c_int
         l.oracle_bsdf_sample.argtypes = [C.POINTER(L.PtMaterial), _D, _D, _D, C.c_uint32,
                                          C.c_int, _D]
@@ -123,6 +126,32 @@ def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_
         raise ValueError("oracle render failed")
     info = {"counts": counts.reshape(2, 3), "seconds": secs.value}
     return (out64 if precision == 64 else out32), info
+
+
+def adjoint(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
+            sobol: np.ndarray | None, d_image: np.ndarray, threads: int = 0):
+    """FP64 forward-mode gradients of sum(d_image * image) w.r.t. every
+    material's albedo color and emission for the mega-kernel estimator with
+    the same samples. Returns (grad_albedo (n, 3), grad_emission (n, 3),
+    image (y, x, 3) FP64, seconds)."""
+    raw = settings.raw if hasattr(settings, "raw") else settings
+    ftw = np.ascontiguousarray(film_to_world, np.float32)
+    sob = np.zeros(1, np.float32) if sobol is None else np.ascontiguousarray(sobol, np.float32)
+    d_img = np.ascontiguousarray(d_image, np.float64).reshape(y_dim, x_dim, 3)
+    nm = flat["n_materials"]
+    ga = np.zeros((nm, 3), np.float64)
+    ge = np.zeros((nm, 3), np.float64)
+    img = np.zeros((y_dim, x_dim, 3), np.float64)
+    secs = C.c_double()
+    rc = lib().oracle_adjoint(
+        _f(flat["tris"]), _f(flat["normals"]), _u(flat["tri_mat"]), flat["tris"].shape[0],
+        flat["materials"], nm, _f(flat["light_tris"]), _f(flat["light_cdf"]),
+        flat["light_tris"].shape[0], _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
+        d_img.ctypes.data_as(_D), threads, ga.ctypes.data_as(_D), ge.ctypes.data_as(_D),
+        img.ctypes.data_as(_D), C.byref(secs))
+    if rc != 0:
+        raise ValueError(f"oracle adjoint failed ({rc})")
+    return ga, ge, img, secs.value
 
 
 def max_threads() -> int:

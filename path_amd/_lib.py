@@ -141,6 +141,9 @@ SIGNATURES = {
     "pt_set_launch_params": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_int]),
     "pt_autotune_batch": (C.c_int, [_P, C.POINTER(PtBatchItem), C.c_uint32, C.c_uint32,
                                     C.POINTER(PtSettings), _U]),
+    "pt_scene_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(PtMaterial)]),
+    "pt_render_adjoint": (C.c_int, [_P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.POINTER(PtSettings), _F, _F, _F, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -45,8 +45,10 @@ SOURCES = [
     "host/streaming.cpp",
     "host/batch.cpp",
     "host/scene_gen.cpp",
+    "host/adjoint.cpp",
     "gpu/megakernel.hip",
     "gpu/wavefront.hip",
+    "gpu/adjoint.hip",
 ]
 
 

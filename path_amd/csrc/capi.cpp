@@ -70,7 +70,7 @@ int pt_scene_load(const char *path, float width_height_ratio, int quiet,
   (void)quiet;
   if (film_to_world)
     std::memcpy(film_to_world, cam.film_to_world.m, 12 * sizeof(float));
-  sc->generation++;
+  sc->generation = next_generation();
   *out = sc.release();
   return PT_OK;
   PT_GUARD_END
@@ -100,6 +100,30 @@ static pt_material to_pt(const MaterialParams &p) {
   m.glossy_weight = p.glossy_weight;
   m.emission[0] = p.emission.x, m.emission[1] = p.emission.y, m.emission[2] = p.emission.z;
   return m;
+}
+
+int pt_scene_set_material(pt_scene *scene, uint32_t idx, const pt_material *m) {
+  PT_GUARD_BEGIN
+  if (!scene || !m)
+    return fail(PT_ERR_INVALID, "null argument");
+  auto &mats = scene->scene.materials;
+  if (idx >= mats.size())
+    return fail(PT_ERR_INVALID, "material index " + std::to_string(idx) +
+                                    " out of range (" + std::to_string(mats.size()) + ")");
+  MaterialParams p = from_pt(*m);
+  auto emissive = [](const MaterialParams &q) {
+    return q.emission.x != 0.f || q.emission.y != 0.f || q.emission.z != 0.f;
+  };
+  if (emissive(p) != emissive(mats[idx]))
+    return fail(PT_ERR_UNSUPPORTED,
+                "a material cannot switch between emissive and non-emissive "
+                "(the emitter list is fixed when the mesh is added)");
+  if (p.type > 5)
+    return fail(PT_ERR_INVALID, "unknown BSDF type");
+  mats[idx] = p;
+  scene->mat_generation = next_generation();
+  return PT_OK;
+  PT_GUARD_END
 }
 
 static Affine affine_from(const float *t) {
@@ -140,7 +164,7 @@ int pt_scene_add_mesh(pt_scene *scene, const float *verts, const float *normals,
             normalize(V3f{normals[9 * i + 3 * k], normals[9 * i + 3 * k + 1],
                           normals[9 * i + 3 * k + 2]});
   }
-  scene->generation++;
+  scene->generation = next_generation();
   return PT_OK;
   PT_GUARD_END
 }
@@ -152,7 +176,7 @@ int pt_scene_add_obj(pt_scene *scene, const char *path, const float *transform) 
   std::string err;
   if (!load_obj_mesh(scene->scene, path, affine_from(transform), &err))
     return fail(PT_ERR_IO, err);
-  scene->generation++;
+  scene->generation = next_generation();
   return PT_OK;
   PT_GUARD_END
 }
@@ -363,7 +387,7 @@ int pt_scene_generate(uint64_t seed, pt_scene **out, float film_to_world[12],
   GeneratedScene g = generate_scene(seed);
   auto sc = std::make_unique<pt_scene>();
   sc->scene = std::move(g.scene);
-  sc->generation++;
+  sc->generation = next_generation();
   if (film_to_world)
     std::memcpy(film_to_world, g.camera.film_to_world.m, 12 * sizeof(float));
   if (resolution)
@@ -401,6 +425,18 @@ int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
   if (!r || !items || !n || !settings || !out)
     return fail(PT_ERR_INVALID, "null argument");
   return autotune_batch(*r, items, n, samples_per, *settings, out);
+  PT_GUARD_END
+}
+
+int pt_render_adjoint(pt_renderer *r, const pt_scene *scene, const float film_to_world[12],
+                      uint32_t x_dim, uint32_t y_dim, uint32_t samples_per,
+                      const pt_settings *settings, const float *d_image,
+                      float *grad_albedo, float *grad_emission, double *seconds) {
+  PT_GUARD_BEGIN
+  if (!r || !scene || !settings)
+    return fail(PT_ERR_INVALID, "null argument");
+  return render_adjoint(*r, *scene, film_to_world, x_dim, y_dim, samples_per, *settings,
+                        d_image, grad_albedo, grad_emission, seconds);
   PT_GUARD_END
 }
 

@@ -438,6 +438,17 @@ template <class T> PT_INLINE bool within_target(T target, T t) {
 }
 
 // Surface data of the vertex shaded by one iteration (G-buffer output).
+// What one iteration added, for derivative replays: the emission terms
+// (material whose emission was scaled, and the multiplier it was scaled by)
+// and the material of the vertex shaded (kNoMaterial if none).
+constexpr uint32_t kNoMaterial = 0xffffffffu;
+template <class T> struct IterTape {
+  uint32_t n_emit;
+  uint32_t emit_mat[2];
+  V3<T> emit_mult[2];
+  uint32_t vertex_mat;
+};
+
 template <class T> struct VertexInfo {
   V3<T> p;
   V3<T> normal;
@@ -470,7 +481,8 @@ PT_INLINE bool rendering_equation_iteration(
     const PathState<T> &st, R &rng, const Ray<T> &last_ray,
     const Hit<T> &light_hit, const Hit<T> &next_hit,
     const IntegratorSettings &s, const SceneView<T> &sc, PathState<T> *out,
-    Ray<T> *light_ray, Ray<T> *next_ray, VertexInfo<T> *vertex = nullptr) {
+    Ray<T> *light_ray, Ray<T> *next_ray, VertexInfo<T> *vertex = nullptr,
+    IterTape<T> *tape = nullptr) {
   PathState<T> ns;
   ns.iters = st.iters + 1;
   ns.total = st.total;
@@ -483,11 +495,21 @@ PT_INLINE bool rendering_equation_iteration(
   ns.count_emission = 0;
   ns.has_next = 0;
 
+  if (tape) {
+    tape->n_emit = 0;
+    tape->vertex_mat = kNoMaterial;
+  }
   if (st.n_light && light_hit.idx != kNoHit &&
       within_target(st.light_target, light_hit.t) &&
-      !(s.back_cull_emission && light_hit.back))
-    ns.total += sc.materials[sc.tris[light_hit.idx].material].emission *
-                st.light_mult;
+      !(s.back_cull_emission && light_hit.back)) {
+    const uint32_t lm = sc.tris[light_hit.idx].material;
+    ns.total += sc.materials[lm].emission * st.light_mult;
+    if (tape) {
+      tape->emit_mat[tape->n_emit] = lm;
+      tape->emit_mult[tape->n_emit] = st.light_mult;
+      tape->n_emit++;
+    }
+  }
 
   if (!st.has_next) {
     *out = ns;
@@ -507,8 +529,16 @@ PT_INLINE bool rendering_equation_iteration(
   const Material<T> mat = sc.materials[tri.material];
   bool performs_samples = s.light_sampler != kLightNone;
   if ((!performs_samples || st.count_emission) &&
-      !(s.back_cull_emission && h.back))
+      !(s.back_cull_emission && h.back)) {
     ns.total += mult * mat.emission;
+    if (tape) {
+      tape->emit_mat[tape->n_emit] = tri.material;
+      tape->emit_mult[tape->n_emit] = mult;
+      tape->n_emit++;
+    }
+  }
+  if (tape)
+    tape->vertex_mat = tri.material;
 
   const TriGeom<T> g = {tri.v0, tri.e1, tri.e2};
   const V3<T> vn[3] = {tri.n0, tri.n1, tri.n2};

@@ -1,0 +1,162 @@
+// Path-replay adjoint (reverse mode) of the mega-kernel estimator.
+//
+// Each thread owns a run of samples of one pixel. For every sample it replays
+// the path from its Sobol coordinates twice: the first replay yields the
+// path's radiance L; the second walks the same vertices again and
+// back-propagates dI (the loss gradient of the pixel):
+//   - an emission term  mult * Le[m]  adds  dI * mult  to dLe[m];
+//   - a vertex with material m scales every later term by albedo[m] exactly
+//     once (single-lobe BSDFs: sample weight and NEE evaluation are linear in
+//     the albedo color), so it adds  dI * (L - radiance so far) / albedo[m]
+//     to dAlbedo[m] (channels with zero albedo are skipped).
+// Nothing is stored per vertex. Gradients accumulate in per-thread LDS slots
+// (no atomics), are summed per block in thread order and finally over blocks
+// in FP64, so the result is deterministic.
+#include "kernels.h"
+
+#include "../core/camera.h"
+#include "../core/rng.h"
+#include "trace.h"
+
+namespace pt {
+namespace gpu {
+
+template <class R>
+__device__ __forceinline__ R adjoint_rng(const RngParams &rp, uint32_t location,
+                                         uint32_t sample) {
+  if constexpr (sizeof(R) == sizeof(TableRng))
+    return TableRng{rp.vals, sample * rp.dim_bound,
+                    initial_dimension(location, rp.initial_dim_bound), rp.dim_bound};
+  else
+    return make_hash_rng(sample + location * rp.samples_per);
+}
+
+// One replay of a sample. kBackprop = false: returns the radiance. kBackprop =
+// true: accumulates into acc[param * stride] (this thread's slots).
+template <class R, bool kBackprop>
+__device__ __forceinline__ V3<float>
+replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
+              V3<float> radiance, V3<float> d_i, float *acc, uint32_t stride) {
+  const uint32_t location = px + py * p.x_dim;
+  R rng = adjoint_rng<R>(p.rng, location, sample);
+  float xo = rng.next();
+  float yo = rng.next();
+  Ray<float> next_ray =
+      camera_ray(p.film_to_world, float(px) + xo, float(py) + yo, p.x_dim, p.y_dim);
+  Ray<float> light_ray = next_ray;
+  PathState<float> st = initial_path_state<float>();
+  const uint32_t n3 = 3 * p.n_mats;
+  for (;;) {
+    Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
+    if (st.n_light) {
+      PrivateStack stack;
+      light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
+                                st.light_target + 2e-6f, st.light_target - 2e-6f, stack);
+    }
+    if (st.has_next) {
+      PrivateStack stack;
+      next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray, 3.4028235e38f,
+                               -1.0f, stack);
+    }
+    PathState<float> ns;
+    Ray<float> nl = light_ray, nn = next_ray;
+    IterTape<float> tape;
+    VertexInfo<float> *no_vertex = nullptr;
+    IterTape<float> *tape_ptr = kBackprop ? &tape : nullptr;
+    bool done = rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
+                                             p.settings, p.scene.view, &ns, &nl, &nn,
+                                             no_vertex, tape_ptr);
+    if constexpr (kBackprop) {
+      for (uint32_t j = 0; j < tape.n_emit; ++j) {
+        float *e = acc + (n3 + 3 * tape.emit_mat[j]) * stride;
+        e[0] += d_i.x * tape.emit_mult[j].x;
+        e[stride] += d_i.y * tape.emit_mult[j].y;
+        e[2 * stride] += d_i.z * tape.emit_mult[j].z;
+      }
+      if (tape.vertex_mat != kNoMaterial) {
+        V3<float> rho = material_albedo(p.scene.view.materials[tape.vertex_mat]);
+        V3<float> rest = radiance - ns.total;
+        float *a = acc + 3 * tape.vertex_mat * stride;
+        if (rho.x != 0.f)
+          a[0] += d_i.x * rest.x / rho.x;
+        if (rho.y != 0.f)
+          a[stride] += d_i.y * rest.y / rho.y;
+        if (rho.z != 0.f)
+          a[2 * stride] += d_i.z * rest.z / rho.z;
+      }
+    }
+    if (done)
+      return ns.total;
+    st = ns;
+    light_ray = nl;
+    next_ray = nn;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
+  extern __shared__ float acc_all[];
+  const uint32_t stride = kAdjointBlock + 1; // odd stride: conflict-free sums
+  const uint32_t n_params = 6 * p.n_mats;
+  float *acc = acc_all + threadIdx.x;
+  for (uint32_t i = 0; i < n_params; ++i)
+    acc[i * stride] = 0.f;
+  const uint32_t t = blockIdx.x * kAdjointBlock + threadIdx.x;
+  if (t < p.n_threads) {
+    const uint32_t pixel = t / p.threads_per_pixel;
+    const uint32_t part = t % p.threads_per_pixel;
+    const uint32_t px = pixel % p.x_dim, py = pixel / p.x_dim;
+    const uint32_t spp = p.rng.samples_per;
+    const uint32_t s0 = part * p.samples_per_thread;
+    const uint32_t s1 = s0 + p.samples_per_thread < spp ? s0 + p.samples_per_thread : spp;
+    const V3<float> d_i = {p.d_image[3 * size_t(pixel)], p.d_image[3 * size_t(pixel) + 1],
+                           p.d_image[3 * size_t(pixel) + 2]};
+    for (uint32_t sample = s0; sample < s1; ++sample) {
+      V3<float> zero = {0.f, 0.f, 0.f};
+      V3<float> radiance =
+          replay_sample<R, false>(p, px, py, sample, zero, zero, nullptr, stride);
+      replay_sample<R, true>(p, px, py, sample, radiance, d_i, acc, stride);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_params; i += kAdjointBlock) {
+    float sum = 0.f;
+    const float *row = acc_all + i * stride;
+    for (uint32_t j = 0; j < kAdjointBlock; ++j)
+      sum += row[j];
+    p.partials[size_t(blockIdx.x) * n_params + i] = sum;
+  }
+}
+
+hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s) {
+  const uint32_t blocks = (p.n_threads + kAdjointBlock - 1) / kAdjointBlock;
+  const size_t lds = size_t(6) * p.n_mats * (kAdjointBlock + 1) * sizeof(float);
+  if (p.rng.type == kRngSobel)
+    hipLaunchKernelGGL(k_adjoint<TableRng>, dim3(blocks), dim3(kAdjointBlock), lds, s, p);
+  else
+    hipLaunchKernelGGL(k_adjoint<HashRng>, dim3(blocks), dim3(kAdjointBlock), lds, s, p);
+  return hipGetLastError();
+}
+
+__global__ void k_adjoint_finalize(const float *__restrict__ partials, uint32_t n_blocks,
+                                   uint32_t n_params, double scale,
+                                   float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_params)
+    return;
+  double sum = 0.0;
+  for (uint32_t b = 0; b < n_blocks; ++b)
+    sum += double(partials[size_t(b) * n_params + i]);
+  out[i] = float(sum * scale);
+}
+
+hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
+                                   uint32_t n_params, double scale, float *out,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint_finalize, dim3((n_params + 63) / 64), dim3(64), 0, s,
+                     partials, n_blocks, n_params, scale, out);
+  return hipGetLastError();
+}
+
+} // namespace gpu
+} // namespace pt

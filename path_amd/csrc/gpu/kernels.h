@@ -41,6 +41,25 @@ struct MegaParams {
   unsigned long long *ray_counter;
 };
 
+// Path-replay adjoint of a SquareImage render: gradients of sum(dI * I) with
+// respect to every material's albedo color (material_albedo) and emission.
+constexpr uint32_t kMaxAdjointMaterials = 40; // LDS: 6 * 40 * 65 floats
+constexpr uint32_t kAdjointBlock = 64;
+
+struct AdjointParams {
+  DeviceScene scene;
+  IntegratorSettings settings;
+  RngParams rng;
+  float film_to_world[12];
+  uint32_t x_dim, y_dim;
+  uint32_t samples_per_thread; // consecutive samples of one pixel per thread
+  uint32_t threads_per_pixel;
+  uint32_t n_threads;          // pixels * threads_per_pixel
+  uint32_t n_mats;
+  const float *d_image;        // dLoss/dI, (x + y * x_dim) * 3 + c
+  float *partials;             // [block][6 * n_mats]: albedo then emission
+};
+
 // Launchers (return hipError_t of the launch).
 hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
                               uint32_t count, float *out, hipStream_t s);
@@ -56,6 +75,12 @@ hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
                                        float *out, hipStream_t s);
 hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
                           hipStream_t s);
+hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s);
+// out[i] = scale * sum over blocks of partials[block][i] (FP64 accumulate, in
+// block order).
+hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
+                                   uint32_t n_params, double scale, float *out,
+                                   hipStream_t s);
 
 } // namespace gpu
 } // namespace pt

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <vector>
 
 using namespace pt;
@@ -20,10 +21,16 @@ using namespace pt::host;
                                        hipGetErrorString(e_));                 \
   } while (0)
 
+uint64_t pt::host::next_generation() {
+  static std::atomic<uint64_t> counter{1};
+  return counter.fetch_add(1);
+}
+
 const FlatScene &pt_scene::flat() const {
-  if (flat_gen_ != generation) {
+  if (flat_gen_ != generation || flat_mat_gen_ != mat_generation) {
     flat_ = flatten(scene);
     flat_gen_ = generation;
+    flat_mat_gen_ = mat_generation;
   }
   return flat_;
 }
@@ -37,16 +44,20 @@ const Bvh &pt_scene::bvh() const {
 }
 
 const PackedScene &pt_scene::packed() const {
+  const FlatScene &f = flat();
   if (packed_gen_ != generation) {
-    const FlatScene &f = flat();
     packed_.bvh = pack_gpu_bvh(bvh(), f.tris);
-    packed_.mats.clear();
-    for (const auto &m : f.materials)
-      packed_.mats.push_back(m.to<float>());
     packed_.shade.resize(f.tris.size());
     for (size_t i = 0; i < f.tris.size(); ++i)
       packed_.shade[i] = make_shade_tri(f.tris[i], &f.normals[3 * i], f.tri_mat[i]);
     packed_gen_ = generation;
+  }
+  if (packed_mat_gen_ != mat_generation || packed_mats_of_gen_ != generation) {
+    packed_.mats.clear();
+    for (const auto &m : f.materials)
+      packed_.mats.push_back(m.to<float>());
+    packed_mat_gen_ = mat_generation;
+    packed_mats_of_gen_ = generation;
   }
   return packed_;
 }
@@ -77,6 +88,22 @@ hipError_t DeviceSceneStore::upload(const pt_scene &scene, hipStream_t s) {
   dev.view.light_cdf = light_cdf.ptr;
   dev.view.n_lights = uint32_t(f.light_tris.size());
   max_depth = g.max_depth;
+  return hipStreamSynchronize(s);
+}
+
+hipError_t DeviceSceneStore::upload_materials(const pt_scene &scene, hipStream_t s) {
+  const FlatScene &f = scene.flat();
+  const PackedScene &pk = scene.packed();
+  hipError_t e;
+  if ((e = materials.upload(pk.mats.data(), pk.mats.size(), s)) != hipSuccess ||
+      (e = light_tris.upload(f.light_tris.data(), f.light_tris.size(), s)) !=
+          hipSuccess ||
+      (e = light_cdf.upload(f.light_cdf.data(), f.light_cdf.size(), s)) != hipSuccess)
+    return e;
+  dev.view.materials = materials.ptr;
+  dev.view.light_tris = light_tris.ptr;
+  dev.view.light_cdf = light_cdf.ptr;
+  dev.view.n_lights = uint32_t(f.light_tris.size());
   return hipStreamSynchronize(s);
 }
 
@@ -115,8 +142,14 @@ bool pt_renderer::init(int dev, std::string *err) {
 }
 
 int pt_renderer::ensure_scene(const pt_scene &scene) {
-  if (cached_scene == &scene && cached_gen == scene.generation)
+  if (cached_scene == &scene && cached_gen == scene.generation) {
+    if (cached_mat_gen == scene.mat_generation)
+      return PT_OK;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(scene_store.upload_materials(scene, stream));
+    cached_mat_gen = scene.mat_generation;
     return PT_OK;
+  }
   const Bvh &b = scene.bvh();
   if (b.max_depth + 1 >= uint32_t(gpu_stack_size()))
     return set_error(PT_ERR_UNSUPPORTED,
@@ -126,6 +159,7 @@ int pt_renderer::ensure_scene(const pt_scene &scene) {
   HIP_TRY(scene_store.upload(scene, stream));
   cached_scene = &scene;
   cached_gen = scene.generation;
+  cached_mat_gen = scene.mat_generation;
   return PT_OK;
 }
 

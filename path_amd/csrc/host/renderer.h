@@ -31,18 +31,30 @@ struct PackedScene {
 } // namespace host
 } // namespace pt
 
-// Caches are rebuilt when `generation` changes; distinct scenes may be built
-// on different threads at once.
+namespace pt {
+namespace host {
+// Process-wide unique stamp for scene versions (never reused, so a renderer's
+// cache cannot mistake a new scene at a recycled address for an old one).
+uint64_t next_generation();
+} // namespace host
+} // namespace pt
+
+// Caches are rebuilt when `generation` (geometry and materials) or
+// `mat_generation` (material values only) changes; distinct scenes may be
+// built on different threads at once.
 struct pt_scene {
   pt::host::Scene scene;
-  uint64_t generation = 0;
+  uint64_t generation = pt::host::next_generation();
+  uint64_t mat_generation = 0;
 
   const pt::host::FlatScene &flat() const;
   const pt::host::Bvh &bvh() const;
   const pt::host::PackedScene &packed() const;
 
 private:
-  mutable uint64_t flat_gen_ = ~0ull, bvh_gen_ = ~0ull, packed_gen_ = ~0ull;
+  mutable uint64_t flat_gen_ = ~0ull, flat_mat_gen_ = ~0ull, bvh_gen_ = ~0ull,
+                   packed_gen_ = ~0ull, packed_mat_gen_ = ~0ull,
+                   packed_mats_of_gen_ = ~0ull;
   mutable pt::host::FlatScene flat_;
   mutable pt::host::Bvh bvh_;
   mutable pt::host::PackedScene packed_;
@@ -94,6 +106,8 @@ struct DeviceSceneStore {
   uint32_t max_depth = 0;
 
   hipError_t upload(const pt_scene &scene, hipStream_t s);
+  // Material values and the emitter CDF only (geometry unchanged).
+  hipError_t upload_materials(const pt_scene &scene, hipStream_t s);
 };
 
 // Grow-only pinned host allocation (staging for batched copies).
@@ -134,7 +148,7 @@ struct pt_renderer {
 
   // scene cache
   const pt_scene *cached_scene = nullptr;
-  uint64_t cached_gen = ~0ull;
+  uint64_t cached_gen = ~0ull, cached_mat_gen = ~0ull;
   pt::host::DeviceSceneStore scene_store;
 
   // sampler cache
@@ -173,6 +187,9 @@ struct pt_renderer {
   pt::host::DeviceBuf<unsigned long long> b_sample_begin;
   pt::host::DeviceBuf<float> b_gbuffer;
   pt::host::PinnedBuf b_stage_in, b_stage_out;
+
+  // adjoint buffers
+  pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
   uint32_t trace_blocks_per_cu = 8;
@@ -202,6 +219,10 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                  uint32_t spp, const pt_settings &settings, double *seconds);
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[4]);
+int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_world[12],
+                   uint32_t x_dim, uint32_t y_dim, uint32_t spp,
+                   const pt_settings &settings, const float *d_image, float *grad_albedo,
+                   float *grad_emission, double *seconds);
 // Wavefront ("Streaming") integrator; writes the final per-pixel FloatRGB
 // (already divided by samples_per) into a device buffer owned by the renderer.
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,

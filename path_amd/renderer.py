@@ -101,6 +101,15 @@ class Scene:
         tr = None if transform is None else np.ascontiguousarray(transform, np.float32)
         L.check(L.lib().pt_scene_add_obj(self.handle, str(path).encode(), L.fptr(tr)))
 
+    def set_material(self, idx: int, m: L.PtMaterial) -> None:
+        """Replace material `idx` (geometry and BVH are kept)."""
+        L.check(L.lib().pt_scene_set_material(self.handle, idx, C.byref(m)))
+
+    def materials(self) -> list:
+        """Copies of the scene's materials (PtMaterial)."""
+        f = self.flat()
+        return [L.PtMaterial.from_buffer_copy(f["materials"][i]) for i in range(f["n_materials"])]
+
     def counts(self):
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
         L.check(L.lib().pt_scene_counts(self.handle, C.byref(a), C.byref(b), C.byref(c)))
@@ -223,6 +232,23 @@ class Renderer:
         L.check(L.lib().pt_intersect(self.handle, scene.handle, L.fptr(rays), n, L.fptr(t),
                                      L.uptr(idx), L.uptr(back)))
         return t, idx, back
+
+    def adjoint(self, scene: Scene, film_to_world, x_dim: int, y_dim: int, samples_per: int,
+                d_image: np.ndarray, settings: Settings | None = None):
+        """Path-replay adjoint: gradients of sum(d_image * image) w.r.t. every
+        material's albedo color and emission. Returns (grad_albedo (n, 3),
+        grad_emission (n, 3), seconds)."""
+        settings = settings or Settings()
+        d = np.ascontiguousarray(d_image, np.float32).reshape(y_dim, x_dim, 3)
+        nm = scene.counts()[1]
+        ga = np.zeros((nm, 3), np.float32)
+        ge = np.zeros((nm, 3), np.float32)
+        ftw = np.ascontiguousarray(np.asarray(film_to_world).ravel(), np.float32)
+        secs = C.c_double()
+        L.check(L.lib().pt_render_adjoint(self.handle, scene.handle, L.fptr(ftw), x_dim, y_dim,
+                                          samples_per, C.byref(settings.raw), L.fptr(d),
+                                          L.fptr(ga), L.fptr(ge), C.byref(secs)))
+        return ga, ge, secs.value
 
     def sobol_table(self, dim_bound: int, count: int) -> np.ndarray:
         out = np.zeros((count, dim_bound), np.float32)
