@@ -410,7 +410,19 @@ def main(argv=None):
                     help="wavefront path-pool size (max_num_samples_per_launch)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c3: keep the default launch parameters")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (scale for the "
+                         "profiler's FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(256 * 1024 * 1024 // 4, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes_per_launch": x.numel() * 4, "launches": 3}))
+        return 0
     if a.gpus != 1:
         print("bench.py measures one GPU only: the path shards by sample range with a single "
               "image sum-reduce, which is not built here; run with --gpus 1", file=sys.stderr)

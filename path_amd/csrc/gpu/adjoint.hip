@@ -138,23 +138,33 @@ hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s) {
   return hipGetLastError();
 }
 
-__global__ void k_adjoint_finalize(const float *__restrict__ partials, uint32_t n_blocks,
-                                   uint32_t n_params, double scale,
-                                   float *__restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_params)
-    return;
+// One block per parameter: each thread sums a fixed stride of blocks in FP64,
+// then a fixed-shape tree over the threads (deterministic).
+__global__ __launch_bounds__(256) void k_adjoint_finalize(const float *__restrict__ partials,
+                                                          uint32_t n_blocks, uint32_t n_params,
+                                                          double scale,
+                                                          float *__restrict__ out) {
+  __shared__ double sums[256];
+  const uint32_t i = blockIdx.x;
   double sum = 0.0;
-  for (uint32_t b = 0; b < n_blocks; ++b)
+  for (uint32_t b = threadIdx.x; b < n_blocks; b += 256u)
     sum += double(partials[size_t(b) * n_params + i]);
-  out[i] = float(sum * scale);
+  sums[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 128u; off >= 1u; off >>= 1) {
+    if (threadIdx.x < off)
+      sums[threadIdx.x] += sums[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[i] = float(sums[0] * scale);
 }
 
 hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
                                    uint32_t n_params, double scale, float *out,
                                    hipStream_t s) {
-  hipLaunchKernelGGL(k_adjoint_finalize, dim3((n_params + 63) / 64), dim3(64), 0, s,
-                     partials, n_blocks, n_params, scale, out);
+  hipLaunchKernelGGL(k_adjoint_finalize, dim3(n_params), dim3(256), 0, s, partials, n_blocks,
+                     n_params, scale, out);
   return hipGetLastError();
 }
 
