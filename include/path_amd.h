@@ -316,6 +316,49 @@ int pt_render_adjoint(pt_renderer *r, const pt_scene *scene,
                       const pt_settings *settings, const float *d_image,
                       float *grad_albedo, float *grad_emission, double *seconds);
 
+/* ------------------------------------------------------------------------ */
+/* Radiance MLP (no reference counterpart: the reference trains a different  */
+/* network in PyTorch, neural_render/src/model.py:276-351). 9 features       */
+/* (position, shading normal, albedo) -> 12-frequency encoding (225, padded  */
+/* to 256) -> 6 ReLU layers of width 256 -> 3 outputs (reflected radiance).  */
+/* Hand-written BF16 MFMA kernels with FP32 accumulation; FP32 master        */
+/* weights and Adam. Weights: 7 row-major [out][in] matrices, in order, with */
+/* (in, out) = (256, 256) x 6 and (256, 16); no biases.                      */
+/* ------------------------------------------------------------------------ */
+typedef struct pt_mlp pt_mlp;
+int pt_mlp_create(int device, uint64_t seed, pt_mlp **out);
+void pt_mlp_destroy(pt_mlp *m);
+int pt_mlp_param_count(uint64_t *count);
+int pt_mlp_get_weights(pt_mlp *m, float *weights);
+/* Also resets the Adam state. */
+int pt_mlp_set_weights(pt_mlp *m, const float *weights);
+/* features: n x 9 (host), out: n x 3 (host). */
+int pt_mlp_forward(pt_mlp *m, const float *features, uint32_t n, float *out);
+/* Gradient of the relative L2 loss mean((p - t)^2 / (p^2 + 0.01)) (p held
+ * constant in the denominator) over one batch; grad has pt_mlp_param_count
+ * floats. */
+int pt_mlp_gradient(pt_mlp *m, const float *features, const float *targets,
+                    uint32_t n, float *grad, float *loss);
+/* One Adam step (beta 0.9 / 0.99, eps 1e-8) on one host batch. */
+int pt_mlp_train_step(pt_mlp *m, const float *features, const float *targets,
+                      uint32_t n, float lr, float *loss);
+/* Keep a training set resident on the device (rows x 9 features, rows x 3
+ * targets), then run `steps` Adam steps on batches drawn from it on the
+ * device; losses (may be NULL) receives one loss per step. */
+int pt_mlp_set_dataset(pt_mlp *m, const float *features, const float *targets,
+                       uint64_t rows);
+int pt_mlp_train(pt_mlp *m, uint32_t steps, uint32_t batch, float lr,
+                 uint64_t seed, float *losses, double *seconds);
+
+/* Radiance-cache render of a SquareImage (FloatRGB, host out_rgb x*y*3):
+ * each sample traces the camera ray and one bounce with the light sample of
+ * the first vertex; the radiance reflected at the second vertex is the MLP's
+ * prediction for that point (clamped at 0). Same samples as pt_render. */
+int pt_render_mlp(pt_renderer *r, pt_mlp *m, const pt_scene *scene,
+                  const float film_to_world[12], uint32_t x_dim, uint32_t y_dim,
+                  uint32_t samples_per, const pt_settings *settings,
+                  float *out_rgb, double *seconds);
+
 /* Statistics of the last pt_render. kernel_ms spans the launches with HIP
  * events on the renderer's stream. The per-kernel fields are filled only when
  * profiling is enabled (pt_set_profiling), which brackets each traversal or

@@ -144,6 +144,19 @@ SIGNATURES = {
     "pt_scene_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(PtMaterial)]),
     "pt_render_adjoint": (C.c_int, [_P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.POINTER(PtSettings), _F, _F, _F, C.POINTER(C.c_double)]),
+    "pt_mlp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "pt_mlp_destroy": (None, [_P]),
+    "pt_mlp_param_count": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "pt_mlp_get_weights": (C.c_int, [_P, _F]),
+    "pt_mlp_set_weights": (C.c_int, [_P, _F]),
+    "pt_mlp_forward": (C.c_int, [_P, _F, C.c_uint32, _F]),
+    "pt_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_uint32, _F, _F]),
+    "pt_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_float, _F]),
+    "pt_mlp_set_dataset": (C.c_int, [_P, _F, _F, C.c_uint64]),
+    "pt_mlp_train": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F,
+                               C.POINTER(C.c_double)]),
+    "pt_render_mlp": (C.c_int, [_P, _P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.POINTER(PtSettings), _F, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -46,9 +46,13 @@ SOURCES = [
     "host/batch.cpp",
     "host/scene_gen.cpp",
     "host/adjoint.cpp",
+    "mlp/mlp.cpp",
+    "mlp/nrc.cpp",
     "gpu/megakernel.hip",
     "gpu/wavefront.hip",
     "gpu/adjoint.hip",
+    "mlp/mlp_kernels.hip",
+    "gpu/nrc.hip",
 ]
 
 

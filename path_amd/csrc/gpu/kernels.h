@@ -60,6 +60,23 @@ struct AdjointParams {
   float *partials;             // [block][6 * n_mats]: albedo then emission
 };
 
+// Radiance-cache estimator: the path is traced for one bounce and the
+// radiance reflected at the second vertex comes from a learned model. Sample i
+// (= pixel * spp + s) records the radiance gathered before that vertex
+// (camera-hit emission and the first vertex's light sample), the path
+// multiplier reaching the vertex (zero if the bounce escaped or the path was
+// terminated) and the vertex features (position, shading normal, albedo).
+struct NrcParams {
+  DeviceScene scene;
+  IntegratorSettings settings;
+  RngParams rng;
+  float film_to_world[12];
+  uint32_t x_dim, y_dim;
+  float *base;     // [n][3]
+  float *weight;   // [n][3]
+  float *features; // [n][9]
+};
+
 // Launchers (return hipError_t of the launch).
 hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
                               uint32_t count, float *out, hipStream_t s);
@@ -76,6 +93,12 @@ hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
 hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
                           hipStream_t s);
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s);
+hipError_t launch_nrc_paths(const NrcParams &p, hipStream_t s);
+// out_rgb[pixel] = sum_s (base + weight * max(pred[0..3], 0)) / spp with pred
+// rows of 16 floats.
+hipError_t launch_nrc_combine(const float *base, const float *weight, const float *pred,
+                              uint32_t n_pixels, uint32_t spp, float *out_rgb,
+                              hipStream_t s);
 // out[i] = scale * sum over blocks of partials[block][i] (FP64 accumulate, in
 // block order).
 hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,

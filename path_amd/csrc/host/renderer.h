@@ -190,6 +190,9 @@ struct pt_renderer {
 
   // adjoint buffers
   pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;
+
+  // radiance-cache render buffers
+  pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
   uint32_t trace_blocks_per_cu = 8;

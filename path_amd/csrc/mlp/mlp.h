@@ -1,0 +1,44 @@
+// Radiance MLP state behind the C ABI.
+#pragma once
+
+#include "../host/renderer.h"
+#include "mlp_kernels.h"
+
+struct pt_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t step = 0; // Adam steps taken
+
+  pt::host::DeviceBuf<float> w32, adam_m, adam_v, grad;
+  pt::host::DeviceBuf<pt::mlp::bf16> wbf;
+
+  // per-batch buffers (grow-only, rows padded to kRowAlign)
+  pt::host::DeviceBuf<float> feat, targ, pred, loss_part, split_part;
+  pt::host::DeviceBuf<pt::mlp::bf16> x0, act[pt::mlp::kHidden], d_pred, g0, g1;
+  pt::host::DeviceBuf<uint32_t> idx;
+
+  // resident training set
+  pt::host::DeviceBuf<float> ds_feat, ds_targ;
+  uint64_t ds_rows = 0;
+
+  ~pt_mlp();
+  bool init(int dev, uint64_t seed, std::string *err);
+  int ensure_rows(uint32_t rows_pad, bool training);
+  // Forward on device features (rows_pad rows, padded rows arbitrary):
+  // pred[rows_pad x 16] fp32. training = keep every layer's activations.
+  int forward_device(const float *d_feat, uint32_t rows_pad, bool training,
+                     hipStream_t on = nullptr);
+  // Loss + backward + gradient into `grad` for the batch already in
+  // feat/targ (n valid rows of rows_pad). Returns the mean loss in *loss.
+  int backward(uint32_t n, uint32_t rows_pad, float *loss);
+  int adam(float lr);
+};
+
+namespace pt {
+namespace host {
+inline uint32_t mlp_pad_rows(uint64_t n) {
+  return uint32_t((n + mlp::kRowAlign - 1) / mlp::kRowAlign * mlp::kRowAlign);
+}
+} // namespace host
+} // namespace pt

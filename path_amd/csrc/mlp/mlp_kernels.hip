@@ -1,0 +1,334 @@
+// Radiance MLP kernels: MFMA GEMM (BF16 in, FP32 accumulate), frequency
+// encoding, relative-L2 loss, split-K reduction, Adam.
+#include "mlp_kernels.h"
+
+namespace pt {
+namespace mlp {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// float -> bf16 bits, round to nearest even (inputs are finite).
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+// ---------------------------------------------------------------------------
+// GEMM: 128 x 128 block tile, K step 32, 4 waves in a 2 x 2 grid, each wave a
+// 64 x 64 tile of 2 x 2 v_mfma_f32_32x32x16_bf16. Operands are staged in LDS
+// as [row][k] (k contiguous, row stride 40 bf16 = 80 B); transposed global
+// layouts are transposed on the way in.
+// MFMA lane layout (32x32x16): lane l holds A[l % 32][8 (l / 32) .. +8],
+// B[8 (l / 32) .. +8][l % 32]; accumulator r of lane l is
+// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBM = 128, kBN = 128, kBK = 32, kLdsStride = 40;
+
+template <bool kT>
+__device__ __forceinline__ void load_tile(const uint16_t *__restrict__ src, uint32_t ld,
+                                          uint32_t r0, uint32_t rows, uint32_t k0,
+                                          uint32_t k_end, uint16_t *__restrict__ dst) {
+  // 128 rows x 32 k = 512 chunks of 8 bf16; 256 threads x 2
+  for (uint32_t c = threadIdx.x; c < kBM * kBK / 8; c += 256) {
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (!kT) {
+      const uint32_t row = c >> 2, kc = (c & 3u) * 8u;
+      const uint32_t gr = r0 + row, gk = k0 + kc;
+      if (gr < rows && gk < k_end)
+        v = *reinterpret_cast<const uint4 *>(src + size_t(gr) * ld + gk);
+      *reinterpret_cast<uint4 *>(dst + row * kLdsStride + kc) = v;
+    } else {
+      const uint32_t kr = c >> 4, rc = (c & 15u) * 8u;
+      const uint32_t gk = k0 + kr, gr = r0 + rc;
+      if (gk < k_end && gr < rows)
+        v = *reinterpret_cast<const uint4 *>(src + size_t(gk) * ld + gr);
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        dst[(rc + i) * kLdsStride + kr] = e[i];
+    }
+  }
+}
+
+template <bool kAT, bool kBT, int kEpi>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[kBM * kLdsStride];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN * kLdsStride];
+  const uint32_t m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
+  const uint32_t k_begin = blockIdx.z * g.k_per_split;
+  const uint32_t k_end = min(g.K, k_begin + g.k_per_split);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t wm = wave >> 1, wn = wave & 1u;
+  const uint16_t *A = reinterpret_cast<const uint16_t *>(g.A);
+  const uint16_t *B = reinterpret_cast<const uint16_t *>(g.B);
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        acc[i][j][r] = 0.f;
+  for (uint32_t k0 = k_begin; k0 < k_end; k0 += kBK) {
+    load_tile<kAT>(A, g.lda, m0, g.M, k0, k_end, As);
+    load_tile<kBT>(B, g.ldb, n0, g.N, k0, k_end, Bs);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t kk = 0; kk < kBK; kk += 16) {
+      bf16x8_t a[2], b[2];
+      const uint32_t ko = kk + (lane >> 5) * 8u;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf16x8_t *>(
+            As + (wm * 64u + i * 32u + (lane & 31u)) * kLdsStride + ko);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *reinterpret_cast<const bf16x8_t *>(
+            Bs + (wn * 64u + j * 32u + (lane & 31u)) * kLdsStride + ko);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t row = m0 + wm * 64u + i * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
+        const uint32_t col = n0 + wn * 64u + j * 32u + (lane & 31u);
+        if (row >= g.M || col >= g.N)
+          continue;
+        const float v = acc[i][j][r];
+        const size_t o = size_t(row) * g.ldc + col;
+        if constexpr (kEpi == kEpiReluBf16) {
+          reinterpret_cast<uint16_t *>(g.C)[o] = f2bf(v > 0.f ? v : 0.f);
+        } else if constexpr (kEpi == kEpiF32) {
+          reinterpret_cast<float *>(g.C)[o] = v;
+        } else if constexpr (kEpi == kEpiMaskBf16) {
+          const uint16_t mk = reinterpret_cast<const uint16_t *>(g.mask)[size_t(row) * g.ldm + col];
+          reinterpret_cast<uint16_t *>(g.C)[o] = f2bf(bf2f(mk) > 0.f ? v : 0.f);
+        } else {
+          reinterpret_cast<float *>(g.C)[size_t(blockIdx.z) * g.M * g.N + o] = v;
+        }
+      }
+}
+
+template <bool kAT, bool kBT>
+static void launch_gemm_t(const GemmArgs &g, int epi, dim3 grid, hipStream_t s) {
+  switch (epi) {
+  case kEpiReluBf16:
+    hipLaunchKernelGGL((k_gemm<kAT, kBT, kEpiReluBf16>), grid, dim3(256), 0, s, g);
+    break;
+  case kEpiF32:
+    hipLaunchKernelGGL((k_gemm<kAT, kBT, kEpiF32>), grid, dim3(256), 0, s, g);
+    break;
+  case kEpiMaskBf16:
+    hipLaunchKernelGGL((k_gemm<kAT, kBT, kEpiMaskBf16>), grid, dim3(256), 0, s, g);
+    break;
+  default:
+    hipLaunchKernelGGL((k_gemm<kAT, kBT, kEpiSplitF32>), grid, dim3(256), 0, s, g);
+    break;
+  }
+}
+
+hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilogue,
+                       uint32_t splits, hipStream_t s) {
+  if (g.M % 8 || g.N % 8 || g.K % 8 || g.lda % 8 || g.ldb % 8 || g.k_per_split % kBK ||
+      splits == 0 || uint64_t(splits) * g.k_per_split < g.K)
+    return hipErrorInvalidValue;
+  dim3 grid((g.M + kBM - 1) / kBM, (g.N + kBN - 1) / kBN, splits);
+  if (!a_trans && !b_trans)
+    launch_gemm_t<false, false>(g, epilogue, grid, s);
+  else if (!a_trans && b_trans)
+    launch_gemm_t<false, true>(g, epilogue, grid, s);
+  else if (a_trans && b_trans)
+    launch_gemm_t<true, true>(g, epilogue, grid, s);
+  else
+    launch_gemm_t<true, false>(g, epilogue, grid, s);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Encoding: column c of row i is feature c (c < 9), then for feature d and
+// frequency k: sin(pi 2^k x_d), cos(pi 2^k x_d) at 9 + 24 d + 2 k (+1), zero
+// from 225 on.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_encode(const float *__restrict__ f, uint32_t n,
+                                                uint16_t *__restrict__ out) {
+  const uint32_t row = blockIdx.x, col = threadIdx.x;
+  if (row >= n)
+    return;
+  float v = 0.f;
+  if (col < kFeatures) {
+    v = f[size_t(row) * kFeatures + col];
+  } else if (col < kEncoded) {
+    const uint32_t j = col - kFeatures, d = j / (2 * kFrequencies), r = j % (2 * kFrequencies);
+    const float x = ldexpf(f[size_t(row) * kFeatures + d], int(r >> 1));
+    v = (r & 1u) ? cospif(x) : sinpif(x);
+  }
+  out[size_t(row) * kInPad + col] = f2bf(v);
+}
+
+hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream_t s) {
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_encode, dim3(n), dim3(kInPad), 0, s, features, n,
+                     reinterpret_cast<uint16_t *>(out));
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Loss.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
+                                              const float *__restrict__ target, uint32_t n,
+                                              uint16_t *__restrict__ d_pred,
+                                              float *__restrict__ partials) {
+  __shared__ float red[256];
+  const uint32_t row = blockIdx.x * 256u + threadIdx.x;
+  float l = 0.f;
+  if (row < n) {
+    const float inv = 1.f / (3.f * float(n));
+#pragma unroll
+    for (uint32_t c = 0; c < kOutPad; ++c) {
+      uint16_t d = 0;
+      if (c < kOut) {
+        const float p = pred[size_t(row) * kOutPad + c], t = target[size_t(row) * kOut + c];
+        const float den = p * p + 0.01f;
+        const float e = p - t;
+        l += e * e / den;
+        d = f2bf(2.f * e / den * inv);
+      }
+      d_pred[size_t(row) * kOutPad + c] = d;
+    }
+    l *= inv;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (uint32_t off = 128; off >= 1; off >>= 1) {
+    if (threadIdx.x < off)
+      red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = red[0];
+}
+
+hipError_t launch_loss(const float *pred, const float *target, uint32_t n, bf16 *d_pred,
+                       float *loss_partials, uint32_t *n_partials, hipStream_t s) {
+  const uint32_t blocks = (n + 255) / 256;
+  *n_partials = blocks;
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, pred, target, n,
+                     reinterpret_cast<uint16_t *>(d_pred), loss_partials);
+  return hipGetLastError();
+}
+
+__global__ void k_sum_splits(const float *__restrict__ p, uint32_t splits, uint64_t count,
+                             float *__restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  float s = 0.f;
+  for (uint32_t z = 0; z < splits; ++z)
+    s += p[z * count + i];
+  out[i] = s;
+}
+
+hipError_t launch_sum_splits(const float *partials, uint32_t splits, uint64_t count,
+                             float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_splits, dim3(uint32_t((count + 255) / 256)), dim3(256), 0, s,
+                     partials, splits, count, out);
+  return hipGetLastError();
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, uint16_t *__restrict__ wb, uint64_t count,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  wb[i] = f2bf(wi);
+}
+
+hipError_t launch_adam(float *w, const float *g, float *m, float *v, bf16 *w_bf16,
+                       uint64_t count, float lr, float beta1, float beta2, float eps,
+                       uint32_t t, hipStream_t s) {
+  const float c1 = 1.f - powf(beta1, float(t)), c2 = 1.f - powf(beta2, float(t));
+  hipLaunchKernelGGL(k_adam, dim3(uint32_t((count + 255) / 256)), dim3(256), 0, s, w, g, m, v,
+                     reinterpret_cast<uint16_t *>(w_bf16), count, lr, beta1, beta2, eps, c1,
+                     c2);
+  return hipGetLastError();
+}
+
+__global__ void k_to_bf16(const float *__restrict__ in, uint16_t *__restrict__ out,
+                          uint64_t count) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count)
+    out[i] = f2bf(in[i]);
+}
+
+hipError_t launch_to_bf16(const float *in, bf16 *out, uint64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((count + 255) / 256)), dim3(256), 0, s, in,
+                     reinterpret_cast<uint16_t *>(out), count);
+  return hipGetLastError();
+}
+
+__global__ void k_gather(const float *__restrict__ feat, const float *__restrict__ targ,
+                         const uint32_t *__restrict__ idx, uint32_t n,
+                         float *__restrict__ fo, float *__restrict__ to) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const size_t r = idx[i];
+#pragma unroll
+  for (uint32_t c = 0; c < kFeatures; ++c)
+    fo[size_t(i) * kFeatures + c] = feat[r * kFeatures + c];
+#pragma unroll
+  for (uint32_t c = 0; c < kOut; ++c)
+    to[size_t(i) * kOut + c] = targ[r * kOut + c];
+}
+
+hipError_t launch_gather(const float *feat, const float *targ, const uint32_t *idx,
+                         uint32_t n, float *feat_out, float *targ_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, feat, targ, idx, n,
+                     feat_out, targ_out);
+  return hipGetLastError();
+}
+
+__global__ void k_random_rows(uint32_t n, uint64_t n_rows, uint64_t seed, uint32_t *idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t(i) + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  idx[i] = uint32_t(z % n_rows);
+}
+
+hipError_t launch_random_rows(uint32_t n, uint64_t n_rows, uint64_t seed, uint32_t *idx,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_random_rows, dim3((n + 255) / 256), dim3(256), 0, s, n, n_rows, seed,
+                     idx);
+  return hipGetLastError();
+}
+
+} // namespace mlp
+} // namespace pt
