@@ -1,9 +1,9 @@
 // BVH traversal on gfx950: one ray per lane, near-first order, per-lane stack.
 //
-// Node layout (64 B, both child boxes in the parent) and leaf-ordered triangles
-// are produced by host/bvh.cpp. Child refs >= 0 are inner nodes, refs < 0 are
-// ~(first triangle of a leaf); the last triangle of a leaf has bit 31 of its
-// index word set.
+// 4-wide nodes (128 B: six float4 of child-slot boxes, child refs, counts) and
+// leaf-ordered triangles are produced by host/bvh.cpp. Child refs >= 0 are
+// inner nodes, refs < 0 are ~(first triangle of a leaf), INT32_MIN is an empty
+// slot; the last triangle of a leaf has bit 31 of its index word set.
 //
 // The closest hit is the minimum of (t, flattened triangle index) over all
 // triangles the float Moller-Trumbore test accepts. Boxes are padded on the
@@ -21,6 +21,8 @@ namespace pt {
 namespace gpu {
 
 constexpr uint32_t kLeafEnd = 0x80000000u;
+constexpr int kEmptyChildRef = INT32_MIN; // host/bvh.h kEmptyChild
+constexpr size_t kNodeFloat4 = 8;         // 128-byte nodes
 
 // Stack in private (scratch) memory: used where a kernel has no LDS budget.
 struct PrivateStack {
@@ -78,42 +80,55 @@ trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
   int node = 0;
   for (;;) {
     if (node >= 0) {
-      const float4 *np = nodes + 4 * node;
-      float4 a = np[0], b = np[1], c = np[2];
-      int4 d = __builtin_bit_cast(int4, np[3]);
-      float lx0 = fmaf(a.x, inv.x, -oi.x), lx1 = fmaf(a.y, inv.x, -oi.x);
-      float ly0 = fmaf(a.z, inv.y, -oi.y), ly1 = fmaf(a.w, inv.y, -oi.y);
-      float lz0 = fmaf(c.x, inv.z, -oi.z), lz1 = fmaf(c.y, inv.z, -oi.z);
-      float rx0 = fmaf(b.x, inv.x, -oi.x), rx1 = fmaf(b.y, inv.x, -oi.x);
-      float ry0 = fmaf(b.z, inv.y, -oi.y), ry1 = fmaf(b.w, inv.y, -oi.y);
-      float rz0 = fmaf(c.z, inv.z, -oi.z), rz1 = fmaf(c.w, inv.z, -oi.z);
-      float lmin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)),
-                         fmaxf(fminf(lz0, lz1), 0.0f));
-      float lmax = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)),
-                         fminf(fmaxf(lz0, lz1), best_t));
-      float rmin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)),
-                         fmaxf(fminf(rz0, rz1), 0.0f));
-      float rmax = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)),
-                         fminf(fmaxf(rz0, rz1), best_t));
-      bool hl = lmin <= lmax;
-      bool hr = rmin <= rmax;
-      if (hl && hr) {
-        int near = d.x, far = d.y;
-        if (rmin < lmin) {
-          near = d.y;
-          far = d.x;
-        }
-        stack.push(far);
-        node = near;
-      } else if (hl) {
-        node = d.x;
-      } else if (hr) {
-        node = d.y;
-      } else {
+      const float4 *np = nodes + kNodeFloat4 * size_t(node);
+      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      const int4 ch = __builtin_bit_cast(int4, np[6]);
+      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+      const int chs[4] = {ch.x, ch.y, ch.z, ch.w};
+      float dist[4];
+      int ref[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
+        const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
+        const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
+        const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t));
+        const bool hit = tmin <= tmax && chs[k] != kEmptyChildRef;
+        dist[k] = hit ? tmin : __builtin_inff();
+        ref[k] = chs[k];
+      }
+      // sort the four slots by entry distance (misses last)
+#define PT_CSWAP(i, j)                                                         \
+  if (dist[j] < dist[i]) {                                                     \
+    const float td = dist[i];                                                  \
+    dist[i] = dist[j];                                                         \
+    dist[j] = td;                                                              \
+    const int tr = ref[i];                                                     \
+    ref[i] = ref[j];                                                           \
+    ref[j] = tr;                                                               \
+  }
+      PT_CSWAP(0, 1)
+      PT_CSWAP(2, 3)
+      PT_CSWAP(0, 2)
+      PT_CSWAP(1, 3)
+      PT_CSWAP(1, 2)
+#undef PT_CSWAP
+      if (dist[0] == __builtin_inff()) {
         if (stack.empty())
           break;
         node = stack.pop();
+        continue;
       }
+      if (dist[3] != __builtin_inff())
+        stack.push(ref[3]);
+      if (dist[2] != __builtin_inff())
+        stack.push(ref[2]);
+      if (dist[1] != __builtin_inff())
+        stack.push(ref[1]);
+      node = ref[0];
       continue;
     }
     // leaf

@@ -349,7 +349,7 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
       bool has_nodes = true;
       if constexpr (kBatch) {
         const SceneDesc &sd = p.batch.scenes[p.st.scene[slot]];
-        nodes = p.batch.nodes + 4 * size_t(sd.node_off);
+        nodes = p.batch.nodes + kNodeFloat4 * size_t(sd.node_off);
         tris = p.batch.tris + 3 * size_t(sd.tri_off);
         has_nodes = sd.has_nodes != 0;
       }

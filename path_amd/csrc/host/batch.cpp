@@ -80,7 +80,7 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
     const pt_scene &sc = *it.scene;
     const FlatScene &f = sc.flat();
     const PackedScene &pk = sc.packed();
-    if (pk.bvh.max_depth + 1 >= uint32_t(gpu_stack_size()))
+    if (pk.bvh.stack_need > uint32_t(gpu_stack_size()))
       return set_error(PT_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
                                                ": BVH too deep for the traversal stack");
     gpu::SceneDesc d{};

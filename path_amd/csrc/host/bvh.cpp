@@ -172,7 +172,6 @@ float bvh_padding(const std::vector<TriGeom<float>> &tris) {
 GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
   GpuBvh g;
   g.pad = bvh_padding(tris);
-  g.max_depth = bvh.max_depth;
   g.tris.resize(bvh.perm.size());
   for (size_t i = 0; i < bvh.perm.size(); ++i) {
     const auto &t = tris[bvh.perm[i]];
@@ -196,53 +195,77 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
   if (bvh.nodes.empty())
     return g; // no triangles: callers treat every ray as a miss
   const float pad = g.pad;
-  // Map reference inner nodes to GPU nodes in the same order; a leaf root
-  // becomes a GPU root whose left child is that leaf and right child empty.
-  std::vector<int32_t> gpu_index(bvh.nodes.size(), -1);
-  int32_t count = 0;
-  for (size_t i = 0; i < bvh.nodes.size(); ++i)
-    if (!bvh.nodes[i].is_leaf)
-      gpu_index[i] = count++;
-  auto child_ref = [&](uint32_t c, int32_t *ref, int32_t *cnt) {
-    const BvhNode &n = bvh.nodes[c];
-    if (n.is_leaf) {
-      *ref = ~int32_t(n.a);
-      *cnt = int32_t(n.b - n.a);
-    } else {
-      *ref = gpu_index[c];
-      *cnt = 0;
-    }
-  };
-  auto put_box = [&](GpuNode &g_, int side, const BvhNode &n) {
-    float *xy = side == 0 ? g_.a : g_.b;
-    xy[0] = n.lo.x - pad;
-    xy[1] = n.hi.x + pad;
-    xy[2] = n.lo.y - pad;
-    xy[3] = n.hi.y + pad;
-    g_.c[2 * side] = n.lo.z - pad;
-    g_.c[2 * side + 1] = n.hi.z + pad;
-  };
-  if (bvh.nodes[0].is_leaf) {
-    // Single-leaf tree: a root whose two children are the same leaf.
-    GpuNode n{};
-    put_box(n, 0, bvh.nodes[0]);
-    put_box(n, 1, bvh.nodes[0]);
-    n.d[0] = n.d[1] = ~int32_t(bvh.nodes[0].a);
-    n.d[2] = n.d[3] = int32_t(bvh.nodes[0].b - bvh.nodes[0].a);
-    g.nodes.push_back(n);
-    return g;
-  }
-  g.nodes.resize(count);
-  for (size_t i = 0; i < bvh.nodes.size(); ++i) {
+  auto area = [&](uint32_t i) {
     const BvhNode &n = bvh.nodes[i];
-    if (n.is_leaf)
-      continue;
-    GpuNode &o = g.nodes[gpu_index[i]];
-    put_box(o, 0, bvh.nodes[n.a]);
-    put_box(o, 1, bvh.nodes[n.b]);
-    child_ref(n.a, &o.d[0], &o.d[2]);
-    child_ref(n.b, &o.d[1], &o.d[3]);
+    const float dx = n.hi.x - n.lo.x, dy = n.hi.y - n.lo.y, dz = n.hi.z - n.lo.z;
+    return dx * dy + dy * dz + dz * dx;
+  };
+  // Depth-first construction: a GPU node is appended when first reached, so
+  // children follow their parents in memory.
+  struct Item {
+    uint32_t bvh2; // binary node (inner, or a leaf root)
+    int32_t slot;  // GPU node index to create
+    uint32_t depth;
+  };
+  std::vector<Item> todo;
+  g.nodes.emplace_back();
+  todo.push_back({0u, 0, 0u});
+  while (!todo.empty()) {
+    const Item it = todo.back();
+    todo.pop_back();
+    // up to four descendants, opening the largest inner child first
+    uint32_t kids[4];
+    uint32_t nk = 0;
+    if (bvh.nodes[it.bvh2].is_leaf) {
+      kids[nk++] = it.bvh2;
+    } else {
+      kids[nk++] = bvh.nodes[it.bvh2].a;
+      kids[nk++] = bvh.nodes[it.bvh2].b;
+      while (nk < 4) {
+        int best = -1;
+        float best_area = -1.f;
+        for (uint32_t k = 0; k < nk; ++k)
+          if (!bvh.nodes[kids[k]].is_leaf && area(kids[k]) > best_area) {
+            best = int(k);
+            best_area = area(kids[k]);
+          }
+        if (best < 0)
+          break;
+        const BvhNode &open = bvh.nodes[kids[best]];
+        kids[best] = open.a;
+        kids[nk++] = open.b;
+      }
+    }
+    GpuNode o{};
+    for (uint32_t k = 0; k < 4; ++k) {
+      if (k >= nk) {
+        o.lo_x[k] = o.lo_y[k] = o.lo_z[k] = 1.f;
+        o.hi_x[k] = o.hi_y[k] = o.hi_z[k] = -1.f;
+        o.child[k] = kEmptyChild;
+        continue;
+      }
+      const BvhNode &n = bvh.nodes[kids[k]];
+      o.lo_x[k] = n.lo.x - pad;
+      o.hi_x[k] = n.hi.x + pad;
+      o.lo_y[k] = n.lo.y - pad;
+      o.hi_y[k] = n.hi.y + pad;
+      o.lo_z[k] = n.lo.z - pad;
+      o.hi_z[k] = n.hi.z + pad;
+      if (n.is_leaf) {
+        o.child[k] = ~int32_t(n.a);
+        o.count[k] = int32_t(n.b - n.a);
+      } else {
+        const int32_t slot = int32_t(g.nodes.size());
+        g.nodes.emplace_back();
+        o.child[k] = slot;
+        todo.push_back({kids[k], slot, it.depth + 1});
+      }
+    }
+    g.nodes[size_t(it.slot)] = o;
+    g.max_depth = std::max(g.max_depth, it.depth);
   }
+  // at most three siblings are pushed per level on the way down
+  g.stack_need = 3 * (g.max_depth + 1);
   return g;
 }
 

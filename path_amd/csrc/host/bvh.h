@@ -3,6 +3,7 @@
 
 #include "../core/geometry.h"
 
+#include <climits>
 #include <cstdint>
 #include <vector>
 
@@ -38,16 +39,22 @@ struct BvhSettings {
 Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
                   const BvhSettings &settings = {});
 
-// GPU layout: each inner node stores both child boxes (64 bytes):
-//   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y), b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y),
-//   c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z), d = (L ref, R ref, L count, R count)
-// A child ref >= 0 is an inner-node index, a ref < 0 is ~(first triangle) of a
-// leaf with `count` triangles. Boxes are padded outward by `pad` so that the
-// box test is conservative with respect to the float triangle test (the
-// closest hit then does not depend on traversal order).
+// GPU layout: a 4-wide BVH collapsed from the binary tree (each GPU node takes
+// the up-to-4 descendants obtained by repeatedly opening the child with the
+// largest surface area). 128 bytes per node, structure of arrays over the
+// four child slots:
+//   lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4], child[4], count[4]
+// child >= 0 is an inner-node index, child < 0 is ~(first triangle) of a leaf
+// with `count` triangles, kEmptyChild marks an unused slot. Boxes are padded
+// outward by `pad` so that the box test is conservative with respect to the
+// float triangle test (the closest hit then does not depend on traversal
+// order).
+constexpr int32_t kEmptyChild = INT32_MIN;
+
 struct GpuNode {
-  float a[4], b[4], c[4];
-  int32_t d[4];
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  int32_t child[4];
+  int32_t count[4];
 };
 
 // Triangle in leaf order: (v0, idx bits), (e1, 0), (e2, 0).
@@ -64,7 +71,8 @@ struct GpuBvh {
   std::vector<GpuNode> nodes;
   std::vector<GpuTri> tris;
   float pad = 0.f;
-  uint32_t max_depth = 0;
+  uint32_t max_depth = 0;   // depth of the 4-wide tree (root = 0)
+  uint32_t stack_need = 0;  // traversal stack entries a ray can need
 };
 
 GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris);

@@ -150,10 +150,10 @@ int pt_renderer::ensure_scene(const pt_scene &scene) {
     cached_mat_gen = scene.mat_generation;
     return PT_OK;
   }
-  const Bvh &b = scene.bvh();
-  if (b.max_depth + 1 >= uint32_t(gpu_stack_size()))
+  const GpuBvh &gb = scene.packed().bvh;
+  if (gb.stack_need > uint32_t(gpu_stack_size()))
     return set_error(PT_ERR_UNSUPPORTED,
-                     "BVH depth " + std::to_string(b.max_depth) +
+                     "BVH depth " + std::to_string(gb.max_depth) +
                          " exceeds the traversal stack");
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(scene_store.upload(scene, stream));

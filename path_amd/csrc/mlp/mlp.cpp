@@ -267,8 +267,11 @@ int pt_mlp_forward(pt_mlp *m, const float *features, uint32_t n, float *out) {
   rc = m->forward_device(m->feat.ptr, rp, false);
   if (rc != PT_OK)
     return rc;
-  HIP_TRY(hipMemcpy2DAsync(out, kOut * 4, m->pred.ptr, kOutPad * 4, kOut * 4, n,
-                           hipMemcpyDeviceToHost, m->stream));
+  // compact to n x 3 on the device (reuses the front of the target buffer)
+  HIP_TRY(m->targ.ensure(size_t(rp) * kOut));
+  HIP_TRY(launch_compact_pred(m->pred.ptr, n, m->targ.ptr, m->stream));
+  HIP_TRY(hipMemcpyAsync(out, m->targ.ptr, size_t(n) * kOut * 4, hipMemcpyDeviceToHost,
+                         m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
   return PT_OK;
 }

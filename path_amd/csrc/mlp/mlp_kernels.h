@@ -78,6 +78,8 @@ hipError_t launch_adam(float *w, const float *g, float *m, float *v, bf16 *w_bf1
                        uint64_t count, float lr, float beta1, float beta2, float eps,
                        uint32_t t, hipStream_t s);
 hipError_t launch_to_bf16(const float *in, bf16 *out, uint64_t count, hipStream_t s);
+// out[i][c] = pred[i][c] for c < 3 (rows of 16 -> rows of 3)
+hipError_t launch_compact_pred(const float *pred, uint32_t n, float *out, hipStream_t s);
 // Gather rows idx[i] of a dataset (features 9 + targets 3 per row).
 hipError_t launch_gather(const float *feat, const float *targ, const uint32_t *idx,
                          uint32_t n, float *feat_out, float *targ_out, hipStream_t s);

@@ -290,6 +290,21 @@ hipError_t launch_to_bf16(const float *in, bf16 *out, uint64_t count, hipStream_
   return hipGetLastError();
 }
 
+__global__ void k_compact_pred(const float *__restrict__ pred, uint32_t n,
+                               float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * n)
+    return;
+  out[i] = pred[size_t(i / 3) * kOutPad + i % 3];
+}
+
+hipError_t launch_compact_pred(const float *pred, uint32_t n, float *out, hipStream_t s) {
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_compact_pred, dim3((3 * n + 255) / 256), dim3(256), 0, s, pred, n, out);
+  return hipGetLastError();
+}
+
 __global__ void k_gather(const float *__restrict__ feat, const float *__restrict__ targ,
                          const uint32_t *__restrict__ idx, uint32_t n,
                          float *__restrict__ fo, float *__restrict__ to) {
