@@ -130,7 +130,7 @@ def bench_datagen(a, cfg):
 
     # per-scene launches: the same scenes, one render call each
     single = settings.copy()
-    set_launch_params(r, 8, 1, False)
+    set_launch_params(r, 48, 1, False)
     for g in gens[:4]:
         r.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp, single)
     sync()
@@ -520,6 +520,10 @@ def main(argv=None):
                     help="override the configuration's integrator")
     ap.add_argument("--max-paths", type=int, default=0,
                     help="wavefront path-pool size (max_num_samples_per_launch)")
+    ap.add_argument("--trace-blocks-per-cu", type=int, default=0,
+                    help="wavefront traversal grid (blocks of 256 per CU)")
+    ap.add_argument("--iters-per-sync", type=int, default=0,
+                    help="wavefront iterations enqueued per host check (hipGraph replay)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c3: keep the default launch parameters")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -557,6 +561,10 @@ def main(argv=None):
         settings.max_num_samples_per_launch = a.max_paths
     n_tris = scene.counts()[0]
     r = Renderer(0)
+    if a.trace_blocks_per_cu or a.iters_per_sync:
+        from path_amd.datagen import set_launch_params
+        set_launch_params(r, a.trace_blocks_per_cu or 48, a.iters_per_sync or 1,
+                          bool(a.iters_per_sync))
     size, spp = cfg["size"], cfg["spp"]
     for _ in range(a.warmup):
         r.render(scene, ftw, size, size, spp, settings)
