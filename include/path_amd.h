@@ -350,6 +350,11 @@ int pt_mlp_set_dataset(pt_mlp *m, const float *features, const float *targets,
 int pt_mlp_train(pt_mlp *m, uint32_t steps, uint32_t batch, float lr,
                  uint64_t seed, float *losses, double *seconds);
 
+/* Inference timing on `rows` device-resident rows: average duration of one
+ * hidden 256x256 layer GEMM and of a whole forward pass (HIP events). */
+int pt_mlp_time_forward(pt_mlp *m, uint32_t rows, uint32_t iters,
+                        float *gemm_ms, float *forward_ms);
+
 /* Radiance-cache render of a SquareImage (FloatRGB, host out_rgb x*y*3):
  * each sample traces the camera ray and one bounce with the light sample of
  * the first vertex; the radiance reflected at the second vertex is the MLP's

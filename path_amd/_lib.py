@@ -155,6 +155,7 @@ SIGNATURES = {
     "pt_mlp_set_dataset": (C.c_int, [_P, _F, _F, C.c_uint64]),
     "pt_mlp_train": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F,
                                C.POINTER(C.c_double)]),
+    "pt_mlp_time_forward": (C.c_int, [_P, C.c_uint32, C.c_uint32, _F, _F]),
     "pt_render_mlp": (C.c_int, [_P, _P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(PtSettings), _F, C.POINTER(C.c_double)]),
 }

@@ -152,7 +152,7 @@ int pt_mlp::backward(uint32_t n, uint32_t rows_pad, float *loss) {
   // split-K over the batch for the weight gradients: about 256 workgroups
   const uint32_t splits = std::max(1u, std::min(64u, rows_pad / 1024u));
   const uint32_t kps = (rows_pad + splits - 1) / splits;
-  const uint32_t kps32 = (kps + 31) / 32 * 32;
+  const uint32_t kps32 = (kps + 63) / 64 * 64;
   HIP_TRY(split_part.ensure(size_t(splits) * kWidth * kWidth));
   const bf16 *G = d_pred.ptr;
   for (int l = int(kLayers) - 1; l >= 0; --l) {
@@ -326,6 +326,48 @@ int pt_mlp_train_step(pt_mlp *m, const float *features, const float *targets, ui
   if (rc != PT_OK)
     return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return PT_OK;
+}
+
+// Times `iters` inference passes over `rows` device-resident rows (features
+// all 0.25): the average duration of one hidden 256x256 layer GEMM (layer 1)
+// and of the whole forward pass, both with HIP events on the MLP's stream.
+int pt_mlp_time_forward(pt_mlp *m, uint32_t rows, uint32_t iters, float *gemm_ms,
+                        float *forward_ms) {
+  if (!m || rows == 0 || iters == 0)
+    return set_error(PT_ERR_INVALID, "null argument or empty workload");
+  HIP_TRY(hipSetDevice(m->device));
+  const uint32_t rp = mlp_pad_rows(rows);
+  int rc = m->ensure_rows(rp, false);
+  if (rc != PT_OK)
+    return rc;
+  std::vector<float> f(size_t(rp) * kFeatures, 0.25f);
+  HIP_TRY(hipMemcpyAsync(m->feat.ptr, f.data(), f.size() * 4, hipMemcpyHostToDevice,
+                         m->stream));
+  rc = m->forward_device(m->feat.ptr, rp, false);
+  if (rc != PT_OK)
+    return rc;
+  GemmArgs g = gemm_args(m->act[0].ptr, kWidth, m->wbf.ptr + layer_offset(1), kWidth, rp,
+                         kWidth, kWidth, m->act[1].ptr, kWidth);
+  HIP_TRY(hipEventRecord(m->ev0, m->stream));
+  for (uint32_t i = 0; i < iters; ++i)
+    HIP_TRY(launch_gemm(g, false, false, kEpiReluBf16, 1, m->stream));
+  HIP_TRY(hipEventRecord(m->ev1, m->stream));
+  HIP_TRY(hipEventSynchronize(m->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+  if (gemm_ms)
+    *gemm_ms = ms / float(iters);
+  HIP_TRY(hipEventRecord(m->ev0, m->stream));
+  for (uint32_t i = 0; i < iters && rc == PT_OK; ++i)
+    rc = m->forward_device(m->feat.ptr, rp, false);
+  if (rc != PT_OK)
+    return rc;
+  HIP_TRY(hipEventRecord(m->ev1, m->stream));
+  HIP_TRY(hipEventSynchronize(m->ev1));
+  HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+  if (forward_ms)
+    *forward_ms = ms / float(iters);
   return PT_OK;
 }
 

@@ -17,38 +17,57 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 // ---------------------------------------------------------------------------
-// GEMM: 128 x 128 block tile, K step 32, 4 waves in a 2 x 2 grid, each wave a
+// GEMM: 128 x 128 block tile, K step 64, 4 waves in a 2 x 2 grid, each wave a
 // 64 x 64 tile of 2 x 2 v_mfma_f32_32x32x16_bf16. Operands are staged in LDS
-// as [row][k] (k contiguous, row stride 40 bf16 = 80 B); transposed global
-// layouts are transposed on the way in.
+// as [row][k] (k contiguous, row stride 72 bf16 = 144 B); transposed global
+// layouts are transposed on the way in. The next K step's global loads are
+// issued into registers before the current step's MFMAs, so their latency
+// overlaps the math.
 // MFMA lane layout (32x32x16): lane l holds A[l % 32][8 (l / 32) .. +8],
 // B[8 (l / 32) .. +8][l % 32]; accumulator r of lane l is
 // C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBM = 128, kBN = 128, kBK = 32, kLdsStride = 40;
+constexpr uint32_t kBM = 128, kBN = 128, kBK = 64, kLdsStride = 72;
+constexpr uint32_t kChunks = kBM * kBK / 8 / 256; // 16-byte chunks per thread per operand
 
+// Global -> registers: chunk c of the tile (8 bf16 along k, or along rows when
+// transposed), zero outside [0, rows) x [k0, k_end).
 template <bool kT>
-__device__ __forceinline__ void load_tile(const uint16_t *__restrict__ src, uint32_t ld,
-                                          uint32_t r0, uint32_t rows, uint32_t k0,
-                                          uint32_t k_end, uint16_t *__restrict__ dst) {
-  // 128 rows x 32 k = 512 chunks of 8 bf16; 256 threads x 2
-  for (uint32_t c = threadIdx.x; c < kBM * kBK / 8; c += 256) {
-    uint4 v = make_uint4(0, 0, 0, 0);
+__device__ __forceinline__ void fetch_tile(const uint16_t *__restrict__ src, uint32_t ld,
+                                           uint32_t r0, uint32_t rows, uint32_t k0,
+                                           uint32_t k_end, uint4 (&v)[kChunks]) {
+#pragma unroll
+  for (uint32_t i = 0; i < kChunks; ++i) {
+    const uint32_t c = threadIdx.x + 256u * i;
+    v[i] = make_uint4(0, 0, 0, 0);
     if (!kT) {
-      const uint32_t row = c >> 2, kc = (c & 3u) * 8u;
+      const uint32_t row = c / (kBK / 8), kc = (c % (kBK / 8)) * 8u;
       const uint32_t gr = r0 + row, gk = k0 + kc;
       if (gr < rows && gk < k_end)
-        v = *reinterpret_cast<const uint4 *>(src + size_t(gr) * ld + gk);
-      *reinterpret_cast<uint4 *>(dst + row * kLdsStride + kc) = v;
+        v[i] = *reinterpret_cast<const uint4 *>(src + size_t(gr) * ld + gk);
     } else {
-      const uint32_t kr = c >> 4, rc = (c & 15u) * 8u;
+      const uint32_t kr = c / (kBM / 8), rc = (c % (kBM / 8)) * 8u;
       const uint32_t gk = k0 + kr, gr = r0 + rc;
       if (gk < k_end && gr < rows)
-        v = *reinterpret_cast<const uint4 *>(src + size_t(gk) * ld + gr);
-      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+        v[i] = *reinterpret_cast<const uint4 *>(src + size_t(gk) * ld + gr);
+    }
+  }
+}
+
+template <bool kT>
+__device__ __forceinline__ void store_tile(const uint4 (&v)[kChunks], uint16_t *__restrict__ dst) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
-        dst[(rc + i) * kLdsStride + kr] = e[i];
+  for (uint32_t i = 0; i < kChunks; ++i) {
+    const uint32_t c = threadIdx.x + 256u * i;
+    if (!kT) {
+      const uint32_t row = c / (kBK / 8), kc = (c % (kBK / 8)) * 8u;
+      *reinterpret_cast<uint4 *>(dst + row * kLdsStride + kc) = v[i];
+    } else {
+      const uint32_t kr = c / (kBM / 8), rc = (c % (kBM / 8)) * 8u;
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v[i]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        dst[(rc + j) * kLdsStride + kr] = e[j];
     }
   }
 }
@@ -72,10 +91,19 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 #pragma unroll
       for (int r = 0; r < 16; ++r)
         acc[i][j][r] = 0.f;
+  uint4 va[kChunks], vb[kChunks];
+  if (k_begin < k_end) {
+    fetch_tile<kAT>(A, g.lda, m0, g.M, k_begin, k_end, va);
+    fetch_tile<kBT>(B, g.ldb, n0, g.N, k_begin, k_end, vb);
+  }
   for (uint32_t k0 = k_begin; k0 < k_end; k0 += kBK) {
-    load_tile<kAT>(A, g.lda, m0, g.M, k0, k_end, As);
-    load_tile<kBT>(B, g.ldb, n0, g.N, k0, k_end, Bs);
+    store_tile<kAT>(va, As);
+    store_tile<kBT>(vb, Bs);
     __syncthreads();
+    if (k0 + kBK < k_end) {
+      fetch_tile<kAT>(A, g.lda, m0, g.M, k0 + kBK, k_end, va);
+      fetch_tile<kBT>(B, g.ldb, n0, g.N, k0 + kBK, k_end, vb);
+    }
 #pragma unroll
     for (uint32_t kk = 0; kk < kBK; kk += 16) {
       bf16x8_t a[2], b[2];
@@ -141,7 +169,7 @@ static void launch_gemm_t(const GemmArgs &g, int epi, dim3 grid, hipStream_t s) 
 
 hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilogue,
                        uint32_t splits, hipStream_t s) {
-  if (g.M % 8 || g.N % 8 || g.K % 8 || g.lda % 8 || g.ldb % 8 || g.k_per_split % kBK ||
+  if (g.M % 8 || g.N % 8 || g.K % 8 || g.lda % 8 || g.ldb % 8 || g.k_per_split % 32 ||
       splits == 0 || uint64_t(splits) * g.k_per_split < g.K)
     return hipErrorInvalidValue;
   dim3 grid((g.M + kBM - 1) / kBM, (g.N + kBN - 1) / kBN, splits);

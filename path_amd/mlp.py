@@ -83,6 +83,13 @@ class RadianceMLP:
                                      L.fptr(losses), C.byref(secs)))
         return losses, secs.value
 
+    def time_forward(self, rows: int, iters: int = 10):
+        """(ms of one hidden-layer GEMM, ms of a whole forward) on `rows`
+        device-resident rows, HIP-event timed."""
+        g, f = C.c_float(), C.c_float()
+        L.check(L.lib().pt_mlp_time_forward(self.handle, rows, iters, C.byref(g), C.byref(f)))
+        return float(g.value), float(f.value)
+
     def render(self, renderer, scene, film_to_world, x_dim: int, y_dim: int, samples_per: int,
                settings=None):
         """Radiance-cache render (one traced bounce + this model). Returns
