@@ -161,6 +161,14 @@ template <class T> struct RenderArgs {
   const float *sobol; // vals[sample * dim_bound + dim]
   uint32_t dim_bound, initial_dim_bound, samples_per;
   uint32_t rng_type;
+  // InitialRays / InitialIdxAndDir sample specs (initial_ray_sample.h:16-75):
+  // one ray (o, d) per "pixel" x; with initial_idx the first intersection is
+  // that triangle at distance 1, front-facing
+  const float *initial_rays = nullptr;
+  const uint32_t *initial_idx = nullptr;
+  // OutputPerStep (rendering_equation.h:56-123): radiance added by iteration
+  // k goes to step image k; paths stop after n_steps iterations
+  uint32_t n_steps = 0;
 };
 
 template <class T> struct TableRngT {
@@ -180,9 +188,10 @@ template <class T> struct HashRngT {
 
 template <class T, class R>
 V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
-                       uint32_t location, KindCounts *cnt) {
+                       uint32_t location, KindCounts *cnt, V3<T> *steps) {
   V3<T> total = {T(0), T(0), T(0)};
   const SceneView<T> view = a.scene->view();
+  const bool per_step = a.n_steps != 0;
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
     R rng;
     if constexpr (std::is_same_v<R, TableRngT<T>>)
@@ -190,27 +199,49 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
              initial_dimension(location, a.initial_dim_bound), a.dim_bound};
     else
       rng = {make_hash_rng(sample + location * a.samples_per)};
-    T xo = rng.next();
-    T yo = rng.next();
-    Ray<T> next_ray = camera_ray(a.film_to_world, T(float(info.x)) + xo,
-                                 T(float(info.y)) + yo, a.wd.x_dim, a.wd.y_dim);
+    Ray<T> next_ray;
+    Hit<T> given{T(0), kNoHit, 0};
+    if (a.initial_rays) {
+      const float *r = a.initial_rays + 6 * size_t(info.x);
+      next_ray = {{T(r[0]), T(r[1]), T(r[2])}, {T(r[3]), T(r[4]), T(r[5])}};
+      if (a.initial_idx)
+        given = {T(1), a.initial_idx[info.x], 0};
+    } else {
+      T xo = rng.next();
+      T yo = rng.next();
+      next_ray = camera_ray(a.film_to_world, T(float(info.x)) + xo, T(float(info.y)) + yo,
+                            a.wd.x_dim, a.wd.y_dim);
+    }
     Ray<T> light_ray = next_ray;
     PathState<T> st = initial_path_state<T>();
-    for (;;) {
+    for (uint32_t step = 0;; ++step) {
+      if (per_step) {
+        if (step >= a.n_steps)
+          break;
+        st.total = {T(0), T(0), T(0)};
+      }
       Hit<T> light_hit{T(0), kNoHit, 0}, next_hit{T(0), kNoHit, 0};
       if (st.n_light) {
         light_hit = bvh_closest(*a.scene, light_ray, &cnt->c[1]);
         cnt->rays[1] += 1;
       }
       if (st.has_next) {
-        next_hit = bvh_closest(*a.scene, next_ray, &cnt->c[0]);
-        cnt->rays[0] += 1;
+        if (step == 0 && a.initial_idx) {
+          next_hit = given;
+        } else {
+          next_hit = bvh_closest(*a.scene, next_ray, &cnt->c[0]);
+          cnt->rays[0] += 1;
+        }
       }
       PathState<T> ns;
       Ray<T> nl = light_ray, nn = next_ray;
-      if (rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
-                                       a.settings, view, &ns, &nl, &nn)) {
-        total += ns.total;
+      const bool done = rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
+                                                     a.settings, view, &ns, &nl, &nn);
+      if (per_step)
+        steps[step] += ns.total;
+      if (done) {
+        if (!per_step)
+          total += ns.total;
         break;
       }
       st = ns;
@@ -357,7 +388,9 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
   const uint32_t n_pixels = wd.x_dim * wd.y_dim;
   if (pixel_end == 0 || pixel_end > n_pixels)
     pixel_end = n_pixels;
-  std::vector<T> partial(size_t(n_pixels) * wd.num_sample_blocks * 3, T(0));
+  const uint32_t n_images = a.n_steps ? a.n_steps : 1;
+  const size_t image = size_t(n_pixels) * wd.num_sample_blocks * 3;
+  std::vector<T> partial(image * n_images, T(0));
   std::atomic<uint64_t> acc[6];
   for (auto &x : acc)
     x = 0;
@@ -366,37 +399,49 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
     omp_set_num_threads(n_threads);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t block = 0; block < total_blocks; ++block) {
-    std::vector<V3<T>> thread_tot(wd.block_size);
+    // [image][thread]
+    std::vector<V3<T>> thread_tot(size_t(wd.block_size) * n_images,
+                                  V3<T>{T(0), T(0), T(0)});
+    std::vector<V3<T>> steps(n_images);
     bool any = false;
     KindCounts kc;
     for (uint32_t th = 0; th < wd.block_size; ++th) {
       ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
-      thread_tot[th] = {T(0), T(0), T(0)};
       if (info.exit)
         continue;
       uint32_t loc = info.x + info.y * wd.x_dim;
       if (loc < pixel_begin || loc >= pixel_end)
         continue;
       any = true;
+      for (auto &v : steps)
+        v = {T(0), T(0), T(0)};
+      V3<T> t;
       if (a.rng_type == kRngSobel)
-        thread_tot[th] = integrate_thread<T, TableRngT<T>>(a, info, loc, &kc);
+        t = integrate_thread<T, TableRngT<T>>(a, info, loc, &kc, steps.data());
       else
-        thread_tot[th] = integrate_thread<T, HashRngT<T>>(a, info, loc, &kc);
+        t = integrate_thread<T, HashRngT<T>>(a, info, loc, &kc, steps.data());
+      if (a.n_steps)
+        for (uint32_t i = 0; i < n_images; ++i)
+          thread_tot[size_t(i) * wd.block_size + th] = steps[i];
+      else
+        thread_tot[th] = t;
     }
     if (!any)
       continue;
     uint32_t sbs = wd.sample_block_size;
-    for (uint32_t th = 0; th < wd.block_size; th += sbs) {
-      ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
-      if (info.exit)
-        continue;
-      V3<T> sum = tree_reduce(thread_tot.data() + th, sbs);
-      size_t idx = size_t(wd.sample_block_idx(uint32_t(block))) +
-                   size_t(wd.num_sample_blocks) * (info.x + size_t(info.y) * wd.x_dim);
-      partial[3 * idx] = sum.x;
-      partial[3 * idx + 1] = sum.y;
-      partial[3 * idx + 2] = sum.z;
-    }
+    for (uint32_t i = 0; i < n_images; ++i)
+      for (uint32_t th = 0; th < wd.block_size; th += sbs) {
+        ThreadInfo info = wd.get_thread_info(uint32_t(block), th);
+        if (info.exit)
+          continue;
+        V3<T> sum = tree_reduce(thread_tot.data() + size_t(i) * wd.block_size + th, sbs);
+        size_t idx = size_t(wd.sample_block_idx(uint32_t(block))) +
+                     size_t(wd.num_sample_blocks) * (info.x + size_t(info.y) * wd.x_dim);
+        T *pp = partial.data() + i * image;
+        pp[3 * idx] = sum.x;
+        pp[3 * idx + 1] = sum.y;
+        pp[3 * idx + 2] = sum.z;
+      }
     for (int k = 0; k < 2; ++k) {
       acc[3 * k] += kc.rays[k];
       acc[3 * k + 1] += kc.c[k].nodes;
@@ -404,15 +449,19 @@ int render_impl(const RenderArgs<T> &a, uint32_t pixel_begin, uint32_t pixel_end
     }
   }
   T spp = T(float(a.samples_per));
-  for (uint32_t p = pixel_begin; p < pixel_end; ++p) {
-    T s[3] = {T(0), T(0), T(0)};
-    for (uint32_t k = 0; k < wd.num_sample_blocks; ++k)
+  for (uint32_t i = 0; i < n_images; ++i) {
+    const T *pp = partial.data() + i * image;
+    T *o = out + size_t(i) * n_pixels * 3;
+    for (uint32_t p = pixel_begin; p < pixel_end; ++p) {
+      T s[3] = {T(0), T(0), T(0)};
+      for (uint32_t k = 0; k < wd.num_sample_blocks; ++k)
+        for (int ch = 0; ch < 3; ++ch)
+          s[ch] = wd.num_sample_blocks == 1
+                      ? pp[3 * (size_t(p) * wd.num_sample_blocks + k) + ch]
+                      : s[ch] + pp[3 * (size_t(p) * wd.num_sample_blocks + k) + ch];
       for (int ch = 0; ch < 3; ++ch)
-        s[ch] = wd.num_sample_blocks == 1
-                    ? partial[3 * (size_t(p) * wd.num_sample_blocks + k) + ch]
-                    : s[ch] + partial[3 * (size_t(p) * wd.num_sample_blocks + k) + ch];
-    for (int ch = 0; ch < 3; ++ch)
-      out[3 * size_t(p) + ch] = s[ch] / spp;
+        o[3 * size_t(p) + ch] = s[ch] / spp;
+    }
   }
   if (counts)
     for (int k = 0; k < 6; ++k)
@@ -633,7 +682,8 @@ int oracle_adjoint(const float *tris, const float *normals, const uint32_t *tri_
 // Mega-kernel-equivalent render. precision: 32 or 64. Output out32 (float) or
 // out64 (double), x_dim*y_dim*3. Counts per render: rays traced, BVH nodes
 // visited and triangles tested by the reference traversal.
-int oracle_render(const float *tris, const float *normals, const uint32_t *tri_mat,
+int oracle_render_ex(const float *initial_rays, const uint32_t *initial_idx,
+                     uint32_t n_steps, const float *tris, const float *normals, const uint32_t *tri_mat,
                   uint32_t n_tris, const pt_material *mats, uint32_t n_mats,
                   const float *light_tris, const float *light_cdf,
                   uint32_t n_lights, const float *film_to_world,
@@ -667,7 +717,7 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
     fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
                light_cdf, n_lights);
     RenderArgs<double> a{&s, is, wd, {}, sobol, dim_bound, init_bound,
-                         samples_per, settings->rng};
+                         samples_per, settings->rng, initial_rays, initial_idx, n_steps};
     for (int i = 0; i < 12; ++i)
       a.film_to_world[i] = film_to_world[i];
     t0 = omp_get_wtime();
@@ -678,7 +728,7 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
     fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
                light_cdf, n_lights);
     RenderArgs<float> a{&s, is, wd, {}, sobol, dim_bound, init_bound,
-                        samples_per, settings->rng};
+                        samples_per, settings->rng, initial_rays, initial_idx, n_steps};
     for (int i = 0; i < 12; ++i)
       a.film_to_world[i] = film_to_world[i];
     t0 = omp_get_wtime();
@@ -690,6 +740,23 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
   if (seconds)
     *seconds = t1 - t0;
   return rc;
+}
+
+int oracle_render(const float *tris, const float *normals, const uint32_t *tri_mat,
+                  uint32_t n_tris, const pt_material *mats, uint32_t n_mats,
+                  const float *light_tris, const float *light_cdf,
+                  uint32_t n_lights, const float *film_to_world,
+                  const pt_settings *settings, uint32_t samples_per,
+                  uint32_t x_dim, uint32_t y_dim, const float *sobol,
+                  int precision, int n_threads, uint32_t pixel_begin,
+                  uint32_t pixel_end, float *out32, double *out64,
+                  uint64_t *counts /* 6: per kind (camera/continuation,
+                                      light): rays, nodes, tris */,
+                  double *seconds /* render time, scene setup excluded */) {
+  return oracle_render_ex(nullptr, nullptr, 0, tris, normals, tri_mat, n_tris, mats, n_mats,
+                          light_tris, light_cdf, n_lights, film_to_world, settings,
+                          samples_per, x_dim, y_dim, sobol, precision, n_threads,
+                          pixel_begin, pixel_end, out32, out64, counts, seconds);
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }

@@ -37,6 +37,7 @@ def lib():
                                     _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                     C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
                                     C.c_uint32, _F, _D, _U64, _D]
+        l.oracle_render_ex.argtypes = [_F, _U, C.c_uint32] + l.oracle_render.argtypes
         l.oracle_adjoint.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
                                      _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                      C.c_uint32, C.c_uint32, _F, _D, C.c_int, _D, _D, _D, _D]
@@ -105,18 +106,26 @@ def thread_infos(wds: L.PtWorkDivisionSettings, spp: int, x: int, y: int, max_th
 
 def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
            sobol: np.ndarray | None, precision: int = 32, threads: int = 0,
-           pixel_begin: int = 0, pixel_end: int = 0):
+           pixel_begin: int = 0, pixel_end: int = 0, initial_rays=None, initial_idx=None,
+           n_steps: int = 0):
     """Returns (image (y, x, 3), info) with info = {"counts": [[rays, nodes,
     tris] for continuation/camera rays, same for light rays], "seconds":
-    render time without scene setup}."""
+    render time without scene setup}. With initial_rays ((n, 6): one ray per
+    "pixel", x_dim = n, y_dim = 1; with initial_idx the first intersection
+    is that triangle at distance 1) and / or n_steps (per-step images,
+    shape (n_steps, y, x, 3))."""
     raw = settings.raw if hasattr(settings, "raw") else settings
     ftw = np.ascontiguousarray(film_to_world, np.float32)
     sob = np.zeros(1, np.float32) if sobol is None else np.ascontiguousarray(sobol, np.float32)
     counts = np.zeros(6, np.uint64)
     secs = C.c_double()
-    out32 = np.zeros((y_dim, x_dim, 3), np.float32)
-    out64 = np.zeros((y_dim, x_dim, 3), np.float64)
-    rc = lib().oracle_render(
+    n_img = max(n_steps, 1)
+    out32 = np.zeros((n_img, y_dim, x_dim, 3), np.float32)
+    out64 = np.zeros((n_img, y_dim, x_dim, 3), np.float64)
+    rays = None if initial_rays is None else np.ascontiguousarray(initial_rays, np.float32)
+    idx = None if initial_idx is None else np.ascontiguousarray(initial_idx, np.uint32)
+    rc = lib().oracle_render_ex(
+        _f(rays) if rays is not None else None, _u(idx) if idx is not None else None, n_steps,
         _f(flat["tris"]), _f(flat["normals"]), _u(flat["tri_mat"]), flat["tris"].shape[0],
         flat["materials"], flat["n_materials"], _f(flat["light_tris"]), _f(flat["light_cdf"]),
         flat["light_tris"].shape[0], _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
@@ -125,7 +134,8 @@ def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_
     if rc != 0:
         raise ValueError("oracle render failed")
     info = {"counts": counts.reshape(2, 3), "seconds": secs.value}
-    return (out64 if precision == 64 else out32), info
+    img = out64 if precision == 64 else out32
+    return (img if n_steps else img[0]), info
 
 
 def adjoint(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,

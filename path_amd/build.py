@@ -86,6 +86,20 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+CLI = LIB_DIR / "path"
+
+
+def build_cli() -> Path:
+    """The `path` command line (src/main.cpp flags) linked against the library."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CLI.with_suffix(".tmp")
+    _run([cxx, *HOST_FLAGS, "-I/opt/rocm/include", str(CSRC / "cli" / "main.cpp"),
+          "-o", str(tmp), f"-L{LIB_DIR}", "-lpath_amd", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build_oracle() -> Path:
     (ORACLE_DIR / "_build").mkdir(parents=True, exist_ok=True)
     cxx = os.environ.get("CXX", "g++")
@@ -107,6 +121,7 @@ def main(argv: list[str] | None = None) -> int:
     both = not (a.lib or a.oracle)
     if a.lib or both:
         print("built", build_lib(a.j))
+        print("built", build_cli())
     if a.oracle or both:
         print("built", build_oracle())
     return 0

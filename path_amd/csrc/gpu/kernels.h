@@ -11,6 +11,8 @@ namespace gpu {
 
 // Entries of the per-lane traversal stack (bounds the BVH depth accepted).
 constexpr int kTraceStackSize = 64;
+// Most per-step images of an OutputPerStep render.
+constexpr uint32_t kMaxOutputSteps = 32;
 
 // Device copy of one flattened scene.
 struct DeviceScene {
@@ -35,8 +37,10 @@ struct MegaParams {
   WorkDivision wd;
   RngParams rng;
   float film_to_world[12];
-  const float *initial_rays; // InitialRays sample spec (6 floats per ray) or null
+  const float *initial_rays; // InitialRays / InitialIdxAndDir rays (6 floats) or null
+  const uint32_t *initial_idx; // InitialIdxAndDir: triangle hit at distance 1, or null
   float *out_rgb;            // num_sample_blocks * x_dim * y_dim * 3
+  uint32_t n_steps;          // OutputPerStep: images (out_rgb holds n_steps of them), else 0
   uint32_t divide_by_spp;    // 1 when num_sample_blocks == 1
   unsigned long long *ray_counter;
 };

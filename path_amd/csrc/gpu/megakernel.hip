@@ -84,10 +84,13 @@ hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
 // reference: include/render/detail/integrate_image/mega_kernel/run_impl.h:140-206,
 // include/integrate/rendering_equation.h:27-129
 // ---------------------------------------------------------------------------
-template <class R>
+// kPerStep (OutputPerStep, reference rendering_equation.h:56-123): the
+// radiance added by iteration k of a path goes to steps[k]; a path stops after
+// n_steps iterations.
+template <class R, bool kPerStep>
 __device__ __forceinline__ V3<float>
 integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location,
-                  uint32_t *rays_traced) {
+                  uint32_t *rays_traced, V3<float> *steps) {
   V3<float> total = {0.f, 0.f, 0.f};
   uint32_t n_rays_traced = 0;
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
@@ -100,9 +103,14 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
       rng = make_hash_rng(sample + location * p.rng.samples_per);
     }
     Ray<float> next_ray, light_ray;
+    Hit<float> given = {0.f, kNoHit, 0};
     if (p.initial_rays) {
       const float *r = p.initial_rays + 6 * size_t(info.x);
       next_ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+      // InitialIdxAndDir: the first intersection is given, at distance 1 and
+      // front-facing by convention (initial_ray_sample.h:58-75)
+      if (p.initial_idx)
+        given = {1.f, p.initial_idx[info.x], 0};
     } else {
       float xo = rng.next();
       float yo = rng.next();
@@ -111,7 +119,12 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
     }
     light_ray = next_ray;
     PathState<float> st = initial_path_state<float>();
-    for (;;) {
+    for (uint32_t step = 0;; ++step) {
+      if constexpr (kPerStep) {
+        if (step >= p.n_steps)
+          break;
+        st.total = {0.f, 0.f, 0.f};
+      }
       Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
       if (st.n_light) {
         PrivateStack stack;
@@ -121,18 +134,25 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
         ++n_rays_traced;
       }
       if (st.has_next) {
-        PrivateStack stack;
-        next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
-                                 3.4028235e38f, -1.0f, stack);
-        ++n_rays_traced;
+        if (step == 0 && p.initial_idx) {
+          next_hit = given;
+        } else {
+          PrivateStack stack;
+          next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
+                                   3.4028235e38f, -1.0f, stack);
+          ++n_rays_traced;
+        }
       }
       PathState<float> ns;
       Ray<float> nl = light_ray, nn = next_ray;
       bool done = rendering_equation_iteration(st, rng, next_ray, light_hit,
                                                next_hit, p.settings,
                                                p.scene.view, &ns, &nl, &nn);
+      if constexpr (kPerStep)
+        steps[step] += ns.total;
       if (done) {
-        total += ns.total;
+        if constexpr (!kPerStep)
+          total += ns.total;
         break;
       }
       st = ns;
@@ -144,21 +164,12 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
   return total;
 }
 
-__global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
-                                                    uint32_t start_block) {
-  uint32_t block = blockIdx.x + start_block;
-  ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
-  V3<float> total = {0.f, 0.f, 0.f};
-  uint32_t traced = 0;
-  if (!info.exit) {
-    uint32_t location = info.x + info.y * p.wd.x_dim;
-    if (p.rng.type == kRngSobel)
-      total = integrate_samples<TableRng>(p, info, location, &traced);
-    else
-      total = integrate_samples<HashRng>(p, info, location, &traced);
-  }
-  // Per-pixel reduction over the sample block (lanes of one pixel are
-  // consecutive; groups larger than a wave go through LDS).
+// Per-pixel reduction over the sample block (lanes of one pixel are
+// consecutive; groups larger than a wave go through LDS), then the store of
+// the pixel's partial (divided by spp when there is one sample block).
+__device__ __forceinline__ void reduce_store(const MegaParams &p, const ThreadInfo &info,
+                                             uint32_t block, V3<float> total,
+                                             float *out) {
   uint32_t sbs = p.wd.sample_block_size;
   uint32_t group = sbs < 64u ? sbs : 64u;
   total.x = group_reduce_sum(total.x, group);
@@ -168,6 +179,7 @@ __global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
   if (sbs > 64u) {
     __shared__ float partial[16][3];
     uint32_t w = threadIdx.x >> 6;
+    __syncthreads(); // partial may still be read by a previous call
     if ((threadIdx.x & 63u) == 0) {
       partial[w][0] = total.x;
       partial[w][1] = total.y;
@@ -195,9 +207,36 @@ __global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
     size_t idx = size_t(p.wd.sample_block_idx(block)) +
                  size_t(p.wd.num_sample_blocks) *
                      (size_t(info.x) + size_t(info.y) * p.wd.x_dim);
-    p.out_rgb[3 * idx + 0] = total.x;
-    p.out_rgb[3 * idx + 1] = total.y;
-    p.out_rgb[3 * idx + 2] = total.z;
+    out[3 * idx + 0] = total.x;
+    out[3 * idx + 1] = total.y;
+    out[3 * idx + 2] = total.z;
+  }
+}
+
+template <bool kPerStep>
+__global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
+                                                    uint32_t start_block) {
+  uint32_t block = blockIdx.x + start_block;
+  ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
+  V3<float> total = {0.f, 0.f, 0.f};
+  V3<float> steps[kPerStep ? kMaxOutputSteps : 1];
+  if constexpr (kPerStep)
+    for (uint32_t i = 0; i < kMaxOutputSteps; ++i)
+      steps[i] = {0.f, 0.f, 0.f};
+  uint32_t traced = 0;
+  if (!info.exit) {
+    uint32_t location = info.x + info.y * p.wd.x_dim;
+    if (p.rng.type == kRngSobel)
+      total = integrate_samples<TableRng, kPerStep>(p, info, location, &traced, steps);
+    else
+      total = integrate_samples<HashRng, kPerStep>(p, info, location, &traced, steps);
+  }
+  if constexpr (kPerStep) {
+    const size_t image = size_t(p.wd.num_sample_blocks) * p.wd.x_dim * p.wd.y_dim * 3;
+    for (uint32_t i = 0; i < p.n_steps; ++i)
+      reduce_store(p, info, block, steps[i], p.out_rgb + i * image);
+  } else {
+    reduce_store(p, info, block, total, p.out_rgb);
   }
   if (p.ray_counter) {
     uint32_t lane_total = traced;
@@ -212,8 +251,12 @@ hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
                              uint32_t end_block, hipStream_t s) {
   if (end_block <= start_block)
     return hipSuccess;
-  hipLaunchKernelGGL(k_megakernel, dim3(end_block - start_block),
-                     dim3(p.wd.block_size), 0, s, p, start_block);
+  if (p.n_steps)
+    hipLaunchKernelGGL(k_megakernel<true>, dim3(end_block - start_block),
+                       dim3(p.wd.block_size), 0, s, p, start_block);
+  else
+    hipLaunchKernelGGL(k_megakernel<false>, dim3(end_block - start_block),
+                       dim3(p.wd.block_size), 0, s, p, start_block);
   return hipGetLastError();
 }
 

@@ -228,21 +228,37 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   if (spec.type == PT_SAMPLE_SQUARE_IMAGE) {
     x_dim = spec.x_dim;
     y_dim = spec.y_dim;
-  } else if (spec.type == PT_SAMPLE_INITIAL_RAYS) {
+  } else if (spec.type == PT_SAMPLE_INITIAL_RAYS ||
+             spec.type == PT_SAMPLE_INITIAL_IDX_AND_DIR) {
     if (!spec.rays)
-      return set_error(PT_ERR_INVALID, "InitialRays sample spec without rays");
+      return set_error(PT_ERR_INVALID, "initial-ray sample spec without rays");
+    if (spec.type == PT_SAMPLE_INITIAL_IDX_AND_DIR && !spec.initial_idx)
+      return set_error(PT_ERR_INVALID, "InitialIdxAndDir sample spec without triangle indices");
     x_dim = spec.n_rays;
     y_dim = 1;
   } else {
-    return set_error(PT_ERR_UNSUPPORTED,
-                     "InitialIdxAndDir sample spec is not supported by this entry point");
+    return set_error(PT_ERR_INVALID, "unknown sample spec type");
   }
   if (x_dim == 0 || y_dim == 0)
     return set_error(PT_ERR_INVALID, "empty image");
-  if (out.type == PT_OUTPUT_PER_STEP)
-    return set_error(PT_ERR_UNSUPPORTED, "per-step output is not supported yet");
-  if (!out.data)
+  const bool per_step = out.type == PT_OUTPUT_PER_STEP;
+  if (per_step) {
+    if (!out.per_step || out.n_steps == 0 || out.n_steps > gpu::kMaxOutputSteps)
+      return set_error(PT_ERR_INVALID, "per-step output needs 1 to " +
+                                           std::to_string(gpu::kMaxOutputSteps) +
+                                           " host buffers");
+    for (uint32_t i = 0; i < out.n_steps; ++i)
+      if (!out.per_step[i])
+        return set_error(PT_ERR_INVALID, "null per-step output buffer");
+  } else if (!out.data) {
     return set_error(PT_ERR_INVALID, "null output buffer");
+  }
+  if ((per_step || spec.type != PT_SAMPLE_SQUARE_IMAGE) &&
+      settings.kernel_approach != PT_MEGA_KERNEL &&
+      !(spec.type == PT_SAMPLE_INITIAL_RAYS && !per_step))
+    return set_error(PT_ERR_UNSUPPORTED,
+                     "per-step output and initial-intersection sample specs run on the "
+                     "MegaKernel integrator");
   if (settings.rng == PT_RNG_SOBEL &&
       (settings.max_sample_size == 0 || settings.max_sample_size > 20000))
     return set_error(PT_ERR_INVALID, "max_sample_size must be in [1, 20000]");
@@ -269,8 +285,17 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   const uint32_t n_pixels = x_dim * y_dim;
   HIP_TRY(counter.ensure(1));
   HIP_TRY(hipMemsetAsync(counter.ptr, 0, sizeof(unsigned long long), stream));
-  if (spec.type == PT_SAMPLE_INITIAL_RAYS)
+  if (spec.type != PT_SAMPLE_SQUARE_IMAGE)
     HIP_TRY(scratch_rays.upload(spec.rays, size_t(spec.n_rays) * 6, stream));
+  if (spec.type == PT_SAMPLE_INITIAL_IDX_AND_DIR) {
+    const uint32_t n_tris = uint32_t(scene.flat().tris.size());
+    for (uint32_t i = 0; i < spec.n_rays; ++i)
+      if (spec.initial_idx[i] >= n_tris)
+        return set_error(PT_ERR_INVALID, "InitialIdxAndDir triangle index " +
+                                             std::to_string(spec.initial_idx[i]) +
+                                             " out of range");
+    HIP_TRY(scratch_idx.upload(spec.initial_idx, spec.n_rays, stream));
+  }
   HIP_TRY(hipStreamSynchronize(stream));
 
   stats = pt_render_stats{};
@@ -285,15 +310,18 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
       return set_error(PT_ERR_INVALID, "invalid render work division settings");
     if (wd.block_size > 1024)
       return set_error(PT_ERR_INVALID, "block_size above 1024");
-    HIP_TRY(rgb.ensure(size_t(wd.num_sample_blocks) * n_pixels * 3));
+    const size_t image_floats = size_t(wd.num_sample_blocks) * n_pixels * 3;
+    HIP_TRY(rgb.ensure(image_floats * (per_step ? out.n_steps : 1)));
     gpu::MegaParams p{};
     p.scene = scene_store.dev;
     p.settings = integrator_settings(settings);
     p.wd = wd;
     p.rng = rng;
     std::memcpy(p.film_to_world, spec.film_to_world, sizeof(p.film_to_world));
-    p.initial_rays = spec.type == PT_SAMPLE_INITIAL_RAYS ? scratch_rays.ptr : nullptr;
+    p.initial_rays = spec.type != PT_SAMPLE_SQUARE_IMAGE ? scratch_rays.ptr : nullptr;
+    p.initial_idx = spec.type == PT_SAMPLE_INITIAL_IDX_AND_DIR ? scratch_idx.ptr : nullptr;
     p.out_rgb = rgb.ptr;
+    p.n_steps = per_step ? out.n_steps : 0;
     p.divide_by_spp = wd.num_sample_blocks == 1;
     p.ray_counter = counter.ptr;
     uint32_t total = wd.total_num_blocks();
@@ -314,10 +342,12 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
     }
     final_rgb = rgb.ptr;
     if (wd.num_sample_blocks != 1) {
-      HIP_TRY(rgb2.ensure(size_t(n_pixels) * 3));
-      HIP_TRY(gpu::launch_reduce_sample_blocks(rgb.ptr, wd.num_sample_blocks,
-                                               n_pixels, samples_per, rgb2.ptr,
-                                               stream));
+      const uint32_t n_images = per_step ? out.n_steps : 1;
+      HIP_TRY(rgb2.ensure(size_t(n_pixels) * 3 * n_images));
+      for (uint32_t i = 0; i < n_images; ++i)
+        HIP_TRY(gpu::launch_reduce_sample_blocks(rgb.ptr + i * image_floats,
+                                                 wd.num_sample_blocks, n_pixels, samples_per,
+                                                 rgb2.ptr + size_t(i) * n_pixels * 3, stream));
       final_rgb = rgb2.ptr;
     }
   } else {
@@ -335,6 +365,10 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   if (out.type == PT_OUTPUT_BGRA)
     HIP_TRY(hipMemcpyAsync(out.data, bgra.ptr, size_t(n_pixels) * 4,
                            hipMemcpyDeviceToHost, stream));
+  else if (per_step)
+    for (uint32_t i = 0; i < out.n_steps; ++i)
+      HIP_TRY(hipMemcpyAsync(out.per_step[i], final_rgb + size_t(i) * n_pixels * 3,
+                             size_t(n_pixels) * 12, hipMemcpyDeviceToHost, stream));
   else
     HIP_TRY(hipMemcpyAsync(out.data, final_rgb, size_t(n_pixels) * 12,
                            hipMemcpyDeviceToHost, stream));

@@ -195,16 +195,28 @@ class Renderer:
     def render(self, scene: Scene, film_to_world, x_dim: int, y_dim: int, samples_per: int,
                settings: Settings | None = None, output: str = "FloatRGB",
                show_progress: bool = False, show_times: bool = False,
-               rays: np.ndarray | None = None):
-        """Renderer::render with a SquareImage (or InitialRays if `rays` is
-        given, shape (n, 6)) sample spec. Returns (image, seconds)."""
+               rays: np.ndarray | None = None, initial_idx: np.ndarray | None = None,
+               n_steps: int = 8):
+        """Renderer::render. Sample spec: SquareImage, or InitialRays if `rays`
+        (n, 6) is given, or InitialIdxAndDir if `initial_idx` (n,) is given
+        too (the first intersection is that triangle at distance 1 along each
+        ray). Output "BGRA", "FloatRGB" or "OutputPerStep" (n_steps images,
+        shape (n_steps, y, x, 3); MegaKernel). Returns (image, seconds)."""
         settings = settings or Settings()
         spec = L.PtSampleSpec()
+        keep = []
         if rays is not None:
             rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+            keep.append(rays)
             spec.type = L.SAMPLE_SPEC["InitialRays"]
             spec.rays = L.fptr(rays)
             spec.n_rays = rays.shape[0]
+            if initial_idx is not None:
+                idx = np.ascontiguousarray(initial_idx, np.uint32).ravel()
+                assert idx.size == rays.shape[0]
+                keep.append(idx)
+                spec.type = L.SAMPLE_SPEC["InitialIdxAndDir"]
+                spec.initial_idx = L.uptr(idx)
             x_dim, y_dim = rays.shape[0], 1
         else:
             spec.type = L.SAMPLE_SPEC["SquareImage"]
@@ -214,9 +226,16 @@ class Renderer:
         out.type = L.OUTPUT[output]
         if output == "BGRA":
             img = np.zeros((y_dim, x_dim), np.uint32)
+            out.data = img.ctypes.data
+        elif output == "OutputPerStep":
+            img = np.zeros((n_steps, y_dim, x_dim, 3), np.float32)
+            ptrs = (C.POINTER(C.c_float) * n_steps)(*[L.fptr(img[i]) for i in range(n_steps)])
+            keep.append(ptrs)
+            out.per_step = ptrs
+            out.n_steps = n_steps
         else:
             img = np.zeros((y_dim, x_dim, 3), np.float32)
-        out.data = img.ctypes.data
+            out.data = img.ctypes.data
         secs = C.c_double()
         L.check(L.lib().pt_render(self.handle, C.byref(spec), C.byref(out), scene.handle,
                                   samples_per, C.byref(settings.raw), int(show_progress),

@@ -85,3 +85,41 @@ def test_streaming_default_settings_multiplier_func(renderer, cornell):
     ref, _ = O.render(scene.flat(), ftw, s, 16, 40, 40, tab, precision=32)
     assert np.array_equal(mk.view(np.uint32), ref.view(np.uint32))
     assert np.array_equal(wf.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("n_steps", [3, 12])
+def test_output_per_step_bit_exact(renderer, cornell, n_steps):
+    """OutputPerStep (rendering_equation.h:56-123): each step image equals the
+    CPU checker's bit for bit."""
+    from oracle import oracle as O
+    scene, ftw = cornell
+    s = _settings("MegaKernel")
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, 16)
+    gpu, _ = renderer.render(scene, ftw, 48, 40, 16, s, output="OutputPerStep", n_steps=n_steps)
+    cpu, _ = O.render(scene.flat(), ftw, s, 16, 48, 40, tab, 32, 0, n_steps=n_steps)
+    assert gpu.shape == cpu.shape == (n_steps, 40, 48, 3)
+    assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32))
+    full, _ = renderer.render(scene, ftw, 48, 40, 16, s)
+    if n_steps >= 8:
+        assert np.allclose(gpu.sum(0), full, rtol=1e-5, atol=1e-6)
+
+
+def test_initial_rays_and_idx_bit_exact(renderer, cornell_sphere):
+    from oracle import oracle as O
+    scene, _ = cornell_sphere
+    flat = scene.flat()
+    tris = flat["tris"].reshape(-1, 3, 3)
+    rng = np.random.default_rng(3)
+    pick = rng.choice(len(tris), 2000, replace=False).astype(np.uint32)
+    v0, e1, e2 = tris[pick, 0], tris[pick, 1], tris[pick, 2]
+    c = v0 + (e1 + e2) / 3.0
+    n = np.cross(e1, e2)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    rays = np.concatenate([c + n, -n], 1).astype(np.float32)
+    s = _settings("MegaKernel")
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, 8)
+    for idx in (None, pick):
+        gpu, _ = renderer.render(scene, None, 0, 0, 8, s, rays=rays, initial_idx=idx)
+        cpu, _ = O.render(flat, np.zeros(12, np.float32), s, 8, len(rays), 1, tab, 32, 0,
+                          initial_rays=rays, initial_idx=idx)
+        assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), idx is None
