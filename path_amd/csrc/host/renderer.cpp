@@ -37,7 +37,7 @@ const FlatScene &pt_scene::flat() const {
 
 const Bvh &pt_scene::bvh() const {
   if (bvh_gen_ != generation) {
-    bvh_ = build_sah_bvh(flat().tris);
+    bvh_ = build_sah_bvh(flat().tris, BvhSettings{kGpuSahTraversalCost});
     bvh_gen_ = generation;
   }
   return bvh_;
