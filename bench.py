@@ -130,7 +130,7 @@ def bench_datagen(a, cfg):
 
     # per-scene launches: the same scenes, one render call each
     single = settings.copy()
-    set_launch_params(r, 48, 1, False)
+    set_launch_params(r, 8, 1, False)
     for g in gens[:4]:
         r.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp, single)
     sync()
@@ -570,7 +570,7 @@ def main(argv=None):
     r = Renderer(0)
     if a.trace_blocks_per_cu or a.iters_per_sync:
         from path_amd.datagen import set_launch_params
-        set_launch_params(r, a.trace_blocks_per_cu or 48, a.iters_per_sync or 1,
+        set_launch_params(r, a.trace_blocks_per_cu or 8, a.iters_per_sync or 1,
                           bool(a.iters_per_sync))
     size, spp = cfg["size"], cfg["spp"]
     for _ in range(a.warmup):

@@ -63,22 +63,35 @@ struct LdsStack {
   }
 };
 
-// Returns the closest hit with t <= t_limit. If occlude_below > 0, stops at
-// the first hit closer than occlude_below (any such hit makes a shadow ray's
-// light sample fail its distance test, so the caller cannot tell the
-// difference).
-template <class Stack>
-__device__ __forceinline__ Hit<float>
-trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
-              const Ray<float> &ray, float t_limit, float occlude_below,
-              Stack &stack) {
-  V3<float> inv = inv_direction(ray.d);
-  V3<float> oi = {ray.o.x * inv.x, ray.o.y * inv.y, ray.o.z * inv.z};
-  float best_t = t_limit;
-  uint32_t best_idx = kNoHit;
-  uint32_t best_back = 0;
-  int node = 0;
-  for (;;) {
+// Resumable closest-hit traversal of one ray: init() then step() until it
+// returns true. One step visits one inner node or one leaf, so a caller can
+// interleave rays (lane refill). The closest hit is kept with t <= t_limit;
+// if occlude_below > 0 the traversal stops at the first hit closer than
+// occlude_below (any such hit makes a shadow ray's light sample fail its
+// distance test, so the caller cannot tell the difference).
+struct TraceState {
+  Ray<float> ray;
+  V3<float> inv, oi;
+  float best_t, occlude_below;
+  uint32_t best_idx, best_back;
+  int node;
+
+  __device__ __forceinline__ void init(const Ray<float> &r, float t_limit, float occl) {
+    ray = r;
+    inv = inv_direction(r.d);
+    oi = {r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z};
+    best_t = t_limit;
+    occlude_below = occl;
+    best_idx = kNoHit;
+    best_back = 0;
+    node = 0;
+  }
+
+  __device__ __forceinline__ Hit<float> hit() const { return {best_t, best_idx, best_back}; }
+
+  template <class Stack>
+  __device__ __forceinline__ bool step(const float4 *__restrict__ nodes,
+                                       const float4 *__restrict__ tris, Stack &stack) {
     if (node >= 0) {
       const float4 *np = nodes + kNodeFloat4 * size_t(node);
       const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -118,9 +131,9 @@ trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
 #undef PT_CSWAP
       if (dist[0] == __builtin_inff()) {
         if (stack.empty())
-          break;
+          return true;
         node = stack.pop();
-        continue;
+        return false;
       }
       if (dist[3] != __builtin_inff())
         stack.push(ref[3]);
@@ -129,21 +142,19 @@ trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
       if (dist[1] != __builtin_inff())
         stack.push(ref[1]);
       node = ref[0];
-      continue;
+      return false;
     }
     // leaf
     uint32_t ti = uint32_t(~node);
     for (;;) {
       const float4 *tp = tris + 3 * ti;
-      float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-      uint32_t word = __float_as_uint(p0.w);
-      uint32_t idx = word & ~kLeafEnd;
-      TriGeom<float> tri = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z},
-                            {p2.x, p2.y, p2.z}};
+      const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+      const uint32_t word = __float_as_uint(p0.w);
+      const uint32_t idx = word & ~kLeafEnd;
+      const TriGeom<float> tri = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z}};
       float t;
       uint32_t back;
-      if (intersect_triangle(ray, tri, &t, &back) &&
-          hit_before(t, idx, best_t, best_idx)) {
+      if (intersect_triangle(ray, tri, &t, &back) && hit_before(t, idx, best_t, best_idx)) {
         best_t = t;
         best_idx = idx;
         best_back = back;
@@ -153,12 +164,24 @@ trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
       ++ti;
     }
     if (best_idx != kNoHit && best_t < occlude_below)
-      break;
+      return true;
     if (stack.empty())
-      break;
+      return true;
     node = stack.pop();
+    return false;
   }
-  return {best_t, best_idx, best_back};
+};
+
+template <class Stack>
+__device__ __forceinline__ Hit<float>
+trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
+              const Ray<float> &ray, float t_limit, float occlude_below, Stack &stack) {
+  TraceState ts;
+  ts.init(ray, t_limit, occlude_below);
+  stack.sp = 0;
+  while (!ts.step(nodes, tris, stack)) {
+  }
+  return ts.hit();
 }
 
 // Hit record packed in 8 bytes: t bits, index with the back-face flag in

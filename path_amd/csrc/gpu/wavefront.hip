@@ -328,6 +328,16 @@ hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
 // Persistent traversal. Static schedule: wave w takes the 64-entry batches
 // w, w + W, w + 2W, ... of its queue (no same-address atomics).
 // ---------------------------------------------------------------------------
+// Lanes are refilled once at least this many are idle (or all are): a refill
+// runs the ray-setup code for the whole wave, so batching it pays (measured
+// 436 / 446 / 446 / 442 Msamples/s for 1 / 8 / 16 / 32 on the 82k-triangle
+// scene).
+constexpr uint32_t kRefillMinIdle = 16;
+
+// Each wave owns one contiguous range of the queue and keeps its 64 lanes
+// busy: a lane whose ray finished takes the next entry of the range (ballot +
+// mbcnt, no atomics) while the other lanes continue, so a long ray delays
+// only itself.
 template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStack * 256];
@@ -335,31 +345,52 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
   const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
   uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t base = (gtid >> 6) * 64u; base < count; base += n_waves * 64u) {
-    uint32_t i = base + lane_id();
-    if (i < count) {
-      uint32_t slot = q[i];
-      float4 o = p.st.origin[slot];
-      float4 d = kShadow ? p.st.light_dir[slot] : p.st.dir[slot];
-      Ray<float> ray = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
-      float limit = kShadow ? d.w + 2e-6f : 3.4028235e38f;
-      float occl = kShadow ? d.w - 2e-6f : -1.0f;
-      const float4 *nodes = p.scene.nodes, *tris = p.scene.tris;
-      bool has_nodes = true;
-      if constexpr (kBatch) {
-        const SceneDesc &sd = p.batch.scenes[p.st.scene[slot]];
-        nodes = p.batch.nodes + kNodeFloat4 * size_t(sd.node_off);
-        tris = p.batch.tris + 3 * size_t(sd.tri_off);
-        has_nodes = sd.has_nodes != 0;
+  const uint32_t wave = gtid >> 6, n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t per = (count + n_waves - 1) / n_waves;
+  uint32_t next = min(wave * per, count);
+  const uint32_t end = min(next + per, count);
+  if (next >= end)
+    return; // whole wave
+  LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid, p.overflow_stride};
+  TraceState ts;
+  const float4 *nodes = p.scene.nodes, *tris = p.scene.tris;
+  uint32_t slot = 0;
+  bool has = false;
+  for (;;) {
+    const uint64_t idle = __ballot(!has);
+    const uint32_t n_idle = uint32_t(__popcll(idle));
+    if (next < end && (n_idle >= kRefillMinIdle || n_idle == 64u)) {
+      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+          uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      if (!has && next + pre < end) {
+        slot = q[next + pre];
+        const float4 o = p.st.origin[slot];
+        const float4 d = kShadow ? p.st.light_dir[slot] : p.st.dir[slot];
+        const Ray<float> ray = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
+        bool has_nodes = true;
+        if constexpr (kBatch) {
+          const SceneDesc &sd = p.batch.scenes[p.st.scene[slot]];
+          nodes = p.batch.nodes + kNodeFloat4 * size_t(sd.node_off);
+          tris = p.batch.tris + 3 * size_t(sd.tri_off);
+          has_nodes = sd.has_nodes != 0;
+        }
+        ts.init(ray, kShadow ? d.w + 2e-6f : 3.4028235e38f, kShadow ? d.w - 2e-6f : -1.0f);
+        stack.sp = 0;
+        if (has_nodes)
+          has = true;
+        else
+          hits[slot] = pack_hit(Hit<float>{0.f, kNoHit, 0});
       }
-      Hit<float> h = {0.f, kNoHit, 0};
-      if (has_nodes) {
-        LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid,
-                       p.overflow_stride};
-        h = trace_closest(nodes, tris, ray, limit, occl, stack);
-      }
-      hits[slot] = pack_hit(h);
+      next = min(next + n_idle, end);
+    }
+    if (__ballot(has) == 0) {
+      if (next >= end)
+        break;
+      continue;
+    }
+    if (has && ts.step(nodes, tris, stack)) {
+      hits[slot] = pack_hit(ts.hit());
+      has = false;
     }
   }
 }

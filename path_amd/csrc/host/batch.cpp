@@ -286,7 +286,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[4]) {
   const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
-  const uint32_t blocks[] = {16, 48, 96};
+  const uint32_t blocks[] = {4, 8, 16};
   const uint32_t iters[] = {1, 4};
   double best = 1e30;
   uint32_t bp = settings.max_num_samples_per_launch, bb = r.trace_blocks_per_cu,

@@ -195,7 +195,7 @@ struct pt_renderer {
   pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
-  uint32_t trace_blocks_per_cu = 48; // oversubscribed: the dispatcher balances uneven rays
+  uint32_t trace_blocks_per_cu = 8; // persistent waves; lanes refill from the wave's queue range
   uint32_t iters_per_sync = 1;
   bool use_graphs = false;
 

@@ -22,7 +22,7 @@ def _single(renderer, g, spp, settings):
 
 def test_batch_equals_single_scene_renders(renderer, gen_scenes):
     from path_amd.datagen import datagen_settings, render_batch, set_launch_params
-    set_launch_params(renderer, 48, 1, False)
+    set_launch_params(renderer, 8, 1, False)
     st = datagen_settings()
     images, gbufs, secs = render_batch(renderer, gen_scenes, SPP, st)
     stats = renderer.last_stats()
@@ -39,7 +39,7 @@ def test_batch_small_pool_and_graphs_bit_exact(renderer, gen_scenes):
     number of iterations between host syncs and hipGraph replay."""
     from path_amd.datagen import datagen_settings, render_batch, set_launch_params
     st = datagen_settings()
-    set_launch_params(renderer, 48, 1, False)
+    set_launch_params(renderer, 8, 1, False)
     base, _, _ = render_batch(renderer, gen_scenes, SPP, st, gbuffer=False)
     for blocks, iters, graphs, pool in [(4, 4, True, 3000), (16, 2, True, 1 << 20),
                                         (8, 3, False, 777)]:
@@ -49,7 +49,7 @@ def test_batch_small_pool_and_graphs_bit_exact(renderer, gen_scenes):
         imgs, _, _ = render_batch(renderer, gen_scenes, SPP, s2, gbuffer=False)
         for a, b in zip(base, imgs):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (blocks, iters, graphs, pool)
-    set_launch_params(renderer, 48, 1, False)
+    set_launch_params(renderer, 8, 1, False)
 
 
 def test_gbuffer_first_hit(renderer, gen_scenes):
