@@ -89,6 +89,8 @@ struct TraceState {
 
   __device__ __forceinline__ Hit<float> hit() const { return {best_t, best_idx, best_back}; }
 
+  // Hit children are visited nearest first, also for occlusion rays (visiting
+  // them in slot order measured 1% slower on the 82k-triangle scene).
   template <class Stack>
   __device__ __forceinline__ bool step(const float4 *__restrict__ nodes,
                                        const float4 *__restrict__ tris, Stack &stack) {

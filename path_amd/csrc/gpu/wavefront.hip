@@ -142,13 +142,18 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
   float light_target = 0.f;
   PathState<float> ns;
   uint32_t location = meta.x, sample = meta.y, counter = meta.z;
+  // Fields that a step leaves unchanged are neither re-read nor re-written:
+  // the light multiplier exists only while a light sample is pending, the
+  // continuation direction only while the path continues, and the radiance
+  // total is stored only when it changed.
+  bool total_dirty = false;
   if (active) {
     SlotScene sc = slot_scene<kBatch>(p, scene);
     float4 m = p.st.mult[slot];
-    float4 lm = p.st.light[slot];
+    float4 lm = (meta.w & kFlagLight) ? p.st.light[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 tot = p.st.total[slot];
     float4 o = p.st.origin[slot];
-    float4 d = p.st.dir[slot];
+    float4 d = (meta.w & kFlagHasNext) ? p.st.dir[slot] : o;
     PathState<float> st;
     st.iters = meta.w & kItersMask;
     st.count_emission = (meta.w & kFlagCountEmission) ? 1u : 0u;
@@ -183,6 +188,7 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
       g[9] = mat.emission.x, g[10] = mat.emission.y, g[11] = mat.emission.z;
     }
     counter = rng_counter(rng);
+    total_dirty = ns.total.x != tot.x || ns.total.y != tot.y || ns.total.z != tot.z;
     if (finished) {
       float *out = p.out_samples +
                    3 * (size_t(sc.pixel_off + location) * p.rng.samples_per + sample);
@@ -244,6 +250,7 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
                                   float(py) + yo, sc.x_dim, sc.y_dim);
       counter = rng_counter(rng);
       ns = initial_path_state<float>();
+      total_dirty = true;
       push_ext = true;
       push_light = false;
       ext_o = cam.o;
@@ -262,11 +269,14 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
       p.st.scene[slot] = scene;
     p.st.mult[slot] = make_float4(ns.ray_mult.x, ns.ray_mult.y, ns.ray_mult.z,
                                   ns.light_target);
-    p.st.light[slot] = make_float4(ns.light_mult.x, ns.light_mult.y,
-                                   ns.light_mult.z, 0.f);
-    p.st.total[slot] = make_float4(ns.total.x, ns.total.y, ns.total.z, 0.f);
+    if (push_light)
+      p.st.light[slot] = make_float4(ns.light_mult.x, ns.light_mult.y,
+                                     ns.light_mult.z, 0.f);
+    if (total_dirty)
+      p.st.total[slot] = make_float4(ns.total.x, ns.total.y, ns.total.z, 0.f);
     p.st.origin[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.f);
-    p.st.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
+    if (push_ext)
+      p.st.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
     if (push_light)
       p.st.light_dir[slot] =
           make_float4(light_d.x, light_d.y, light_d.z, light_target);
@@ -333,6 +343,7 @@ hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
 // 436 / 446 / 446 / 442 Msamples/s for 1 / 8 / 16 / 32 on the 82k-triangle
 // scene).
 constexpr uint32_t kRefillMinIdle = 16;
+
 
 // Each wave owns one contiguous range of the queue and keeps its 64 lanes
 // busy: a lane whose ray finished takes the next entry of the range (ballot +
