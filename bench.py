@@ -72,12 +72,22 @@ def _roofline(name, ms, launches, n_rays, bpr, config):
     tf = ROOT / "profiles" / f"{name}_{config}_traffic.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("bytes_per_dispatch")
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-            "traffic": traffic, "kernel": name,
-            "avg_launch_ms": round(avg_s * 1e3, 4),
-            "algorithmic_bytes_per_ray": round(bpr, 1),
-            "rays_per_launch": round(n_rays / max(launches, 1), 1)}
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+           "traffic": traffic, "kernel": name,
+           "avg_launch_ms": round(avg_s * 1e3, 4),
+           "algorithmic_bytes_per_ray": round(bpr, 1),
+           "rays_per_launch": round(n_rays / max(launches, 1), 1)}
+    if traffic and avg_s > 0:
+        # what the memory system actually moved (scaled FETCH + WRITE counters)
+        out["measured_traffic_gbps"] = round(traffic / avg_s / 1e9, 2)
+        out["measured_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBPS, 5)
+    if out["frac"] > 1.0:
+        out["note"] = ("algorithmic bytes follow the reference's binary-BVH traversal "
+                       "(target_objects = 4), which visits more nodes than the 4-wide "
+                       "near-first traversal here, and most reads hit L2 / Infinity Cache: "
+                       "frac > 1 is not a bound; measured_frac is the HBM-side figure")
+    return out
 
 
 def _bytes_per_ray(counts, kind):
@@ -382,7 +392,8 @@ def bench_mlp(a, cfg):
     size = cfg["size"]
     stm = st.copy()
     stm.kernel_approach = "Streaming"
-    gt, _ = r.render(g.scene, g.film_to_world, size, size, cfg["gt_spp"], stm)
+    r.render(g.scene, g.film_to_world, size, size, cfg["gt_spp"], stm)
+    gt = r.last_render_fp64(size, size)  # per-sample radiance summed in FP64
 
     def curve(render, spps):
         out = []

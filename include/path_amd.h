@@ -366,6 +366,11 @@ int pt_render_mlp(pt_renderer *r, pt_mlp *m, const pt_scene *scene,
                   uint32_t samples_per, const pt_settings *settings,
                   float *out_rgb, double *seconds);
 
+/* Per-pixel means of the last Streaming SquareImage render re-summed in FP64
+ * (sequentially over each pixel's per-sample radiance, in sample order):
+ * host out_rgb of x_dim*y_dim*3 doubles. For ground-truth images. */
+int pt_last_render_fp64(pt_renderer *r, double *out_rgb);
+
 /* Statistics of the last pt_render. kernel_ms spans the launches with HIP
  * events on the renderer's stream. The per-kernel fields are filled only when
  * profiling is enabled (pt_set_profiling), which brackets each traversal or

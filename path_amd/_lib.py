@@ -156,6 +156,7 @@ SIGNATURES = {
     "pt_mlp_train": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F,
                                C.POINTER(C.c_double)]),
     "pt_mlp_time_forward": (C.c_int, [_P, C.c_uint32, C.c_uint32, _F, _F]),
+    "pt_last_render_fp64": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "pt_render_mlp": (C.c_int, [_P, _P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(PtSettings), _F, C.POINTER(C.c_double)]),
 }

@@ -440,6 +440,32 @@ int pt_render_adjoint(pt_renderer *r, const pt_scene *scene, const float film_to
   PT_GUARD_END
 }
 
+int pt_last_render_fp64(pt_renderer *r, double *out_rgb) {
+  PT_GUARD_BEGIN
+  if (!r || !out_rgb)
+    return fail(PT_ERR_INVALID, "null argument");
+  if (r->last_wf_spp == 0)
+    return fail(PT_ERR_INVALID,
+                "no Streaming SquareImage render to re-sum (render with kernel_approach "
+                "Streaming first)");
+  const uint32_t n = r->last_wf_x * r->last_wf_y;
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = r->sum64.ensure(size_t(n) * 3);
+  if (e == hipSuccess)
+    e = gpu::launch_sum_samples_f64(r->wf_samples.ptr, n, r->last_wf_spp, r->sum64.ptr,
+                                    r->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_rgb, r->sum64.ptr, size_t(n) * 3 * sizeof(double),
+                       hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("fp64 re-sum: ") + hipGetErrorString(e));
+  return PT_OK;
+  PT_GUARD_END
+}
+
 int pt_set_profiling(pt_renderer *r, int enable) {
   if (!r)
     return fail(PT_ERR_INVALID, "null renderer");

@@ -90,6 +90,9 @@ hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,
 hipError_t launch_wf_reduce(const float *samples, const WorkDivision &wd,
                             uint32_t samples_per, float *out_rgb,
                             hipStream_t s);
+// out[pixel] = (sum over the pixel's samples, sequential in FP64) / spp
+hipError_t launch_sum_samples_f64(const float *samples, uint32_t n_pixels,
+                                  uint32_t samples_per, double *out, hipStream_t s);
 
 } // namespace gpu
 } // namespace pt

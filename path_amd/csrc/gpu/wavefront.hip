@@ -488,6 +488,30 @@ __global__ __launch_bounds__(1024) void k_wf_reduce(const float *__restrict__ sa
   }
 }
 
+__global__ void k_sum_samples_f64(const float *__restrict__ samples, uint32_t n_pixels,
+                                  uint32_t spp, double *__restrict__ out) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= n_pixels)
+    return;
+  double r = 0.0, g = 0.0, b = 0.0;
+  const float *v = samples + 3 * size_t(px) * spp;
+  for (uint32_t s = 0; s < spp; ++s) {
+    r += double(v[3 * s]);
+    g += double(v[3 * s + 1]);
+    b += double(v[3 * s + 2]);
+  }
+  out[3 * size_t(px)] = r / double(spp);
+  out[3 * size_t(px) + 1] = g / double(spp);
+  out[3 * size_t(px) + 2] = b / double(spp);
+}
+
+hipError_t launch_sum_samples_f64(const float *samples, uint32_t n_pixels,
+                                  uint32_t samples_per, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_samples_f64, dim3((n_pixels + 255) / 256), dim3(256), 0, s,
+                     samples, n_pixels, samples_per, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_wf_reduce(const float *samples, const WorkDivision &wd,
                             uint32_t samples_per, float *out_rgb,
                             hipStream_t s) {

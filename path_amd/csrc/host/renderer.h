@@ -191,6 +191,10 @@ struct pt_renderer {
   // adjoint buffers
   pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;
 
+  // last Streaming SquareImage render (for the FP64 re-summation)
+  uint32_t last_wf_x = 0, last_wf_y = 0, last_wf_spp = 0;
+  pt::host::DeviceBuf<double> sum64;
+
   // radiance-cache render buffers
   pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features;
   int n_cus = 256;

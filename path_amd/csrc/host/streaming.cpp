@@ -229,6 +229,9 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
   }
   *final_rgb = out;
   *rays_out = r.stats.ext_rays + r.stats.shadow_rays;
+  r.last_wf_x = x_dim;
+  r.last_wf_y = y_dim;
+  r.last_wf_spp = samples_per;
   return PT_OK;
 }
 

@@ -242,6 +242,13 @@ class Renderer:
                                   int(show_times), C.byref(secs)))
         return img, secs.value
 
+    def last_render_fp64(self, x_dim: int, y_dim: int) -> np.ndarray:
+        """The last Streaming SquareImage render re-summed per pixel in FP64."""
+        out = np.zeros((y_dim, x_dim, 3), np.float64)
+        L.check(L.lib().pt_last_render_fp64(self.handle,
+                                            out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def intersect(self, scene: Scene, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
         n = rays.shape[0]

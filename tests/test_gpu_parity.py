@@ -123,3 +123,12 @@ def test_initial_rays_and_idx_bit_exact(renderer, cornell_sphere):
         cpu, _ = O.render(flat, np.zeros(12, np.float32), s, 8, len(rays), 1, tab, 32, 0,
                           initial_rays=rays, initial_idx=idx)
         assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), idx is None
+
+
+def test_fp64_resum_of_streaming_render(renderer, cornell):
+    scene, ftw = cornell
+    s = _settings("Streaming")
+    img, _ = renderer.render(scene, ftw, 40, 32, 64, s)
+    img64 = renderer.last_render_fp64(40, 32)
+    assert img64.dtype == np.float64
+    assert np.allclose(img64, img, rtol=2e-6, atol=1e-7)
