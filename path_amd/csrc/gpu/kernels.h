@@ -16,7 +16,7 @@ constexpr uint32_t kMaxOutputSteps = 32;
 
 // Device copy of one flattened scene.
 struct DeviceScene {
-  const float4 *nodes = nullptr; // GpuNode array (8 float4 each)
+  const float4 *nodes = nullptr; // GpuNode array (4 float4 each)
   const float4 *tris = nullptr;  // GpuTri array in leaf order (3 float4 each)
   SceneView<float> view{};       // shading data in flattened order
   uint32_t n_tris = 0;

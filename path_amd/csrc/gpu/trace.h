@@ -1,7 +1,7 @@
 // BVH traversal on gfx950: one ray per lane, near-first order, per-lane stack.
 //
-// 4-wide nodes (128 B: six float4 of child-slot boxes, child refs, counts) and
-// leaf-ordered triangles are produced by host/bvh.cpp. Child refs >= 0 are
+// 4-wide nodes (64 B: grid origin and exponents, child refs, 8-bit quantised
+// child boxes) and leaf-ordered triangles are produced by host/bvh.cpp. Child refs >= 0 are
 // inner nodes, refs < 0 are ~(first triangle of a leaf), INT32_MIN is an empty
 // slot; the last triangle of a leaf has bit 31 of its index word set.
 //
@@ -22,7 +22,7 @@ namespace gpu {
 
 constexpr uint32_t kLeafEnd = 0x80000000u;
 constexpr int kEmptyChildRef = INT32_MIN; // host/bvh.h kEmptyChild
-constexpr size_t kNodeFloat4 = 8;         // 128-byte nodes
+constexpr size_t kNodeFloat4 = 4;         // 64-byte nodes
 
 // Stack in private (scratch) memory: used where a kernel has no LDS budget.
 struct PrivateStack {
@@ -96,19 +96,31 @@ struct TraceState {
                                        const float4 *__restrict__ tris, Stack &stack) {
     if (node >= 0) {
       const float4 *np = nodes + kNodeFloat4 * size_t(node);
-      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      const int4 ch = __builtin_bit_cast(int4, np[6]);
-      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+      const float4 f0 = np[0];
+      const int4 ch = __builtin_bit_cast(int4, np[1]);
+      const uint4 qa = __builtin_bit_cast(uint4, np[2]);
+      const uint4 qb = __builtin_bit_cast(uint4, np[3]);
+      // plane of code q on axis a: fma(q, 2^e_a, origin_a); in ray distance
+      // t = (plane - o) / d  ~  fma(q, 2^e_a / d_a, (origin_a - o_a) / d_a)
+      const uint32_t ew = __float_as_uint(f0.w);
+      const auto pow2 = [](uint32_t byte) {
+        return __uint_as_float(uint32_t(int32_t(int8_t(byte & 0xffu)) + 127) << 23);
+      };
+      const float sx = pow2(ew) * inv.x, sy = pow2(ew >> 8) * inv.y, sz = pow2(ew >> 16) * inv.z;
+      const float bx = fmaf(f0.x, inv.x, -oi.x), by = fmaf(f0.y, inv.y, -oi.y),
+                  bz = fmaf(f0.z, inv.z, -oi.z);
       const int chs[4] = {ch.x, ch.y, ch.z, ch.w};
       float dist[4];
       int ref[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float x0 = fmaf(lxs[k], inv.x, -oi.x), x1 = fmaf(hxs[k], inv.x, -oi.x);
-        const float y0 = fmaf(lys[k], inv.y, -oi.y), y1 = fmaf(hys[k], inv.y, -oi.y);
-        const float z0 = fmaf(lzs[k], inv.z, -oi.z), z1 = fmaf(hzs[k], inv.z, -oi.z);
+        const uint32_t sh = 8u * uint32_t(k);
+        const float x0 = fmaf(float((qa.x >> sh) & 0xffu), sx, bx);
+        const float x1 = fmaf(float((qa.y >> sh) & 0xffu), sx, bx);
+        const float y0 = fmaf(float((qa.z >> sh) & 0xffu), sy, by);
+        const float y1 = fmaf(float((qa.w >> sh) & 0xffu), sy, by);
+        const float z0 = fmaf(float((qb.x >> sh) & 0xffu), sz, bz);
+        const float z1 = fmaf(float((qb.y >> sh) & 0xffu), sz, bz);
         const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
         const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t));
         const bool hit = tmin <= tmax && chs[k] != kEmptyChildRef;

@@ -1,6 +1,7 @@
 #include "bvh.h"
 
 #include <algorithm>
+#include <stdexcept>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -236,31 +237,70 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
         kids[nk++] = open.b;
       }
     }
-    GpuNode o{};
-    for (uint32_t k = 0; k < 4; ++k) {
-      if (k >= nk) {
-        o.lo_x[k] = o.lo_y[k] = o.lo_z[k] = 1.f;
-        o.hi_x[k] = o.hi_y[k] = o.hi_z[k] = -1.f;
-        o.child[k] = kEmptyChild;
-        continue;
-      }
+    // child boxes (padded), then quantised to the node's grid
+    float clo[4][3], chi[4][3];
+    int32_t child[4];
+    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = 0; k < nk; ++k) {
       const BvhNode &n = bvh.nodes[kids[k]];
-      o.lo_x[k] = n.lo.x - pad;
-      o.hi_x[k] = n.hi.x + pad;
-      o.lo_y[k] = n.lo.y - pad;
-      o.hi_y[k] = n.hi.y + pad;
-      o.lo_z[k] = n.lo.z - pad;
-      o.hi_z[k] = n.hi.z + pad;
+      const float lo[3] = {n.lo.x - pad, n.lo.y - pad, n.lo.z - pad};
+      const float hi[3] = {n.hi.x + pad, n.hi.y + pad, n.hi.z + pad};
+      for (int a = 0; a < 3; ++a) {
+        clo[k][a] = lo[a];
+        chi[k][a] = hi[a];
+        nlo[a] = std::min(nlo[a], lo[a]);
+        nhi[a] = std::max(nhi[a], hi[a]);
+      }
       if (n.is_leaf) {
-        o.child[k] = ~int32_t(n.a);
-        o.count[k] = int32_t(n.b - n.a);
+        child[k] = ~int32_t(n.a);
       } else {
         const int32_t slot = int32_t(g.nodes.size());
         g.nodes.emplace_back();
-        o.child[k] = slot;
+        child[k] = slot;
         todo.push_back({kids[k], slot, it.depth + 1});
       }
     }
+    GpuNode o{};
+    uint8_t *qlo[3] = {o.lo_x, o.lo_y, o.lo_z}, *qhi[3] = {o.hi_x, o.hi_y, o.hi_z};
+    for (int a = 0; a < 3; ++a) {
+      // smallest power of two with 254 steps covering the extent (one step
+      // of slack so outward rounding never needs code 256)
+      const float extent = nhi[a] - nlo[a];
+      int e = -126;
+      if (extent > 0.f) {
+        int ex;
+        std::frexp(extent / 254.0f, &ex);
+        e = std::max(ex, -126);
+        while (std::ldexp(254.0f, e - 1) >= extent && e > -126)
+          --e;
+      }
+      const float scale = std::ldexp(1.0f, e);
+      o.origin[a] = nlo[a];
+      o.exponent[a] = int8_t(e);
+      for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= nk) {
+          qlo[a][k] = 255;
+          qhi[a][k] = 0;
+          continue;
+        }
+        int ql = int(std::floor((clo[k][a] - nlo[a]) / scale));
+        int qh = int(std::ceil((chi[k][a] - nlo[a]) / scale));
+        ql = std::clamp(ql, 0, 255);
+        qh = std::clamp(qh, 0, 255);
+        // outward on the exact float decode
+        while (ql > 0 && std::fmaf(float(ql), scale, nlo[a]) > clo[k][a])
+          --ql;
+        while (qh < 255 && std::fmaf(float(qh), scale, nlo[a]) < chi[k][a])
+          ++qh;
+        if (std::fmaf(float(ql), scale, nlo[a]) > clo[k][a] ||
+            std::fmaf(float(qh), scale, nlo[a]) < chi[k][a])
+          throw std::runtime_error("BVH node quantisation is not conservative");
+        qlo[a][k] = uint8_t(ql);
+        qhi[a][k] = uint8_t(qh);
+      }
+    }
+    for (uint32_t k = 0; k < 4; ++k)
+      o.child[k] = k < nk ? child[k] : kEmptyChild;
     g.nodes[size_t(it.slot)] = o;
     g.max_depth = std::max(g.max_depth, it.depth);
   }

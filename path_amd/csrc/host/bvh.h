@@ -48,21 +48,30 @@ Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
 
 // GPU layout: a 4-wide BVH collapsed from the binary tree (each GPU node takes
 // the up-to-4 descendants obtained by repeatedly opening the child with the
-// largest surface area). 128 bytes per node, structure of arrays over the
-// four child slots:
-//   lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4], child[4], count[4]
-// child >= 0 is an inner-node index, child < 0 is ~(first triangle) of a leaf
-// with `count` triangles, kEmptyChild marks an unused slot. Boxes are padded
-// outward by `pad` so that the box test is conservative with respect to the
-// float triangle test (the closest hit then does not depend on traversal
-// order).
+// largest surface area), 64 bytes per node with the four child boxes
+// quantised to 8 bits on a per-node grid:
+//   origin (x, y, z) float and exponents (ex, ey, ez) int8 -> plane
+//     = fma(q, 2^e, origin) per axis,
+//   child[4] (>= 0 inner node, < 0 ~(first triangle) of a leaf, kEmptyChild
+//     for an unused slot),
+//   q lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4] (uint8).
+// The grid codes round outward and are checked with the same float fma the
+// GPU evaluates, so each decoded box contains its child's padded box: boxes
+// only grow, and since the padded boxes are already conservative with
+// respect to the float triangle test, the closest hit still does not depend
+// on the tree.
 constexpr int32_t kEmptyChild = INT32_MIN;
 
 struct GpuNode {
-  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  float origin[3];
+  int8_t exponent[3];
+  uint8_t pad0;
   int32_t child[4];
-  int32_t count[4];
+  uint8_t lo_x[4], hi_x[4], lo_y[4], hi_y[4];
+  uint8_t lo_z[4], hi_z[4];
+  uint32_t pad1[2];
 };
+static_assert(sizeof(GpuNode) == 64, "GPU BVH node is one 64-byte record");
 
 // Triangle in leaf order: (v0, idx bits), (e1, 0), (e2, 0).
 struct GpuTri {
