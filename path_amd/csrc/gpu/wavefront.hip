@@ -348,7 +348,9 @@ constexpr uint32_t kRefillMinIdle = 16;
 // Each wave owns one contiguous range of the queue and keeps its 64 lanes
 // busy: a lane whose ray finished takes the next entry of the range (ballot +
 // mbcnt, no atomics) while the other lanes continue, so a long ray delays
-// only itself.
+// only itself. Lanes step one node or leaf at a time: holding leaves until
+// the whole wave has one (as trace_closest does) measured 438 vs 456
+// Msamples/s here, since it delays the refill.
 template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStack * 256];
@@ -399,7 +401,7 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
         break;
       continue;
     }
-    if (has && ts.step(nodes, tris, stack)) {
+    if (has && ts.template step<false>(nodes, tris, stack)) {
       hits[slot] = pack_hit(ts.hit());
       has = false;
     }
