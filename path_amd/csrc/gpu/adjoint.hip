@@ -36,7 +36,8 @@ __device__ __forceinline__ R adjoint_rng(const RngParams &rp, uint32_t location,
 template <class R, bool kBackprop>
 __device__ __forceinline__ V3<float>
 replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
-              V3<float> radiance, V3<float> d_i, float *acc, uint32_t stride) {
+              V3<float> radiance, V3<float> d_i, float *acc, uint32_t stride,
+              LdsStack &stack) {
   const uint32_t location = px + py * p.x_dim;
   R rng = adjoint_rng<R>(p.rng, location, sample);
   float xo = rng.next();
@@ -49,12 +50,10 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
   for (;;) {
     Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
     if (st.n_light) {
-      PrivateStack stack;
       light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
                                 st.light_target + 2e-6f, st.light_target - 2e-6f, stack);
     }
     if (st.has_next) {
-      PrivateStack stack;
       next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray, 3.4028235e38f,
                                -1.0f, stack);
     }
@@ -96,6 +95,9 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
 template <class R>
 __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
   extern __shared__ float acc_all[];
+  __shared__ int stack_lds[kLdsStackInts * (kAdjointBlock / 64)];
+  int spill[kOverflowDepth];
+  LdsStack stack{lds_stack_column(stack_lds), spill, 1};
   const uint32_t stride = kAdjointBlock + 1; // odd stride: conflict-free sums
   const uint32_t n_params = 6 * p.n_mats;
   float *acc = acc_all + threadIdx.x;
@@ -114,8 +116,8 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
     for (uint32_t sample = s0; sample < s1; ++sample) {
       V3<float> zero = {0.f, 0.f, 0.f};
       V3<float> radiance =
-          replay_sample<R, false>(p, px, py, sample, zero, zero, nullptr, stride);
-      replay_sample<R, true>(p, px, py, sample, radiance, d_i, acc, stride);
+          replay_sample<R, false>(p, px, py, sample, zero, zero, nullptr, stride, stack);
+      replay_sample<R, true>(p, px, py, sample, radiance, d_i, acc, stride, stack);
     }
   }
   __syncthreads();

@@ -90,7 +90,7 @@ hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
 template <class R, bool kPerStep>
 __device__ __forceinline__ V3<float>
 integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location,
-                  uint32_t *rays_traced, V3<float> *steps) {
+                  uint32_t *rays_traced, V3<float> *steps, LdsStack &stack) {
   V3<float> total = {0.f, 0.f, 0.f};
   uint32_t n_rays_traced = 0;
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
@@ -127,7 +127,6 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
       }
       Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
       if (st.n_light) {
-        PrivateStack stack;
         light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
                                   st.light_target + 2e-6f,
                                   st.light_target - 2e-6f, stack);
@@ -137,7 +136,6 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
         if (step == 0 && p.initial_idx) {
           next_hit = given;
         } else {
-          PrivateStack stack;
           next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
                                    3.4028235e38f, -1.0f, stack);
           ++n_rays_traced;
@@ -213,9 +211,14 @@ __device__ __forceinline__ void reduce_store(const MegaParams &p, const ThreadIn
   }
 }
 
+// Blocks of up to 1024 threads (16 waves); dynamic LDS holds the traversal
+// stacks (kLdsStackInts per wave), deeper entries go to a private array.
 template <bool kPerStep>
-__global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
-                                                    uint32_t start_block) {
+__global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
+                                                     uint32_t start_block) {
+  extern __shared__ int mk_stack[];
+  int spill[kOverflowDepth];
+  LdsStack stack{lds_stack_column(mk_stack), spill, 1};
   uint32_t block = blockIdx.x + start_block;
   ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
   V3<float> total = {0.f, 0.f, 0.f};
@@ -227,9 +230,9 @@ __global__ __launch_bounds__(256) void k_megakernel(MegaParams p,
   if (!info.exit) {
     uint32_t location = info.x + info.y * p.wd.x_dim;
     if (p.rng.type == kRngSobel)
-      total = integrate_samples<TableRng, kPerStep>(p, info, location, &traced, steps);
+      total = integrate_samples<TableRng, kPerStep>(p, info, location, &traced, steps, stack);
     else
-      total = integrate_samples<HashRng, kPerStep>(p, info, location, &traced, steps);
+      total = integrate_samples<HashRng, kPerStep>(p, info, location, &traced, steps, stack);
   }
   if constexpr (kPerStep) {
     const size_t image = size_t(p.wd.num_sample_blocks) * p.wd.x_dim * p.wd.y_dim * 3;
@@ -251,12 +254,22 @@ hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
                              uint32_t end_block, hipStream_t s) {
   if (end_block <= start_block)
     return hipSuccess;
+  if (p.wd.block_size == 0 || p.wd.block_size > 1024 || p.wd.block_size % 64 != 0)
+    return hipErrorInvalidConfiguration;
+  const size_t lds = size_t(p.wd.block_size / 64) * kLdsStackInts * sizeof(int);
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess)
+    return hipErrorInvalidDevice;
+  if (lds + 256 > size_t(lds_max)) // 256: the reduction's static LDS
+    return hipErrorInvalidConfiguration;
   if (p.n_steps)
     hipLaunchKernelGGL(k_megakernel<true>, dim3(end_block - start_block),
-                       dim3(p.wd.block_size), 0, s, p, start_block);
+                       dim3(p.wd.block_size), lds, s, p, start_block);
   else
     hipLaunchKernelGGL(k_megakernel<false>, dim3(end_block - start_block),
-                       dim3(p.wd.block_size), 0, s, p, start_block);
+                       dim3(p.wd.block_size), lds, s, p, start_block);
   return hipGetLastError();
 }
 

@@ -28,7 +28,9 @@ __global__ __launch_bounds__(256) void k_nrc_paths(NrcParams p, uint32_t n) {
   const SceneView<float> &sc = p.scene.view;
   V3<float> base = {0.f, 0.f, 0.f}, weight = {0.f, 0.f, 0.f};
   float feat[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  PrivateStack stack;
+  __shared__ int stack_lds[kLdsStackInts * 4];
+  int spill[kOverflowDepth];
+  LdsStack stack{lds_stack_column(stack_lds), spill, 1};
   Hit<float> h1 = trace_closest(p.scene.nodes, p.scene.tris, cam, 3.4028235e38f, -1.0f, stack);
   PathState<float> st = initial_path_state<float>(), ns;
   Ray<float> nl = cam, nn = cam;

@@ -33,21 +33,29 @@ struct PrivateStack {
   __device__ __forceinline__ int pop() { return v[--sp]; }
 };
 
-// The top kLdsStack entries live in LDS (one column per lane of a 256-thread
-// block); deeper entries rotate into a per-thread global overflow area, laid
-// out entry-major so a wave's spills are contiguous. Logical entry k sits at
-// LDS row k % kLdsStack while k >= sp - kLdsStack.
+// The top kLdsStack entries live in LDS, deeper entries rotate into an
+// overflow area (a per-thread column of a global buffer, laid out entry-major
+// so a wave's spills are contiguous, or a private array). Each wave owns a
+// kLdsStack x 64 tile of LDS, one column per lane (row stride 64 ints: the
+// lanes of a row hit distinct banks). Logical entry k sits at LDS row
+// k % kLdsStack while k >= sp - kLdsStack.
 constexpr int kLdsStack = 16;
 constexpr int kOverflowDepth = kTraceStackSize - kLdsStack;
+constexpr uint32_t kLdsStackInts = kLdsStack * 64; // per wave
+
+// This thread's LDS column inside `lds` (kLdsStackInts per wave of the block).
+__device__ __forceinline__ int *lds_stack_column(int *lds) {
+  return lds + (threadIdx.x >> 6) * kLdsStackInts + (threadIdx.x & 63u);
+}
 
 struct LdsStack {
-  int *lds;       // &shared[threadIdx.x]; row stride 256
-  int *overflow;  // &global[thread id]; entry stride `ostride`
+  int *lds;       // lds_stack_column()
+  int *overflow;  // entry k >= kLdsStack at overflow[(k - kLdsStack) * ostride]
   uint32_t ostride;
   int sp = 0;
   __device__ __forceinline__ bool empty() const { return sp == 0; }
   __device__ __forceinline__ void push(int x) {
-    int *slot = lds + 256 * (sp & (kLdsStack - 1));
+    int *slot = lds + 64 * (sp & (kLdsStack - 1));
     if (sp >= kLdsStack)
       overflow[size_t(sp - kLdsStack) * ostride] = *slot;
     *slot = x;
@@ -55,7 +63,7 @@ struct LdsStack {
   }
   __device__ __forceinline__ int pop() {
     --sp;
-    int *slot = lds + 256 * (sp & (kLdsStack - 1));
+    int *slot = lds + 64 * (sp & (kLdsStack - 1));
     int x = *slot;
     if (sp >= kLdsStack)
       *slot = overflow[size_t(sp - kLdsStack) * ostride];

@@ -353,7 +353,7 @@ constexpr uint32_t kRefillMinIdle = 16;
 // Msamples/s here, since it delays the refill.
 template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
-  __shared__ int lds_stack[kLdsStack * 256];
+  __shared__ int lds_stack[kLdsStackInts * 4];
   const uint32_t count = p.counters[kShadow ? kShadowCount : kExtCount];
   const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
   uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
@@ -364,7 +364,7 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
   const uint32_t end = min(next + per, count);
   if (next >= end)
     return; // whole wave
-  LdsStack stack{lds_stack + threadIdx.x, p.stack_overflow + gtid, p.overflow_stride};
+  LdsStack stack{lds_stack_column(lds_stack), p.stack_overflow + gtid, p.overflow_stride};
   TraceState ts;
   const float4 *nodes = p.scene.nodes, *tris = p.scene.tris;
   uint32_t slot = 0;

@@ -56,6 +56,31 @@ def test_megakernel_image_bit_exact(renderer, request, which, spp, size):
     assert rel_rmse(img, ref64) < 2e-2
 
 
+@pytest.mark.parametrize("block_size,target_x,spp,per_thread", [
+    (64, 8, 16, 2),       # one wave per block, 8 threads per pixel
+    (512, 32, 512, 1),    # 512 threads per pixel: reduction through LDS
+    (1024, 32, 512, 1),   # 16 waves per block
+    (512, 32, 1024, 1),   # two sample blocks per pixel: second reduction pass
+])
+def test_megakernel_block_sizes_bit_exact(renderer, cornell_sphere, block_size, target_x, spp,
+                                          per_thread):
+    """Mega-kernel blocks of 64..1024 threads (traversal stacks in LDS per
+    wave) against the CPU checker, with the reference's work division."""
+    from oracle import oracle as O
+    scene, ftw = cornell_sphere
+    s = _settings()
+    wd = s.render_work_division
+    wd.block_size = block_size
+    wd.target_x_block_size = target_x
+    wd.force_target_samples = 1
+    wd.forced_target_samples_per_thread = per_thread
+    size = 24
+    img, _ = renderer.render(scene, ftw, size, size, spp, s)
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp)
+    ref, _ = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=32)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("which,spp,size", [("cornell", 16, 64), ("cornell_sphere", 8, 48)])
 def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
     """The wavefront integrator produces the mega-kernel's image bit for bit
