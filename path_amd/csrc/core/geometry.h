@@ -56,6 +56,53 @@ PT_INLINE bool intersect_triangle(const Ray<T> &ray, const TriGeom<T> &tri,
   return false;
 }
 
+// Screen in front of intersect_triangle for traversal loops. With
+//   a = dot(e1, h), x = dot(s, h), y = dot(d, q), z = dot(e2, q)
+// (the quantities of intersect_triangle) and r any approximation of 1/a with
+// relative error below 2^-21 (a hardware reciprocal), the screen evaluates
+// u' = r x, v' = r y, t' = r z and returns true only when its margin makes
+// the exact test reject the triangle or find t > t_best:
+//   |a| < 1e-7                 (the exact test's own first check)
+//   u' < -2^-100            => u < 0 (same sign, no underflow to zero)
+//   u' > 1 + m              => u > 1
+//   v' < -2^-100            => v < 0
+//   u' + v' > 1 + m         => fl(u + v) > 1 (u', v' >= -2^-100 here)
+//   t' < 1e-5 (1 - m)       => t < 1e-5
+//   t' > t_best (1 + m)     => t > t_best
+// for m = 2^-16 (the product errors stay below 2^-20 relative). NaN fails
+// every comparison, so such a triangle reaches the exact test.
+template <class T>
+PT_INLINE bool triangle_screen_rejects(T a, T x, T y, T z, T r, T t_best,
+                                       T m = T(0x1p-16f)) {
+  const T u = r * x, v = r * y, t = r * z;
+  const T tiny = T(0x1p-100f), one = T(1) + m;
+  return abs_(a) < T(1e-7f) || u < -tiny || u > one || v < -tiny || u + v > one ||
+         t < T(1e-5f) * (T(1) - m) || t > t_best * one;
+}
+
+// intersect_triangle on the screen's quantities: the same operations
+// (f = 1 / a correctly rounded, then f * x, f * y, f * z) and predicates.
+template <class T>
+PT_INLINE bool intersect_triangle_exact(T a, T x, T y, T z, T *t_out,
+                                        uint32_t *back_out) {
+  if (abs_(a) < T(1e-7f))
+    return false;
+  const T f = T(1) / a;
+  const T u = f * x;
+  if (u < T(0) || u > T(1))
+    return false;
+  const T v = f * y;
+  if (v < T(0) || u + v > T(1))
+    return false;
+  const T t = f * z;
+  if (t > T(1e-5f)) {
+    *t_out = t;
+    *back_out = a < T(0) ? 1u : 0u;
+    return true;
+  }
+  return false;
+}
+
 // Total order used to pick the closest hit: smaller distance first, then the
 // smaller flattened triangle index. The reference keeps whichever equal-distance
 // candidate it tested last (include/intersect/optional_min.h:16-25), which

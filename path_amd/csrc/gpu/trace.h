@@ -71,6 +71,27 @@ struct LdsStack {
   }
 };
 
+// intersect_triangle with triangle_screen_rejects (core/geometry.h) in front,
+// on the hardware reciprocal (v_rcp_f32, 1 ulp): most triangles are rejected
+// without the correctly rounded division and the exact test's branches; the
+// result equals intersect_triangle's whenever that one finds a hit with
+// t <= t_best.
+__device__ __forceinline__ bool intersect_triangle_screened(const Ray<float> &ray,
+                                                            const TriGeom<float> &tri,
+                                                            float t_best, float *t_out,
+                                                            uint32_t *back_out) {
+  const V3<float> h = cross(ray.d, tri.e2);
+  const float a = dot(tri.e1, h);
+  const V3<float> s = ray.o - tri.v0;
+  const float x = dot(s, h);
+  const V3<float> q = cross(s, tri.e1);
+  const float y = dot(ray.d, q);
+  const float z = dot(tri.e2, q);
+  if (triangle_screen_rejects(a, x, y, z, __builtin_amdgcn_rcpf(a), t_best))
+    return false;
+  return intersect_triangle_exact(a, x, y, z, t_out, back_out);
+}
+
 // Resumable closest-hit traversal of one ray: init() then step() until it
 // returns true. One step visits one inner node or one leaf, so a caller can
 // interleave rays (lane refill). The closest hit is kept with t <= t_limit;
@@ -178,7 +199,8 @@ struct TraceState {
       const TriGeom<float> tri = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z}};
       float t;
       uint32_t back;
-      if (intersect_triangle(ray, tri, &t, &back) && hit_before(t, idx, best_t, best_idx)) {
+      if (intersect_triangle_screened(ray, tri, best_t, &t, &back) &&
+          hit_before(t, idx, best_t, best_idx)) {
         best_t = t;
         best_idx = idx;
         best_back = back;
