@@ -243,8 +243,8 @@ int pt_work_division(const pt_work_division_settings *s, uint32_t samples_per,
 /* Host SAH BVH of the flattened scene in the reference node layout
  * (node.h:22-126): per node lo[3], hi[3] (float) and is_leaf, a, b (uint32);
  * perm[n_tris]. This is the binary tree the GPU's 4-wide tree is collapsed
- * from (SAH traversal cost 1.0 instead of the reference's 0.5: larger leaves,
- * same closest hits). Pass NULL arrays to query n_nodes only. */
+ * from (SAH traversal cost 0.5, the reference default). Pass NULL arrays to
+ * query n_nodes only. */
 int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
                  uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
                  uint32_t *max_depth);

@@ -31,12 +31,13 @@ struct BvhSettings {
   float traversal_per_intersect_cost = 0.5f; // reference default (sbvh/settings.h)
 };
 
-// The tree the GPU traverses is built with a higher traversal cost (larger
-// leaves): measured on MI355X (82k-triangle scene, 4-wide nodes) 0.25 -> 308,
-// 0.5 -> 322, 1.0 -> 339, 2.0 -> 331 Msamples/s. Closest hits do not depend
-// on the tree, so images are unchanged; the CPU checker keeps the reference
-// default for its traversal counts.
-constexpr float kGpuSahTraversalCost = 1.0f;
+// Traversal cost of the tree the GPU traverses. Measured on MI355X (82k-
+// triangle scene, quantised 4-wide nodes, screened triangle test): 0.25 ->
+// 487, 0.375 -> 490, 0.5 -> 488, 0.75 -> 484, 1.0 -> 475, 1.5 -> 376
+// Msamples/s, so the reference default is kept (with 128-byte float nodes
+// and the full triangle test, 1.0 had been best). Closest hits do not depend
+// on the tree, so images are unchanged.
+constexpr float kGpuSahTraversalCost = 0.5f;
 
 // SAH object-split builder following src/intersect/accel/sbvh/detail/generator.cu:
 // 16-438 (spatial splits off, the reference default): per node and axis, sort
