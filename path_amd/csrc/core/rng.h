@@ -60,18 +60,22 @@ PT_INLINE float sobol_table_value(const uint32_t *all_dirs, uint32_t dim,
                            sobol_scramble(dim));
 }
 
-// Rng over a precomputed per-sample table with the per-pixel dimension walk:
-// start dimension fnv_hash(location) % initial_dimension_bound, then
+// Rng over a precomputed table with the per-pixel dimension walk: start
+// dimension fnv_hash(location) % initial_dimension_bound, then
 // dim = (dim + 1) % dimension_bound after every draw.
 // reference: include/rng/rng_from_sequence_gen.h:45-54,88-95
+// The GPU table is dimension-major, value(sample, dim) at
+// vals[dim * n_samples + sample]: lanes that draw the same dimension for
+// consecutive samples (the samples of one pixel) read consecutive words.
 struct TableRng {
   const float *vals;
-  uint32_t row;      // sample * dimension_bound
+  uint32_t sample;
   uint32_t dim;
   uint32_t dim_bound;
+  uint32_t n_samples; // table row length
 
   PT_INLINE float next() {
-    float out = vals[row + dim];
+    float out = vals[size_t(dim) * n_samples + sample];
     dim = dim + 1 == dim_bound ? 0 : dim + 1;
     return out;
   }

@@ -24,9 +24,10 @@ namespace gpu {
 template <class R>
 __device__ __forceinline__ R adjoint_rng(const RngParams &rp, uint32_t location,
                                          uint32_t sample) {
-  if constexpr (sizeof(R) == sizeof(TableRng))
-    return TableRng{rp.vals, sample * rp.dim_bound,
-                    initial_dimension(location, rp.initial_dim_bound), rp.dim_bound};
+  if constexpr (std::is_same_v<R, TableRng>)
+    return TableRng{rp.vals, sample,
+                    initial_dimension(location, rp.initial_dim_bound), rp.dim_bound,
+                    rp.samples_per};
   else
     return make_hash_rng(sample + location * rp.samples_per);
 }

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../core/shading.h"
 #include "../core/work_division.h"
 
@@ -82,6 +84,7 @@ struct NrcParams {
 };
 
 // Launchers (return hipError_t of the launch).
+// out[dim * count + sample] (dimension-major, read by TableRng)
 hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
                               uint32_t count, float *out, hipStream_t s);
 hipError_t launch_intersect(const DeviceScene &sc, const float *rays,

@@ -13,7 +13,7 @@ namespace gpu {
 // Sobol table: one wave per (dimension, 64 consecutive samples). Lane 0 of
 // the wave evaluates its point directly; every other lane contributes the
 // Gray-code step v[ctz(s)], and an inclusive XOR scan across the wave (DPP)
-// turns the steps into the 64 points.
+// turns the steps into the 64 points. Dimension-major output (TableRng).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sobol_table(const uint32_t *__restrict__ dirs,
                                                      uint32_t dim_bound,
@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_sobol_table(const uint32_t *__restrict_
   uint32_t delta = lane == 0 ? sobol_u32(d, s0) : d[__builtin_ctz(s)];
   uint32_t x = wave_xor_scan(delta);
   if (s < count)
-    out[size_t(s) * dim_bound + dim] = u32_to_unit_float(x ^ sobol_scramble(dim));
+    out[size_t(dim) * count + s] = u32_to_unit_float(x ^ sobol_scramble(dim));
 }
 
 hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
@@ -95,10 +95,10 @@ integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location
   uint32_t n_rays_traced = 0;
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
     R rng;
-    if constexpr (sizeof(R) == sizeof(TableRng)) {
-      rng = TableRng{p.rng.vals, sample * p.rng.dim_bound,
+    if constexpr (std::is_same_v<R, TableRng>) {
+      rng = TableRng{p.rng.vals, sample,
                      initial_dimension(location, p.rng.initial_dim_bound),
-                     p.rng.dim_bound};
+                     p.rng.dim_bound, p.rng.samples_per};
     } else {
       rng = make_hash_rng(sample + location * p.rng.samples_per);
     }

@@ -17,9 +17,10 @@ __global__ __launch_bounds__(256) void k_nrc_paths(NrcParams p, uint32_t n) {
   const uint32_t location = i / spp, sample = i % spp;
   const uint32_t px = location % p.x_dim, py = location / p.x_dim;
   R rng;
-  if constexpr (sizeof(R) == sizeof(TableRng))
-    rng = TableRng{p.rng.vals, sample * p.rng.dim_bound,
-                   initial_dimension(location, p.rng.initial_dim_bound), p.rng.dim_bound};
+  if constexpr (std::is_same_v<R, TableRng>)
+    rng = TableRng{p.rng.vals, sample,
+                   initial_dimension(location, p.rng.initial_dim_bound), p.rng.dim_bound,
+                   p.rng.samples_per};
   else
     rng = make_hash_rng(sample + location * spp);
   const float xo = rng.next();

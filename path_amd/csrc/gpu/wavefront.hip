@@ -53,8 +53,8 @@ hipError_t launch_wf_reset(uint32_t *counters, hipStream_t s) {
 template <class R>
 __device__ __forceinline__ R make_rng(const RngParams &rp, uint32_t location,
                                       uint32_t sample, uint32_t counter) {
-  if constexpr (sizeof(R) == sizeof(TableRng)) {
-    return TableRng{rp.vals, sample * rp.dim_bound, counter, rp.dim_bound};
+  if constexpr (std::is_same_v<R, TableRng>) {
+    return TableRng{rp.vals, sample, counter, rp.dim_bound, rp.samples_per};
   } else {
     HashRng h = make_hash_rng(sample + location * rp.samples_per);
     h.counter = counter;
@@ -63,7 +63,7 @@ __device__ __forceinline__ R make_rng(const RngParams &rp, uint32_t location,
 }
 
 template <class R> __device__ __forceinline__ uint32_t rng_counter(const R &r) {
-  if constexpr (sizeof(R) == sizeof(TableRng))
+  if constexpr (std::is_same_v<R, TableRng>)
     return r.dim;
   else
     return r.counter;
@@ -72,7 +72,7 @@ template <class R> __device__ __forceinline__ uint32_t rng_counter(const R &r) {
 template <class R>
 __device__ __forceinline__ uint32_t rng_start(const RngParams &rp,
                                               uint32_t location) {
-  if constexpr (sizeof(R) == sizeof(TableRng))
+  if constexpr (std::is_same_v<R, TableRng>)
     return initial_dimension(location, rp.initial_dim_bound);
   else
     return 0;

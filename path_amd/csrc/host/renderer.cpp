@@ -189,9 +189,14 @@ int pt_renderer::sobol_table_to_host(uint32_t dim_bound, uint32_t count,
   int rc = ensure_sobol(dim_bound, count);
   if (rc != PT_OK)
     return rc;
-  HIP_TRY(hipMemcpyAsync(out, sobol_vals.ptr, size_t(dim_bound) * count * 4,
+  // the device table is dimension-major; the caller gets [sample][dim]
+  std::vector<float> cols(size_t(dim_bound) * count);
+  HIP_TRY(hipMemcpyAsync(cols.data(), sobol_vals.ptr, cols.size() * 4,
                          hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
+  for (uint32_t d = 0; d < dim_bound; ++d)
+    for (uint32_t s = 0; s < count; ++s)
+      out[size_t(s) * dim_bound + d] = cols[size_t(d) * count + s];
   return PT_OK;
 }
 
