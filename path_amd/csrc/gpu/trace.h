@@ -142,19 +142,27 @@ struct TraceState {
     const float bx = fmaf(f0.x, inv.x, -oi.x), by = fmaf(f0.y, inv.y, -oi.y),
                 bz = fmaf(f0.z, inv.z, -oi.z);
     const int chs[4] = {ch.x, ch.y, ch.z, ch.w};
+    // The decode is monotone in the code (fma with a step of the sign of
+    // 1/d), so the entry plane of each axis is the lo code for a positive
+    // direction and the hi code for a negative one: select the code words
+    // once per node instead of ordering every slab pair.
+    const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
+    const uint32_t in_x = nx ? qa.y : qa.x, out_x = nx ? qa.x : qa.y;
+    const uint32_t in_y = ny ? qa.w : qa.z, out_y = ny ? qa.z : qa.w;
+    const uint32_t in_z = nz ? qb.y : qb.x, out_z = nz ? qb.x : qb.y;
     float dist[4];
     int ref[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t sh = 8u * uint32_t(k);
-      const float x0 = fmaf(float((qa.x >> sh) & 0xffu), sx, bx);
-      const float x1 = fmaf(float((qa.y >> sh) & 0xffu), sx, bx);
-      const float y0 = fmaf(float((qa.z >> sh) & 0xffu), sy, by);
-      const float y1 = fmaf(float((qa.w >> sh) & 0xffu), sy, by);
-      const float z0 = fmaf(float((qb.x >> sh) & 0xffu), sz, bz);
-      const float z1 = fmaf(float((qb.y >> sh) & 0xffu), sz, bz);
-      const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-      const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t));
+      const float x0 = fmaf(float((in_x >> sh) & 0xffu), sx, bx);
+      const float x1 = fmaf(float((out_x >> sh) & 0xffu), sx, bx);
+      const float y0 = fmaf(float((in_y >> sh) & 0xffu), sy, by);
+      const float y1 = fmaf(float((out_y >> sh) & 0xffu), sy, by);
+      const float z0 = fmaf(float((in_z >> sh) & 0xffu), sz, bz);
+      const float z1 = fmaf(float((out_z >> sh) & 0xffu), sz, bz);
+      const float tmin = fmaxf(fmaxf(x0, y0), fmaxf(z0, 0.0f));
+      const float tmax = fminf(fminf(x1, y1), fminf(z1, best_t));
       const bool hit = tmin <= tmax && chs[k] != kEmptyChildRef;
       dist[k] = hit ? tmin : __builtin_inff();
       ref[k] = chs[k];
