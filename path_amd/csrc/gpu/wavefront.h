@@ -16,6 +16,28 @@ enum WfCounter : uint32_t {
   kNumCounters = 8,
 };
 
+// Queue groups. Shade block b belongs to group b % n_groups (n_groups =
+// min(kQueueGroups, shade blocks)): it appends its rays to the group's queue
+// segment and starts new samples from the group's share of the sample range
+// (shares proportional to the group's blocks). Each group's counters sit in
+// their own 4 KB: device-scope atomics on one address serialise (about 12 ns
+// each measured on MI355X, i.e. ~200 us per shade step for one counter hit by
+// all 16k blocks), so one counter per group spreads them over 16 addresses.
+constexpr uint32_t kQueueGroups = 16;
+constexpr uint32_t kGroupStride = 1024; // uint32 words between group counters
+enum GroupCounter : uint32_t {
+  kGroupExt = 0,    // entries in the group's extension-ray segment
+  kGroupShadow = 1, // entries in the group's shadow-ray segment
+  kGroupSample = 2, // 64-bit: samples the group has started
+  kGroupKinds = 3,
+};
+constexpr size_t kGroupCounterWords = size_t(kGroupKinds) * kQueueGroups * kGroupStride;
+
+__host__ __device__ inline uint32_t *group_counter(uint32_t *base, uint32_t kind,
+                                                   uint32_t g) {
+  return base + (size_t(kind) * kQueueGroups + g) * kGroupStride;
+}
+
 // Per-slot path state, structure of arrays (one 16-byte vector per slot and
 // array so a wave's accesses are contiguous).
 struct WfState {
@@ -32,9 +54,14 @@ struct WfState {
 };
 
 // Ray queues hold slot indices; the rays themselves are read from the state.
+// Group g's entries start at g * group_cap; prefix[0..n] (extension) and
+// prefix[kQueueGroups + 1 ..] (shadow) are the exclusive prefix sums of the
+// group counts, written by k_wf_prefix after each shade step.
 struct WfQueues {
   uint32_t *ext;
   uint32_t *shadow;
+  uint32_t *prefix;
+  uint32_t group_cap;
 };
 
 // One scene of a batched render. Offsets index the concatenated arrays of all
@@ -68,8 +95,10 @@ struct WfParams {
   uint32_t n_slots;
   WfState st;
   WfQueues q;
-  uint32_t *counters;                 // WfCounter words
-  unsigned long long *sample_counter; // next sample to start
+  uint32_t *counters;       // WfCounter words
+  uint32_t *group_counters; // GroupCounter words per group (group_counter())
+  uint32_t n_groups;        // min(kQueueGroups, shade blocks)
+  uint32_t n_shade_blocks;
   float *out_samples;                 // [pixel * spp + sample] * 3
   float *gbuffer; // first hit of sample 0 per pixel: position, normal,
                   // albedo, emission (12 floats) or null
@@ -78,8 +107,12 @@ struct WfParams {
 };
 
 // Fold the queue counts of the previous iteration into the running totals and
-// zero them (one thread; safe inside a captured graph).
-hipError_t launch_wf_reset(uint32_t *counters, hipStream_t s);
+// zero them and the group queue counters (one block; safe inside a captured
+// graph).
+hipError_t launch_wf_reset(uint32_t *counters, uint32_t *group_counters, hipStream_t s);
+// Prefix sums of the group queue counts into q.prefix, totals into
+// counters[kExtCount], counters[kShadowCount].
+hipError_t launch_wf_prefix(const WfParams &p, hipStream_t s);
 hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s);
 hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
                                uint32_t grid_blocks, hipStream_t s);

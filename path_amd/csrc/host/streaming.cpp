@@ -6,6 +6,7 @@
 #include "../gpu/trace.h"
 #include "../gpu/wavefront.h"
 
+#include <algorithm>
 #include <cstring>
 
 using namespace pt;
@@ -35,10 +36,15 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
   HIP_TRY(r.wf_light_dir.ensure(n_slots));
   if (batched)
     HIP_TRY(r.wf_scene.ensure(n_slots));
-  HIP_TRY(r.wf_q_ext.ensure(n_slots));
-  HIP_TRY(r.wf_q_shadow.ensure(n_slots));
+  // queue segments: one per group, sized for the group's shade blocks
+  const uint32_t shade_blocks = (n_slots + 255) / 256;
+  const uint32_t n_groups = std::min(gpu::kQueueGroups, shade_blocks);
+  const uint32_t group_cap = (shade_blocks + n_groups - 1) / n_groups * 256;
+  HIP_TRY(r.wf_q_ext.ensure(size_t(group_cap) * n_groups));
+  HIP_TRY(r.wf_q_shadow.ensure(size_t(group_cap) * n_groups));
   HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
-  HIP_TRY(r.wf_sample_counter.ensure(1));
+  HIP_TRY(r.wf_group_counters.ensure(gpu::kGroupCounterWords));
+  HIP_TRY(r.wf_queue_prefix.ensure(2 * (gpu::kQueueGroups + 1)));
   HIP_TRY(r.wf_samples.ensure(size_t(total_samples) * 3));
   const uint32_t trace_threads = trace_blocks * 256;
   HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
@@ -48,9 +54,11 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
           r.wf_total.ptr,    r.wf_origin.ptr, r.wf_dir.ptr,
           r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr,
           batched ? r.wf_scene.ptr : nullptr};
-  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr};
+  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr, r.wf_queue_prefix.ptr, group_cap};
   p.counters = r.wf_counters.ptr;
-  p.sample_counter = r.wf_sample_counter.ptr;
+  p.group_counters = r.wf_group_counters.ptr;
+  p.n_groups = n_groups;
+  p.n_shade_blocks = shade_blocks;
   p.out_samples = r.wf_samples.ptr;
   p.stack_overflow = r.wf_stack_overflow.ptr;
   p.overflow_stride = trace_threads;
@@ -65,9 +73,11 @@ static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
                                     bool profile_events) {
   hipStream_t s = r.stream;
   hipError_t e;
-  if ((e = gpu::launch_wf_reset(p.counters, s)) != hipSuccess)
+  if ((e = gpu::launch_wf_reset(p.counters, p.group_counters, s)) != hipSuccess)
     return e;
   if ((e = gpu::launch_wf_shade(p, batched, s)) != hipSuccess)
+    return e;
+  if ((e = gpu::launch_wf_prefix(p, s)) != hipSuccess)
     return e;
   if ((e = hipMemcpyAsync(r.wf_host_counts, p.counters,
                           sizeof(uint32_t) * gpu::kNumCounters,
@@ -90,7 +100,7 @@ int wavefront_run(pt_renderer &r, gpu::WfParams &p, bool batched,
                   uint32_t trace_blocks, uint32_t iters_per_sync) {
   hipStream_t s = r.stream;
   HIP_TRY(hipMemsetAsync(p.st.meta, 0, size_t(p.n_slots) * sizeof(uint4), s));
-  HIP_TRY(hipMemsetAsync(p.sample_counter, 0, sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(p.group_counters, 0, sizeof(uint32_t) * gpu::kGroupCounterWords, s));
   HIP_TRY(hipMemsetAsync(p.counters, 0, sizeof(uint32_t) * gpu::kNumCounters, s));
   if (!r.wf_host_counts)
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&r.wf_host_counts),
