@@ -336,6 +336,11 @@ int pt_mlp_get_weights(pt_mlp *m, float *weights);
 int pt_mlp_set_weights(pt_mlp *m, const float *weights);
 /* features: n x 9 (host), out: n x 3 (host). */
 int pt_mlp_forward(pt_mlp *m, const float *features, uint32_t n, float *out);
+/* Inference kernel choice: fused != 0 (default) runs the whole forward in one
+ * kernel per 128-row tile with the activations in LDS; 0 runs the encoding and
+ * one GEMM per layer through HBM (the training forward). Both give the same
+ * bits. */
+int pt_mlp_set_forward_kernel(pt_mlp *m, int fused);
 /* Gradient of the relative L2 loss mean((p - t)^2 / (p^2 + 0.01)) (p held
  * constant in the denominator) over one batch; grad has pt_mlp_param_count
  * floats. */

@@ -150,6 +150,7 @@ SIGNATURES = {
     "pt_mlp_get_weights": (C.c_int, [_P, _F]),
     "pt_mlp_set_weights": (C.c_int, [_P, _F]),
     "pt_mlp_forward": (C.c_int, [_P, _F, C.c_uint32, _F]),
+    "pt_mlp_set_forward_kernel": (C.c_int, [_P, C.c_int]),
     "pt_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_uint32, _F, _F]),
     "pt_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_float, _F]),
     "pt_mlp_set_dataset": (C.c_int, [_P, _F, _F, C.c_uint64]),

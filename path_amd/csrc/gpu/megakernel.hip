@@ -242,11 +242,19 @@ __global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
     reduce_store(p, info, block, total, p.out_rgb);
   }
   if (p.ray_counter) {
+    // one device atomic per block: atomics on one address serialise
+    __shared__ uint32_t block_rays;
+    if (threadIdx.x == 0)
+      block_rays = 0;
+    __syncthreads();
     uint32_t lane_total = traced;
     for (int off = 32; off >= 1; off >>= 1)
       lane_total += __shfl_xor(lane_total, off, 64);
     if ((threadIdx.x & 63u) == 0)
-      atomicAdd(p.ray_counter, (unsigned long long)lane_total);
+      atomicAdd(&block_rays, lane_total);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      atomicAdd(p.ray_counter, (unsigned long long)block_rays);
   }
 }
 

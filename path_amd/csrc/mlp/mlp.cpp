@@ -126,6 +126,10 @@ static GemmArgs gemm_args(const bf16 *A, uint32_t lda, const bf16 *B, uint32_t l
 int pt_mlp::forward_device(const float *d_feat, uint32_t rows_pad, bool training,
                            hipStream_t on) {
   hipStream_t stream = on ? on : this->stream;
+  if (!training && fused_forward) {
+    HIP_TRY(launch_mlp_forward(d_feat, rows_pad, wbf.ptr, pred.ptr, stream));
+    return PT_OK;
+  }
   HIP_TRY(launch_encode(d_feat, rows_pad, x0.ptr, stream));
   const bf16 *in = x0.ptr;
   for (uint32_t l = 0; l < kLayers; ++l) {
@@ -248,6 +252,13 @@ int pt_mlp_set_weights(pt_mlp *m, const float *w) {
   HIP_TRY(hipMemsetAsync(m->adam_v.ptr, 0, param_count() * 4, m->stream));
   m->step = 0;
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return PT_OK;
+}
+
+int pt_mlp_set_forward_kernel(pt_mlp *m, int fused) {
+  if (!m)
+    return set_error(PT_ERR_INVALID, "null argument");
+  m->fused_forward = fused != 0;
   return PT_OK;
 }
 

@@ -9,6 +9,7 @@ struct pt_mlp {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t step = 0; // Adam steps taken
+  bool fused_forward = true; // inference through k_mlp_forward
 
   pt::host::DeviceBuf<float> w32, adam_m, adam_v, grad;
   pt::host::DeviceBuf<pt::mlp::bf16> wbf;
@@ -26,7 +27,8 @@ struct pt_mlp {
   bool init(int dev, uint64_t seed, std::string *err);
   int ensure_rows(uint32_t rows_pad, bool training);
   // Forward on device features (rows_pad rows, padded rows arbitrary):
-  // pred[rows_pad x 16] fp32. training = keep every layer's activations.
+  // pred[rows_pad x 16] fp32. training = keep every layer's activations
+  // (layer-by-layer GEMMs); otherwise the fused kernel unless disabled.
   int forward_device(const float *d_feat, uint32_t rows_pad, bool training,
                      hipStream_t on = nullptr);
   // Loss + backward + gradient into `grad` for the batch already in

@@ -64,6 +64,11 @@ hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilog
                        uint32_t splits, hipStream_t s);
 // features [n x 9] fp32 -> encoded [n x 256] bf16
 hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream_t s);
+// Whole inference forward in one kernel (encoding, hidden layers, output):
+// features [rows_pad x 9] -> pred [rows_pad x 16] fp32, rows_pad a multiple
+// of 128; equal bit for bit to encode + the layer GEMMs.
+hipError_t launch_mlp_forward(const float *features, uint32_t rows_pad, const bf16 *w,
+                              float *pred, hipStream_t s);
 // Relative L2 loss on the first 3 outputs: l = (p - t)^2 / (p^2 + 0.01) with p
 // held constant in the denominator, mean over n * 3. Writes d(loss)/d(pred)
 // as bf16 [n x 16] (padding zero) and per-block loss partials (fp32).

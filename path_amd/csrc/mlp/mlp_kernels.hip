@@ -189,20 +189,25 @@ hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilog
 // frequency k: sin(pi 2^k x_d), cos(pi 2^k x_d) at 9 + 24 d + 2 k (+1), zero
 // from 225 on.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t encode_value(const float *__restrict__ f, size_t row,
+                                                 uint32_t col) {
+  float v = 0.f;
+  if (col < kFeatures) {
+    v = f[row * kFeatures + col];
+  } else if (col < kEncoded) {
+    const uint32_t j = col - kFeatures, d = j / (2 * kFrequencies), r = j % (2 * kFrequencies);
+    const float x = ldexpf(f[row * kFeatures + d], int(r >> 1));
+    v = (r & 1u) ? cospif(x) : sinpif(x);
+  }
+  return f2bf(v);
+}
+
 __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ f, uint32_t n,
                                                 uint16_t *__restrict__ out) {
   const uint32_t row = blockIdx.x, col = threadIdx.x;
   if (row >= n)
     return;
-  float v = 0.f;
-  if (col < kFeatures) {
-    v = f[size_t(row) * kFeatures + col];
-  } else if (col < kEncoded) {
-    const uint32_t j = col - kFeatures, d = j / (2 * kFrequencies), r = j % (2 * kFrequencies);
-    const float x = ldexpf(f[size_t(row) * kFeatures + d], int(r >> 1));
-    v = (r & 1u) ? cospif(x) : sinpif(x);
-  }
-  out[size_t(row) * kInPad + col] = f2bf(v);
+  out[size_t(row) * kInPad + col] = encode_value(f, row, col);
 }
 
 hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream_t s) {
@@ -210,6 +215,120 @@ hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream
     return hipSuccess;
   hipLaunchKernelGGL(k_encode, dim3(n), dim3(kInPad), 0, s, features, n,
                      reinterpret_cast<uint16_t *>(out));
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference forward: one workgroup (4 waves) per 128-row tile. The
+// tile's activations stay in LDS (128 x 256 bf16, row stride 264) from the
+// encoding through the six hidden layers; only the 9 input features and the
+// 16 outputs per row touch HBM, and the weights (128 KB per layer, shared by
+// every tile) come from L2. Wave w computes output columns [64 w, 64 w + 64)
+// of all 128 rows (4 x 2 tiles of v_mfma_f32_32x32x16_bf16) with its weight
+// fragments loaded one K step ahead; the output layer (16 columns) gives
+// wave w rows [32 w, 32 w + 32). The MFMAs of every output element run in the
+// same K order as k_gemm's, and the epilogues round the same way, so the
+// result equals the layer-by-layer forward bit for bit.
+// ---------------------------------------------------------------------------
+static_assert(kInPad == kWidth, "hidden weight matrices are all kWidth x kWidth");
+constexpr uint32_t kFuseRows = 128, kXStride = kWidth + 8;
+constexpr size_t kFuseLds = size_t(kFuseRows) * kXStride * sizeof(uint16_t);
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ feat,
+                                                     const uint16_t *__restrict__ w,
+                                                     float *__restrict__ pred) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t X[];
+  const size_t row0 = size_t(blockIdx.x) * kFuseRows;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t l32 = lane & 31u, khalf = (lane >> 5) * 8u;
+  for (uint32_t i = threadIdx.x; i < kFuseRows * kInPad; i += 256u) {
+    const uint32_t r = i / kInPad, c = i % kInPad;
+    X[r * kXStride + c] = encode_value(feat, row0 + r, c);
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (uint32_t l = 0; l < kHidden; ++l) {
+    // hidden layers are 256 x 256 (layer 0's input is padded to 256)
+    const uint16_t *W = w + size_t(l) * kWidth * kWidth;
+    f32x16_t acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          acc[i][j][r] = 0.f;
+    const uint16_t *wrow[2] = {W + size_t(wave * 64u + l32) * kWidth + khalf,
+                               W + size_t(wave * 64u + 32u + l32) * kWidth + khalf};
+    bf16x8_t b_next[2] = {*reinterpret_cast<const bf16x8_t *>(wrow[0]),
+                          *reinterpret_cast<const bf16x8_t *>(wrow[1])};
+#pragma unroll
+    for (uint32_t k0 = 0; k0 < kWidth; k0 += 16) {
+      const bf16x8_t b[2] = {b_next[0], b_next[1]};
+      if (k0 + 16 < kWidth) {
+        b_next[0] = *reinterpret_cast<const bf16x8_t *>(wrow[0] + k0 + 16);
+        b_next[1] = *reinterpret_cast<const bf16x8_t *>(wrow[1] + k0 + 16);
+      }
+      bf16x8_t a[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[i] = *reinterpret_cast<const bf16x8_t *>(X + (i * 32u + l32) * kXStride + k0 + khalf);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads(); // every wave has read this layer's input
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t row = i * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
+          const uint32_t col = wave * 64u + j * 32u + l32;
+          const float v = acc[i][j][r];
+          X[row * kXStride + col] = f2bf(v > 0.f ? v : 0.f);
+        }
+    __syncthreads();
+  }
+  // output layer: [16 x 256] weights, columns 16..31 of the MFMA tile are zero
+  const uint16_t *W = w + size_t(kHidden) * kWidth * kWidth;
+  f32x16_t acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    acc[r] = 0.f;
+  for (uint32_t k0 = 0; k0 < kWidth; k0 += 16) {
+    bf16x8_t b;
+    if (l32 < kOutPad) {
+      b = *reinterpret_cast<const bf16x8_t *>(W + size_t(l32) * kWidth + k0 + khalf);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        b[e] = __bf16(0.f);
+    }
+    const bf16x8_t a =
+        *reinterpret_cast<const bf16x8_t *>(X + (wave * 32u + l32) * kXStride + k0 + khalf);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  if (l32 < kOutPad) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t row = wave * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
+      pred[(row0 + row) * kOutPad + l32] = acc[r];
+    }
+  }
+}
+
+hipError_t launch_mlp_forward(const float *features, uint32_t rows_pad, const bf16 *w,
+                              float *pred, hipStream_t s) {
+  if (rows_pad % kFuseRows)
+    return hipErrorInvalidValue;
+  if (rows_pad == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(rows_pad / kFuseRows), dim3(256), kFuseLds, s,
+                     features, reinterpret_cast<const uint16_t *>(w), pred);
   return hipGetLastError();
 }
 

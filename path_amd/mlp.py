@@ -51,6 +51,10 @@ class RadianceMLP:
         L.check(L.lib().pt_mlp_forward(self.handle, L.fptr(f), f.shape[0], L.fptr(out)))
         return out
 
+    def set_forward_kernel(self, fused: bool) -> None:
+        """Inference through the fused kernel (default) or the layer GEMMs."""
+        L.check(L.lib().pt_mlp_set_forward_kernel(self.handle, int(bool(fused))))
+
     def gradient(self, features: np.ndarray, targets: np.ndarray):
         f = np.ascontiguousarray(features, np.float32).reshape(-1, FEATURES)
         t = np.ascontiguousarray(targets, np.float32).reshape(-1, OUTPUTS)

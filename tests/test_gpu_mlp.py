@@ -43,6 +43,26 @@ def test_forward_matches_reference(net, n):
     assert np.sqrt(((out - ref) ** 2).mean()) <= 3e-3 * scale
 
 
+@pytest.mark.parametrize("n", [1, 128, 130, 5000])
+def test_fused_forward_equals_layer_gemms(net, n):
+    """The one-kernel inference forward (activations in LDS) returns the bits
+    of the encoding kernel + one GEMM per layer."""
+    f, _ = _data(n, 100 + n)
+    w = net.get_weights()
+    w2 = w * np.float32(1.7)   # larger activations than the initial weights give
+    net.set_weights(w2)
+    try:
+        net.set_forward_kernel(True)
+        fused = net.forward(f)
+        net.set_forward_kernel(False)
+        layered = net.forward(f)
+    finally:
+        net.set_forward_kernel(True)
+        net.set_weights(w)
+    assert np.abs(fused).max() > 0
+    assert np.array_equal(fused.view(np.uint32), layered.view(np.uint32))
+
+
 def test_gradient_matches_reference(net):
     from oracle import mlp_ref as M
     f, t = _data(2048, 7)
