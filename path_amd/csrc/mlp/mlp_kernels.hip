@@ -189,32 +189,45 @@ hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilog
 // frequency k: sin(pi 2^k x_d), cos(pi 2^k x_d) at 9 + 24 d + 2 k (+1), zero
 // from 225 on.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t encode_value(const float *__restrict__ f, size_t row,
-                                                 uint32_t col) {
-  float v = 0.f;
-  if (col < kFeatures) {
-    v = f[row * kFeatures + col];
-  } else if (col < kEncoded) {
-    const uint32_t j = col - kFeatures, d = j / (2 * kFrequencies), r = j % (2 * kFrequencies);
-    const float x = ldexpf(f[row * kFeatures + d], int(r >> 1));
-    v = (r & 1u) ? cospif(x) : sinpif(x);
+// The 25 columns of feature d with value x in one encoded row (row[0] is
+// column 0): the raw value and, per frequency, sin and cos from one sincospi.
+__device__ __forceinline__ void encode_feature(float x, uint32_t d, uint16_t *row) {
+  row[d] = f2bf(x);
+  uint16_t *o = row + kFeatures + 2 * kFrequencies * d;
+#pragma unroll
+  for (uint32_t k = 0; k < kFrequencies; ++k) {
+    float s, c;
+    sincospif(ldexpf(x, int(k)), &s, &c);
+    o[2 * k] = f2bf(s);
+    o[2 * k + 1] = f2bf(c);
   }
-  return f2bf(v);
 }
+
+// 28 rows per block: thread t < 252 encodes feature t % 9 of row t / 9, then
+// the block zeroes the padding columns of its rows.
+constexpr uint32_t kEncRows = 256 / kFeatures;
 
 __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ f, uint32_t n,
                                                 uint16_t *__restrict__ out) {
-  const uint32_t row = blockIdx.x, col = threadIdx.x;
-  if (row >= n)
-    return;
-  out[size_t(row) * kInPad + col] = encode_value(f, row, col);
+  const size_t row0 = size_t(blockIdx.x) * kEncRows;
+  const uint32_t t = threadIdx.x;
+  if (t < kEncRows * kFeatures) {
+    const size_t row = row0 + t / kFeatures;
+    if (row < n)
+      encode_feature(f[row * kFeatures + t % kFeatures], t % kFeatures, out + row * kInPad);
+  }
+  for (uint32_t i = t; i < kEncRows * (kInPad - kEncoded); i += 256u) {
+    const size_t row = row0 + i / (kInPad - kEncoded);
+    if (row < n)
+      out[row * kInPad + kEncoded + i % (kInPad - kEncoded)] = 0;
+  }
 }
 
 hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream_t s) {
   if (n == 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_encode, dim3(n), dim3(kInPad), 0, s, features, n,
-                     reinterpret_cast<uint16_t *>(out));
+  hipLaunchKernelGGL(k_encode, dim3((n + kEncRows - 1) / kEncRows), dim3(256), 0, s, features,
+                     n, reinterpret_cast<uint16_t *>(out));
   return hipGetLastError();
 }
 
@@ -241,10 +254,12 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ f
   const size_t row0 = size_t(blockIdx.x) * kFuseRows;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t l32 = lane & 31u, khalf = (lane >> 5) * 8u;
-  for (uint32_t i = threadIdx.x; i < kFuseRows * kInPad; i += 256u) {
-    const uint32_t r = i / kInPad, c = i % kInPad;
-    X[r * kXStride + c] = encode_value(feat, row0 + r, c);
+  for (uint32_t i = threadIdx.x; i < kFuseRows * kFeatures; i += 256u) {
+    const uint32_t r = i / kFeatures, d = i % kFeatures;
+    encode_feature(feat[(row0 + r) * kFeatures + d], d, X + r * kXStride);
   }
+  for (uint32_t i = threadIdx.x; i < kFuseRows * (kInPad - kEncoded); i += 256u)
+    X[(i / (kInPad - kEncoded)) * kXStride + kEncoded + i % (kInPad - kEncoded)] = 0;
   __syncthreads();
 #pragma unroll 1
   for (uint32_t l = 0; l < kHidden; ++l) {
