@@ -8,11 +8,11 @@ namespace mlp {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-// float -> bf16 bits, round to nearest even (inputs are finite).
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
+  return __builtin_bit_cast(uint16_t, __bf16(x));
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
@@ -237,14 +237,19 @@ hipError_t launch_encode(const float *features, uint32_t n, bf16 *out, hipStream
 // encoding through the six hidden layers; only the 9 input features and the
 // 16 outputs per row touch HBM, and the weights (128 KB per layer, shared by
 // every tile) come from L2. Wave w computes output columns [64 w, 64 w + 64)
-// of all 128 rows (4 x 2 tiles of v_mfma_f32_32x32x16_bf16) with its weight
-// fragments loaded one K step ahead; the output layer (16 columns) gives
-// wave w rows [32 w, 32 w + 32). The MFMAs of every output element run in the
-// same K order as k_gemm's, and the epilogues round the same way, so the
-// result equals the layer-by-layer forward bit for bit.
+// of all 128 rows (4 x 2 tiles of v_mfma_f32_32x32x16_bf16, weights as the
+// A operand so each lane ends with 4 adjacent columns of one row: one 8-byte
+// LDS store) with its weight fragments loaded kWAhead K steps ahead; the
+// output layer (16 columns) gives wave w rows [32 w, 32 w + 32). The MFMAs
+// of every output element run in the same K order as k_gemm's and the
+// epilogues round the same way, so the result equals the layer-by-layer
+// forward bit for bit (GPU test).
 // ---------------------------------------------------------------------------
 static_assert(kInPad == kWidth, "hidden weight matrices are all kWidth x kWidth");
 constexpr uint32_t kFuseRows = 128, kXStride = kWidth + 8;
+// weight fragments in flight: 1 / 2 / 3 / 4 steps ahead measured 1.12 / 1.04 /
+// 1.01 / 1.03 ms per 2^20 queries
+constexpr uint32_t kWAhead = 3;
 constexpr size_t kFuseLds = size_t(kFuseRows) * kXStride * sizeof(uint16_t);
 
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ feat,
@@ -275,15 +280,26 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ f
           acc[i][j][r] = 0.f;
     const uint16_t *wrow[2] = {W + size_t(wave * 64u + l32) * kWidth + khalf,
                                W + size_t(wave * 64u + 32u + l32) * kWidth + khalf};
-    bf16x8_t b_next[2] = {*reinterpret_cast<const bf16x8_t *>(wrow[0]),
-                          *reinterpret_cast<const bf16x8_t *>(wrow[1])};
+    // Weight fragments are loaded kWAhead K steps before their MFMAs (a ring
+    // of registers); scheduling barriers keep the compiler from sinking the
+    // loads next to their use, which left every step waiting on L2.
+    constexpr uint32_t kSteps = kWidth / 16;
+    bf16x8_t bq[kWAhead + 1][2];
 #pragma unroll
-    for (uint32_t k0 = 0; k0 < kWidth; k0 += 16) {
-      const bf16x8_t b[2] = {b_next[0], b_next[1]};
-      if (k0 + 16 < kWidth) {
-        b_next[0] = *reinterpret_cast<const bf16x8_t *>(wrow[0] + k0 + 16);
-        b_next[1] = *reinterpret_cast<const bf16x8_t *>(wrow[1] + k0 + 16);
+    for (uint32_t s = 0; s < kWAhead; ++s)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        bq[s][j] = *reinterpret_cast<const bf16x8_t *>(wrow[j] + s * 16u);
+#pragma unroll
+    for (uint32_t ks = 0; ks < kSteps; ++ks) {
+      const uint32_t k0 = ks * 16u;
+      if (ks + kWAhead < kSteps) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          bq[(ks + kWAhead) % (kWAhead + 1)][j] =
+              *reinterpret_cast<const bf16x8_t *>(wrow[j] + k0 + kWAhead * 16u);
       }
+      __builtin_amdgcn_sched_barrier(0);
       bf16x8_t a[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -292,19 +308,29 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ f
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              bq[ks % (kWAhead + 1)][j], a[i], acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads(); // every wave has read this layer's input
+    // acc[i][j] holds the transposed tile: register r of lane l is output
+    // column 64 w + 32 j + 8 (r / 4) + 4 (l / 32) + r % 4 of row 32 i + l % 32,
+    // so each lane stores 4 adjacent columns at once
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const uint32_t row = i * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
-          const uint32_t col = wave * 64u + j * 32u + l32;
-          const float v = acc[i][j][r];
-          X[row * kXStride + col] = f2bf(v > 0.f ? v : 0.f);
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t row = i * 32u + l32;
+          const uint32_t col = wave * 64u + j * 32u + q * 8u + (lane >> 5) * 4u;
+          bf16x4_t v;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float x = acc[i][j][4 * q + e];
+            v[e] = __bf16(x > 0.f ? x : 0.f);
+          }
+          *reinterpret_cast<bf16x4_t *>(X + row * kXStride + col) = v;
         }
     __syncthreads();
   }
