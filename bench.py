@@ -450,23 +450,43 @@ def bench_mlp(a, cfg):
         "train_tflops": round(train_tflops, 2),
         "inference_mqueries_per_s": round((1 << 20) / fwd_s / 1e6, 2),
     }
-    # dominant kernel: the fused inference forward (k_mlp_forward) over 2^20
-    # rows; the layer-by-layer GEMM (training forward) is reported beside it
+    out.update(mlp_forward_timing(m))
+    print(json.dumps(out))
+    return 0
+
+
+def mlp_forward_timing(m, iters=20):
+    """Roofline of the dominant inference kernel (k_mlp_forward over 2^20
+    queries) and the layer-by-layer GEMM (training forward) beside it."""
     rows_t = 1 << 20
-    gemm_ms, fwd_ms = m.time_forward(rows_t, 20)
+    gemm_ms, fwd_ms = m.time_forward(rows_t, iters)
     gemm_tflops = 2.0 * rows_t * 256 * 256 / (gemm_ms * 1e-3) / 1e12
     fwd_tflops = 2.0 * rows_t * MLP_MACS / (fwd_ms * 1e-3) / 1e12
-    out["forward_ms_per_1M_queries"] = round(fwd_ms, 4)
-    out["layer_gemm"] = {"ms": round(gemm_ms, 4), "tflops": round(gemm_tflops, 2),
-                         "shape": "2^20 x 256 x 256, BF16 in, FP32 accumulate"}
-    out["roofline"] = {"bound": "mfma", "achieved": round(fwd_tflops, 2), "peak": 2500.0,
-                       "unit": "TFLOP/s", "frac": round(fwd_tflops / 2500.0, 5),
-                       "traffic": None, "kernel": "k_mlp_forward",
-                       "avg_launch_ms": round(fwd_ms, 4),
-                       "flops_per_launch": 2 * rows_t * MLP_MACS,
-                       "note": "whole inference forward of 2^20 queries in one kernel "
-                               "(encoding, 6 x 256 hidden, output; activations in LDS), "
-                               "valid (unpadded) MACs, HIP events on the MLP stream"}
+    return {
+        "forward_ms_per_1M_queries": round(fwd_ms, 4),
+        "layer_gemm": {"ms": round(gemm_ms, 4), "tflops": round(gemm_tflops, 2),
+                       "shape": "2^20 x 256 x 256, BF16 in, FP32 accumulate"},
+        "roofline": {"bound": "mfma", "achieved": round(fwd_tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(fwd_tflops / 2500.0, 5),
+                     "traffic": None, "kernel": "k_mlp_forward",
+                     "avg_launch_ms": round(fwd_ms, 4),
+                     "flops_per_launch": 2 * rows_t * MLP_MACS,
+                     "note": "whole inference forward of 2^20 queries in one kernel "
+                             "(encoding, 6 x 256 hidden, output; activations in LDS), "
+                             "valid (unpadded) MACs, HIP events on the MLP stream"},
+    }
+
+
+def bench_mlp_forward(a):
+    """Inference forward alone (random weights): the c5 network's dominant
+    kernel, for profiling."""
+    from path_amd.mlp import RadianceMLP
+    m = RadianceMLP(0, seed=1)
+    out = {"metric": "radiance-MLP inference forward (1 GPU)", "unit": "M queries/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 (fp32 accumulate)",
+           "data": "synthetic", "config": {"workload": "radiance_mlp_6x256_forward_2^20_queries"}}
+    out.update(mlp_forward_timing(m, a.steps))
+    out["value"] = round((1 << 20) / (out["forward_ms_per_1M_queries"] * 1e-3) / 1e6, 2)
     print(json.dumps(out))
     return 0
 
@@ -552,7 +572,12 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (scale for the "
                          "profiler's FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--mlp-forward", action="store_true",
+                    help="only time the radiance MLP's fused inference forward (2^20 queries, "
+                         "--steps launches)")
     a = ap.parse_args(argv)
+    if a.mlp_forward:
+        return bench_mlp_forward(a)
     if a.copy_calibration:
         import torch
         x = torch.ones(256 * 1024 * 1024 // 4, device="cuda")

@@ -54,13 +54,11 @@ struct WfState {
 };
 
 // Ray queues hold slot indices; the rays themselves are read from the state.
-// Group g's entries start at g * group_cap; prefix[0..n] (extension) and
-// prefix[kQueueGroups + 1 ..] (shadow) are the exclusive prefix sums of the
-// group counts, written by k_wf_prefix after each shade step.
+// Group g's entries start at g * group_cap; the traversal kernels read the
+// group counts and concatenate the segments.
 struct WfQueues {
   uint32_t *ext;
   uint32_t *shadow;
-  uint32_t *prefix;
   uint32_t group_cap;
 };
 
@@ -110,10 +108,9 @@ struct WfParams {
 // zero them and the group queue counters (one block; safe inside a captured
 // graph).
 hipError_t launch_wf_reset(uint32_t *counters, uint32_t *group_counters, hipStream_t s);
-// Prefix sums of the group queue counts into q.prefix, totals into
-// counters[kExtCount], counters[kShadowCount].
-hipError_t launch_wf_prefix(const WfParams &p, hipStream_t s);
 hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s);
+// The extension trace also writes the iteration's queue totals into
+// counters[kExtCount], counters[kShadowCount].
 hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
                                uint32_t grid_blocks, hipStream_t s);
 hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,

@@ -54,27 +54,6 @@ hipError_t launch_wf_reset(uint32_t *counters, uint32_t *group_counters, hipStre
   return hipGetLastError();
 }
 
-__global__ void k_wf_prefix(uint32_t *counters, uint32_t *group_counters,
-                            uint32_t *prefix) {
-  if (threadIdx.x >= 2)
-    return;
-  const uint32_t kind = threadIdx.x == 0 ? kGroupExt : kGroupShadow;
-  uint32_t *out = prefix + (threadIdx.x == 0 ? 0 : kQueueGroups + 1);
-  uint32_t sum = 0;
-  for (uint32_t g = 0; g < kQueueGroups; ++g) {
-    out[g] = sum;
-    sum += *group_counter(group_counters, kind, g);
-  }
-  out[kQueueGroups] = sum;
-  counters[threadIdx.x == 0 ? kExtCount : kShadowCount] = sum;
-}
-
-hipError_t launch_wf_prefix(const WfParams &p, hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_prefix, dim3(1), dim3(64), 0, s, p.counters, p.group_counters,
-                     p.q.prefix);
-  return hipGetLastError();
-}
-
 // Samples are dealt to the groups in chunks of kSampleChunk consecutive
 // samples, round robin, so every group sweeps the whole image at the same
 // pace (contiguous shares measured 8% slower on the 82k-triangle scene: the
@@ -416,9 +395,25 @@ constexpr uint32_t kRefillMinIdle = 16;
 template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStackInts * 4];
+  // exclusive prefix of the group segment sizes (the shade step's counters)
   __shared__ uint32_t s_prefix[kQueueGroups + 1];
-  if (threadIdx.x <= kQueueGroups)
-    s_prefix[threadIdx.x] = p.q.prefix[(kShadow ? kQueueGroups + 1 : 0) + threadIdx.x];
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (uint32_t g = 0; g < kQueueGroups; ++g) {
+      s_prefix[g] = sum;
+      sum += *group_counter(p.group_counters, kShadow ? kGroupShadow : kGroupExt, g);
+    }
+    s_prefix[kQueueGroups] = sum;
+    // the totals the host polls: written once per iteration, before the
+    // host's copy of the counters (which follows the extension trace)
+    if (!kShadow && blockIdx.x == 0) {
+      uint32_t shadow = 0;
+      for (uint32_t g = 0; g < kQueueGroups; ++g)
+        shadow += *group_counter(p.group_counters, kGroupShadow, g);
+      p.counters[kExtCount] = sum;
+      p.counters[kShadowCount] = shadow;
+    }
+  }
   __syncthreads();
   const uint32_t count = s_prefix[kQueueGroups];
   const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;

@@ -171,7 +171,7 @@ struct pt_renderer {
   pt::host::DeviceBuf<uint32_t> wf_q_ext, wf_q_shadow;
   pt::host::DeviceBuf<int> wf_stack_overflow;
   pt::host::DeviceBuf<uint32_t> wf_counters;
-  pt::host::DeviceBuf<uint32_t> wf_group_counters, wf_queue_prefix;
+  pt::host::DeviceBuf<uint32_t> wf_group_counters;
   pt::host::DeviceBuf<float> wf_samples;
   pt::host::DeviceBuf<uint32_t> wf_scene;
   uint32_t *wf_host_counts = nullptr; // pinned

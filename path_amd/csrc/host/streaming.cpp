@@ -44,7 +44,6 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
   HIP_TRY(r.wf_q_shadow.ensure(size_t(group_cap) * n_groups));
   HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
   HIP_TRY(r.wf_group_counters.ensure(gpu::kGroupCounterWords));
-  HIP_TRY(r.wf_queue_prefix.ensure(2 * (gpu::kQueueGroups + 1)));
   HIP_TRY(r.wf_samples.ensure(size_t(total_samples) * 3));
   const uint32_t trace_threads = trace_blocks * 256;
   HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
@@ -54,7 +53,7 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
           r.wf_total.ptr,    r.wf_origin.ptr, r.wf_dir.ptr,
           r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr,
           batched ? r.wf_scene.ptr : nullptr};
-  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr, r.wf_queue_prefix.ptr, group_cap};
+  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr, group_cap};
   p.counters = r.wf_counters.ptr;
   p.group_counters = r.wf_group_counters.ptr;
   p.n_groups = n_groups;
@@ -66,8 +65,8 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
 }
 
 // Enqueue one iteration: fold the previous queue counts into the totals and
-// zero them, shade, copy the counts to pinned host memory, trace extension
-// and shadow rays.
+// zero them, shade, trace extension and shadow rays, copy the counts to
+// pinned host memory.
 static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
                                     bool batched, uint32_t trace_blocks,
                                     bool profile_events) {
@@ -76,12 +75,6 @@ static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
   if ((e = gpu::launch_wf_reset(p.counters, p.group_counters, s)) != hipSuccess)
     return e;
   if ((e = gpu::launch_wf_shade(p, batched, s)) != hipSuccess)
-    return e;
-  if ((e = gpu::launch_wf_prefix(p, s)) != hipSuccess)
-    return e;
-  if ((e = hipMemcpyAsync(r.wf_host_counts, p.counters,
-                          sizeof(uint32_t) * gpu::kNumCounters,
-                          hipMemcpyDeviceToHost, s)) != hipSuccess)
     return e;
   if (profile_events && (e = hipEventRecord(r.ev_prof[0], s)) != hipSuccess)
     return e;
@@ -93,7 +86,9 @@ static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
     return e;
   if (profile_events && (e = hipEventRecord(r.ev_prof[2], s)) != hipSuccess)
     return e;
-  return hipSuccess;
+  // queue totals of this iteration (written by the extension trace)
+  return hipMemcpyAsync(r.wf_host_counts, p.counters, sizeof(uint32_t) * gpu::kNumCounters,
+                        hipMemcpyDeviceToHost, s);
 }
 
 int wavefront_run(pt_renderer &r, gpu::WfParams &p, bool batched,
