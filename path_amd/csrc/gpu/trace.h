@@ -1,6 +1,6 @@
 // BVH traversal on gfx950: one ray per lane, near-first order, per-lane stack.
 //
-// 4-wide nodes (64 B: grid origin and exponents, child refs, 8-bit quantised
+// 4-wide nodes (64 B: grid origin and power-of-two steps, child refs, 8-bit quantised
 // child boxes) and leaf-ordered triangles are produced by host/bvh.cpp. Child refs >= 0 are
 // inner nodes, refs < 0 are ~(first triangle of a leaf), INT32_MIN is an empty
 // slot; the last triangle of a leaf has bit 31 of its index word set.
@@ -127,21 +127,20 @@ struct TraceState {
   // 82k-triangle scene).
   template <class Stack>
   __device__ __forceinline__ void visit_inner(const float4 *__restrict__ nodes, Stack &stack) {
+    // origin xyz, step x | step yz, child 0-1 | child 2-3, lo_x, hi_x |
+    // lo_y, hi_y, lo_z, hi_z
     const float4 *np = nodes + kNodeFloat4 * size_t(node);
-    const float4 f0 = np[0];
-    const int4 ch = __builtin_bit_cast(int4, np[1]);
-    const uint4 qa = __builtin_bit_cast(uint4, np[2]);
-    const uint4 qb = __builtin_bit_cast(uint4, np[3]);
-    // plane of code q on axis a: fma(q, 2^e_a, origin_a); in ray distance
-    // t = (plane - o) / d  ~  fma(q, 2^e_a / d_a, (origin_a - o_a) / d_a)
-    const uint32_t ew = __float_as_uint(f0.w);
-    const auto pow2 = [](uint32_t byte) {
-      return __uint_as_float(uint32_t(int32_t(int8_t(byte & 0xffu)) + 127) << 23);
-    };
-    const float sx = pow2(ew) * inv.x, sy = pow2(ew >> 8) * inv.y, sz = pow2(ew >> 16) * inv.z;
+    const float4 f0 = np[0], f1 = np[1];
+    const uint4 f2 = __builtin_bit_cast(uint4, np[2]), f3 = __builtin_bit_cast(uint4, np[3]);
+    const uint4 qa = make_uint4(f2.z, f2.w, f3.x, f3.y); // lo_x, hi_x, lo_y, hi_y
+    const uint2 qb = make_uint2(f3.z, f3.w);             // lo_z, hi_z
+    // plane of code q on axis a: fma(q, step_a, origin_a); in ray distance
+    // t = (plane - o) / d  ~  fma(q, step_a / d_a, (origin_a - o_a) / d_a)
+    const float sx = f0.w * inv.x, sy = f1.x * inv.y, sz = f1.y * inv.z;
     const float bx = fmaf(f0.x, inv.x, -oi.x), by = fmaf(f0.y, inv.y, -oi.y),
                 bz = fmaf(f0.z, inv.z, -oi.z);
-    const int chs[4] = {ch.x, ch.y, ch.z, ch.w};
+    const int chs[4] = {int(__float_as_uint(f1.z)), int(__float_as_uint(f1.w)), int(f2.x),
+                        int(f2.y)};
     // The decode is monotone in the code (fma with a step of the sign of
     // 1/d), so the entry plane of each axis is the lo code for a positive
     // direction and the hi code for a negative one: select the code words

@@ -276,7 +276,7 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
       }
       const float scale = std::ldexp(1.0f, e);
       o.origin[a] = nlo[a];
-      o.exponent[a] = int8_t(e);
+      o.step[a] = scale;
       for (uint32_t k = 0; k < 4; ++k) {
         if (k >= nk) {
           qlo[a][k] = 255;

@@ -1,6 +1,8 @@
 // Host SAH BVH builder and its repack into the GPU node layout.
 #pragma once
 
+#include <cstddef>
+
 #include "../core/geometry.h"
 
 #include <climits>
@@ -51,8 +53,8 @@ Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
 // the up-to-4 descendants obtained by repeatedly opening the child with the
 // largest surface area), 64 bytes per node with the four child boxes
 // quantised to 8 bits on a per-node grid:
-//   origin (x, y, z) float and exponents (ex, ey, ez) int8 -> plane
-//     = fma(q, 2^e, origin) per axis,
+//   origin (x, y, z) and step (sx, sy, sz) floats, each step a power of two
+//     -> plane = fma(q, step, origin) per axis,
 //   child[4] (>= 0 inner node, < 0 ~(first triangle) of a leaf, kEmptyChild
 //     for an unused slot),
 //   q lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4] (uint8).
@@ -65,14 +67,18 @@ constexpr int32_t kEmptyChild = INT32_MIN;
 
 struct GpuNode {
   float origin[3];
-  int8_t exponent[3];
-  uint8_t pad0;
+  float step[3];
   int32_t child[4];
   uint8_t lo_x[4], hi_x[4], lo_y[4], hi_y[4];
   uint8_t lo_z[4], hi_z[4];
-  uint32_t pad1[2];
 };
 static_assert(sizeof(GpuNode) == 64, "GPU BVH node is one 64-byte record");
+// gpu/trace.h reads the node as four float4: origin + step x, steps y z +
+// child 0 1, child 2 3 + lo_x hi_x, lo_y hi_y lo_z hi_z
+static_assert(offsetof(GpuNode, step) == 12 && offsetof(GpuNode, child) == 24 &&
+                  offsetof(GpuNode, lo_x) == 40 && offsetof(GpuNode, lo_y) == 48 &&
+                  offsetof(GpuNode, lo_z) == 56 && offsetof(GpuNode, hi_z) == 60,
+              "GpuNode layout read by the traversal");
 
 // Triangle in leaf order: (v0, idx bits), (e1, 0), (e2, 0).
 struct GpuTri {
