@@ -38,15 +38,20 @@ __host__ __device__ inline uint32_t *group_counter(uint32_t *base, uint32_t kind
   return base + (size_t(kind) * kQueueGroups + g) * kGroupStride;
 }
 
-// Per-slot path state, structure of arrays (one 16-byte vector per slot and
-// array so a wave's accesses are contiguous).
+// Three packed floats (12 bytes: one dwordx3 access per lane).
+struct F3 {
+  float x, y, z;
+};
+
+// Per-slot path state, structure of arrays (one 16- or 12-byte record per
+// slot and array, so a wave's accesses are contiguous).
 struct WfState {
   uint4 *meta;       // location, sample, rng counter, flags
   float4 *mult;      // ray multiplier xyz, light target distance
-  float4 *light;     // light-sample multiplier xyz
-  float4 *total;     // accumulated radiance xyz
-  float4 *origin;    // origin of the pending rays (last vertex / camera)
-  float4 *dir;       // direction of the pending continuation ray
+  F3 *light;         // light-sample multiplier
+  F3 *total;         // accumulated radiance
+  F3 *origin;        // origin of the pending rays (last vertex / camera)
+  F3 *dir;           // direction of the pending continuation ray
   float4 *light_dir; // direction of the pending light ray, target distance
   uint2 *hit_ext;    // packed closest hit of the continuation ray
   uint2 *hit_light;  // packed closest hit of the light ray

@@ -179,10 +179,10 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
   if (active) {
     SlotScene sc = slot_scene<kBatch>(p, scene);
     float4 m = p.st.mult[slot];
-    float4 lm = (meta.w & kFlagLight) ? p.st.light[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 tot = p.st.total[slot];
-    float4 o = p.st.origin[slot];
-    float4 d = (meta.w & kFlagHasNext) ? p.st.dir[slot] : o;
+    const F3 lm = (meta.w & kFlagLight) ? p.st.light[slot] : F3{0.f, 0.f, 0.f};
+    const F3 tot = p.st.total[slot];
+    const F3 o = p.st.origin[slot];
+    const F3 d = (meta.w & kFlagHasNext) ? p.st.dir[slot] : o;
     PathState<float> st;
     st.iters = meta.w & kItersMask;
     st.count_emission = (meta.w & kFlagCountEmission) ? 1u : 0u;
@@ -305,13 +305,12 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     p.st.mult[slot] = make_float4(ns.ray_mult.x, ns.ray_mult.y, ns.ray_mult.z,
                                   ns.light_target);
     if (push_light)
-      p.st.light[slot] = make_float4(ns.light_mult.x, ns.light_mult.y,
-                                     ns.light_mult.z, 0.f);
+      p.st.light[slot] = F3{ns.light_mult.x, ns.light_mult.y, ns.light_mult.z};
     if (total_dirty)
-      p.st.total[slot] = make_float4(ns.total.x, ns.total.y, ns.total.z, 0.f);
-    p.st.origin[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.f);
+      p.st.total[slot] = F3{ns.total.x, ns.total.y, ns.total.z};
+    p.st.origin[slot] = F3{ext_o.x, ext_o.y, ext_o.z};
     if (push_ext)
-      p.st.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 0.f);
+      p.st.dir[slot] = F3{ext_d.x, ext_d.y, ext_d.z};
     if (push_light)
       p.st.light_dir[slot] =
           make_float4(light_d.x, light_d.y, light_d.z, light_target);
@@ -444,8 +443,14 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
           if (s_prefix[g + step] <= v)
             g += step;
         slot = q[size_t(g) * p.q.group_cap + (v - s_prefix[g])];
-        const float4 o = p.st.origin[slot];
-        const float4 d = kShadow ? p.st.light_dir[slot] : p.st.dir[slot];
+        const F3 o = p.st.origin[slot];
+        float4 d;
+        if constexpr (kShadow) {
+          d = p.st.light_dir[slot];
+        } else {
+          const F3 e = p.st.dir[slot];
+          d = make_float4(e.x, e.y, e.z, 0.f);
+        }
         const Ray<float> ray = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
         bool has_nodes = true;
         if constexpr (kBatch) {

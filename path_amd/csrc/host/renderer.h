@@ -167,7 +167,8 @@ struct pt_renderer {
   // wavefront integrator buffers
   pt::host::DeviceBuf<uint4> wf_meta;
   pt::host::DeviceBuf<uint2> wf_hit_ext, wf_hit_light;
-  pt::host::DeviceBuf<float4> wf_mult, wf_light, wf_total, wf_origin, wf_dir, wf_light_dir;
+  pt::host::DeviceBuf<float4> wf_mult, wf_light_dir;
+  pt::host::DeviceBuf<pt::gpu::F3> wf_light, wf_total, wf_origin, wf_dir;
   pt::host::DeviceBuf<uint32_t> wf_q_ext, wf_q_shadow;
   pt::host::DeviceBuf<int> wf_stack_overflow;
   pt::host::DeviceBuf<uint32_t> wf_counters;
