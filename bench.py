@@ -611,7 +611,7 @@ def main(argv=None):
     r = Renderer(0)
     if a.trace_blocks_per_cu or a.iters_per_sync:
         from path_amd.datagen import set_launch_params
-        set_launch_params(r, a.trace_blocks_per_cu or 8, a.iters_per_sync or 1,
+        set_launch_params(r, a.trace_blocks_per_cu or 6, a.iters_per_sync or 1,
                           bool(a.iters_per_sync))
     size, spp = cfg["size"], cfg["spp"]
     for _ in range(a.warmup):

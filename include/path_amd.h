@@ -280,9 +280,12 @@ int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                     double *seconds);
 
 /* Launch parameters of the wavefront integrator. Results never depend on
- * them. trace_blocks_per_cu: traversal grid = CUs x this x 256 threads;
- * iters_per_sync: iterations enqueued between host checks; use_graphs:
- * replay those iterations as one captured hipGraph. */
+ * them. The path pool (max_num_samples_per_launch paths) runs as two
+ * independent pipelines of half the pool each, on two streams driven by two
+ * host threads, so one's shading overlaps the other's traversal.
+ * trace_blocks_per_cu: traversal grid per pipeline = CUs x this x 256
+ * threads (default 6); iters_per_sync: iterations enqueued between host
+ * checks; use_graphs: replay those iterations as one captured hipGraph. */
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
                          uint32_t iters_per_sync, int use_graphs);
 /* Time candidate (path pool, traversal grid, iterations per sync) triples on

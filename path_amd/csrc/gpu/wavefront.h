@@ -102,6 +102,10 @@ struct WfParams {
   uint32_t *group_counters; // GroupCounter words per group (group_counter())
   uint32_t n_groups;        // min(kQueueGroups, shade blocks)
   uint32_t n_shade_blocks;
+  // Samples are dealt over the groups of every pipeline of a render (the
+  // path pool can be split into independent pipelines on separate streams):
+  // this launch's group g is group group_base + g of n_groups_total.
+  uint32_t group_base, n_groups_total;
   float *out_samples;                 // [pixel * spp + sample] * 3
   float *gbuffer; // first hit of sample 0 per pixel: position, normal,
                   // albedo, emission (12 floats) or null

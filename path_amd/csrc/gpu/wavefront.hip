@@ -60,22 +60,22 @@ hipError_t launch_wf_reset(uint32_t *counters, uint32_t *group_counters, hipStre
 // groups whose image band has the longer paths set the tail).
 constexpr uint32_t kSampleChunk = 256;
 
-// Number of samples group g starts.
+// Number of samples group g (of this pipeline) starts.
 __device__ __forceinline__ unsigned long long group_sample_count(const WfParams &p,
                                                                  uint32_t g) {
-  const unsigned long long round = (unsigned long long)p.n_groups * kSampleChunk;
+  const unsigned long long round = (unsigned long long)p.n_groups_total * kSampleChunk;
   const unsigned long long rem = p.total_samples % round;
-  const unsigned long long first = (unsigned long long)g * kSampleChunk;
+  const unsigned long long first = (unsigned long long)(p.group_base + g) * kSampleChunk;
   const unsigned long long tail =
       rem > first ? (rem - first < kSampleChunk ? rem - first : kSampleChunk) : 0ull;
   return p.total_samples / round * kSampleChunk + tail;
 }
 
-// Global index of the i-th sample of group g.
+// Global index of the i-th sample of group g (of this pipeline).
 __device__ __forceinline__ unsigned long long group_sample(const WfParams &p, uint32_t g,
                                                            unsigned long long i) {
-  return i / kSampleChunk * ((unsigned long long)p.n_groups * kSampleChunk) +
-         (unsigned long long)g * kSampleChunk + i % kSampleChunk;
+  return i / kSampleChunk * ((unsigned long long)p.n_groups_total * kSampleChunk) +
+         (unsigned long long)(p.group_base + g) * kSampleChunk + i % kSampleChunk;
 }
 
 template <class R>
@@ -357,7 +357,7 @@ hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
   uint32_t blocks = (p.n_slots + 255) / 256;
   // groups and sample shares are laid out for exactly this grid
   if (blocks != p.n_shade_blocks || p.n_groups == 0 || p.n_groups > kQueueGroups ||
-      p.n_groups > blocks)
+      p.n_groups > blocks || p.group_base + p.n_groups > p.n_groups_total)
     return hipErrorInvalidValue;
   bool sobol = p.rng.type == kRngSobel;
   if (batched) {

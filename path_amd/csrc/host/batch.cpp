@@ -216,9 +216,6 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   p.batch.materials = r.b_mats.ptr;
   p.batch.light_tris = r.b_light_tris.ptr;
   p.batch.light_cdf = r.b_light_cdf.ptr;
-  rc = wavefront_alloc(r, p, uint32_t(slots), total, true, trace_blocks);
-  if (rc != PT_OK)
-    return rc;
   bool want_gbuffer = false;
   for (uint32_t i = 0; i < n; ++i)
     want_gbuffer |= items[i].out_gbuffer != nullptr;
@@ -227,11 +224,15 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     HIP_TRY(hipMemsetAsync(r.b_gbuffer.ptr, 0, size_t(n_pixels) * 12 * sizeof(float), s));
     p.gbuffer = r.b_gbuffer.ptr;
   }
+  WfPipes w;
+  rc = wavefront_alloc(r, w, p, uint32_t(slots), total, true, trace_blocks);
+  if (rc != PT_OK)
+    return rc;
   HIP_TRY(r.rgb.ensure(size_t(n_pixels) * 3));
   r.stats = pt_render_stats{};
   r.stats.profiled = r.profiling ? 1u : 0u;
   HIP_TRY(hipEventRecord(r.ev_start, s));
-  rc = wavefront_run(r, p, true, trace_blocks, r.iters_per_sync);
+  rc = wavefront_run(r, w, true, trace_blocks, r.iters_per_sync);
   if (rc != PT_OK)
     return rc;
   // per-scene reduction in the single-scene order
@@ -286,7 +287,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[4]) {
   const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
-  const uint32_t blocks[] = {4, 8, 16};
+  const uint32_t blocks[] = {4, 6, 8, 16};
   const uint32_t iters[] = {1, 4};
   double best = 1e30;
   uint32_t bp = settings.max_num_samples_per_launch, bb = r.trace_blocks_per_cu,

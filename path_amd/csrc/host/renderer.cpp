@@ -112,11 +112,16 @@ pt_renderer::~pt_renderer() {
     (void)hipEventDestroy(ev_start);
   if (ev_stop)
     (void)hipEventDestroy(ev_stop);
-  for (hipEvent_t e : ev_prof)
-    if (e)
-      (void)hipEventDestroy(e);
+  if (ev_fork)
+    (void)hipEventDestroy(ev_fork);
+  for (auto &pipe : ev_prof)
+    for (hipEvent_t e : pipe)
+      if (e)
+        (void)hipEventDestroy(e);
   if (wf_host_counts)
     (void)hipHostFree(wf_host_counts);
+  if (stream_b)
+    (void)hipStreamDestroy(stream_b);
   if (stream)
     (void)hipStreamDestroy(stream);
 }
@@ -127,11 +132,16 @@ bool pt_renderer::init(int dev, std::string *err) {
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
   if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking);
+  if (e == hipSuccess)
     e = hipEventCreate(&ev_start);
   if (e == hipSuccess)
     e = hipEventCreate(&ev_stop);
-  for (int i = 0; i < 3 && e == hipSuccess; ++i)
-    e = hipEventCreate(&ev_prof[i]);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+  for (int h = 0; h < kMaxPipes && e == hipSuccess; ++h)
+    for (int i = 0; i < 3 && e == hipSuccess; ++i)
+      e = hipEventCreate(&ev_prof[h][i]);
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&n_cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) {
@@ -334,13 +344,13 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
     for (uint32_t start = 0; start < total; start += chunk) {
       uint32_t end = start + chunk < total ? start + chunk : total;
       if (profiling)
-        HIP_TRY(hipEventRecord(ev_prof[0], stream));
+        HIP_TRY(hipEventRecord(ev_prof[0][0], stream));
       HIP_TRY(gpu::launch_megakernel(p, start, end, stream));
       if (profiling) {
-        HIP_TRY(hipEventRecord(ev_prof[1], stream));
-        HIP_TRY(hipEventSynchronize(ev_prof[1]));
+        HIP_TRY(hipEventRecord(ev_prof[0][1], stream));
+        HIP_TRY(hipEventSynchronize(ev_prof[0][1]));
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, ev_prof[0], ev_prof[1]));
+        HIP_TRY(hipEventElapsedTime(&ms, ev_prof[0][0], ev_prof[0][1]));
         stats.mega_ms += ms;
       }
       stats.mega_launches++;

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
+#include <thread>
 
 using namespace pt;
 using namespace pt::host;
@@ -23,8 +25,9 @@ using namespace pt::host;
 namespace pt {
 namespace host {
 
-int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
-                    uint64_t total_samples, bool batched, uint32_t trace_blocks) {
+int wavefront_alloc(pt_renderer &r, WfPipes &w, const gpu::WfParams &base,
+                    uint32_t n_slots, uint64_t total_samples, bool batched,
+                    uint32_t trace_blocks) {
   HIP_TRY(r.wf_meta.ensure(n_slots));
   HIP_TRY(r.wf_hit_ext.ensure(n_slots));
   HIP_TRY(r.wf_hit_light.ensure(n_slots));
@@ -36,155 +39,221 @@ int wavefront_alloc(pt_renderer &r, gpu::WfParams &p, uint32_t n_slots,
   HIP_TRY(r.wf_light_dir.ensure(n_slots));
   if (batched)
     HIP_TRY(r.wf_scene.ensure(n_slots));
-  // queue segments: one per group, sized for the group's shade blocks
-  const uint32_t shade_blocks = (n_slots + 255) / 256;
-  const uint32_t n_groups = std::min(gpu::kQueueGroups, shade_blocks);
-  const uint32_t group_cap = (shade_blocks + n_groups - 1) / n_groups * 256;
-  HIP_TRY(r.wf_q_ext.ensure(size_t(group_cap) * n_groups));
-  HIP_TRY(r.wf_q_shadow.ensure(size_t(group_cap) * n_groups));
-  HIP_TRY(r.wf_counters.ensure(gpu::kNumCounters));
-  HIP_TRY(r.wf_group_counters.ensure(gpu::kGroupCounterWords));
   HIP_TRY(r.wf_samples.ensure(size_t(total_samples) * 3));
+  // Two pipelines (half the slots each, their own stream, queues and
+  // counters) once there is at least one shade block for each: a memory-bound
+  // shade step of one overlaps the ALU-bound traversal of the other.
+  w.n = n_slots >= 2 * 256 ? kMaxPipes : 1;
+  uint32_t first[kMaxPipes + 1] = {0};
+  for (int h = 0; h < w.n; ++h) {
+    const uint32_t left = n_slots - first[h], pipes_left = uint32_t(w.n - h);
+    uint32_t take = (left + pipes_left - 1) / pipes_left;
+    take = std::min(left, (take + 255) / 256 * 256);
+    first[h + 1] = first[h] + take;
+  }
+  uint32_t groups[kMaxPipes], caps[kMaxPipes], groups_total = 0;
+  size_t queue_total = 0;
+  for (int h = 0; h < w.n; ++h) {
+    const uint32_t blocks = (first[h + 1] - first[h] + 255) / 256;
+    groups[h] = std::min(gpu::kQueueGroups, blocks);
+    caps[h] = (blocks + groups[h] - 1) / groups[h] * 256;
+    groups_total += groups[h];
+    queue_total += size_t(caps[h]) * groups[h];
+  }
+  HIP_TRY(r.wf_q_ext.ensure(queue_total));
+  HIP_TRY(r.wf_q_shadow.ensure(queue_total));
+  HIP_TRY(r.wf_counters.ensure(size_t(kMaxPipes) * gpu::kNumCounters));
+  HIP_TRY(r.wf_group_counters.ensure(size_t(kMaxPipes) * gpu::kGroupCounterWords));
   const uint32_t trace_threads = trace_blocks * 256;
-  HIP_TRY(r.wf_stack_overflow.ensure(size_t(gpu::kOverflowDepth) * trace_threads));
-  p.total_samples = total_samples;
-  p.n_slots = n_slots;
-  p.st = {r.wf_meta.ptr,     r.wf_mult.ptr,   r.wf_light.ptr,
-          r.wf_total.ptr,    r.wf_origin.ptr, r.wf_dir.ptr,
-          r.wf_light_dir.ptr, r.wf_hit_ext.ptr, r.wf_hit_light.ptr,
-          batched ? r.wf_scene.ptr : nullptr};
-  p.q = {r.wf_q_ext.ptr, r.wf_q_shadow.ptr, group_cap};
-  p.counters = r.wf_counters.ptr;
-  p.group_counters = r.wf_group_counters.ptr;
-  p.n_groups = n_groups;
-  p.n_shade_blocks = shade_blocks;
-  p.out_samples = r.wf_samples.ptr;
-  p.stack_overflow = r.wf_stack_overflow.ptr;
-  p.overflow_stride = trace_threads;
+  HIP_TRY(r.wf_stack_overflow.ensure(size_t(kMaxPipes) * gpu::kOverflowDepth * trace_threads));
+  size_t queue_off = 0;
+  uint32_t group_base = 0;
+  for (int h = 0; h < w.n; ++h) {
+    gpu::WfParams &p = w.p[h];
+    const uint32_t f = first[h];
+    p = base;
+    p.total_samples = total_samples;
+    p.n_slots = first[h + 1] - f;
+    p.st = {r.wf_meta.ptr + f,      r.wf_mult.ptr + f,    r.wf_light.ptr + f,
+            r.wf_total.ptr + f,     r.wf_origin.ptr + f,  r.wf_dir.ptr + f,
+            r.wf_light_dir.ptr + f, r.wf_hit_ext.ptr + f, r.wf_hit_light.ptr + f,
+            batched ? r.wf_scene.ptr + f : nullptr};
+    p.q = {r.wf_q_ext.ptr + queue_off, r.wf_q_shadow.ptr + queue_off, caps[h]};
+    queue_off += size_t(caps[h]) * groups[h];
+    p.counters = r.wf_counters.ptr + size_t(h) * gpu::kNumCounters;
+    p.group_counters = r.wf_group_counters.ptr + size_t(h) * gpu::kGroupCounterWords;
+    p.n_groups = groups[h];
+    p.n_shade_blocks = (p.n_slots + 255) / 256;
+    p.group_base = group_base;
+    p.n_groups_total = groups_total;
+    group_base += groups[h];
+    p.out_samples = r.wf_samples.ptr;
+    p.stack_overflow = r.wf_stack_overflow.ptr + size_t(h) * gpu::kOverflowDepth * trace_threads;
+    p.overflow_stride = trace_threads;
+  }
   return PT_OK;
 }
 
-// Enqueue one iteration: fold the previous queue counts into the totals and
-// zero them, shade, trace extension and shadow rays, copy the counts to
-// pinned host memory.
-static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p,
+// Enqueue one iteration of one pipeline: fold the previous queue counts into
+// the totals and zero them, shade, trace extension and shadow rays, copy the
+// counts to pinned host memory.
+static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p, int pipe,
                                     bool batched, uint32_t trace_blocks,
                                     bool profile_events) {
-  hipStream_t s = r.stream;
+  hipStream_t s = r.pipe_stream(pipe);
+  hipEvent_t *ev = r.ev_prof[pipe];
   hipError_t e;
   if ((e = gpu::launch_wf_reset(p.counters, p.group_counters, s)) != hipSuccess)
     return e;
   if ((e = gpu::launch_wf_shade(p, batched, s)) != hipSuccess)
     return e;
-  if (profile_events && (e = hipEventRecord(r.ev_prof[0], s)) != hipSuccess)
+  if (profile_events && (e = hipEventRecord(ev[0], s)) != hipSuccess)
     return e;
   if ((e = gpu::launch_wf_trace_ext(p, batched, trace_blocks, s)) != hipSuccess)
     return e;
-  if (profile_events && (e = hipEventRecord(r.ev_prof[1], s)) != hipSuccess)
+  if (profile_events && (e = hipEventRecord(ev[1], s)) != hipSuccess)
     return e;
   if ((e = gpu::launch_wf_trace_shadow(p, batched, trace_blocks, s)) != hipSuccess)
     return e;
-  if (profile_events && (e = hipEventRecord(r.ev_prof[2], s)) != hipSuccess)
+  if (profile_events && (e = hipEventRecord(ev[2], s)) != hipSuccess)
     return e;
   // queue totals of this iteration (written by the extension trace)
-  return hipMemcpyAsync(r.wf_host_counts, p.counters, sizeof(uint32_t) * gpu::kNumCounters,
-                        hipMemcpyDeviceToHost, s);
+  return hipMemcpyAsync(r.wf_host_counts + size_t(pipe) * gpu::kNumCounters, p.counters,
+                        sizeof(uint32_t) * gpu::kNumCounters, hipMemcpyDeviceToHost, s);
 }
 
-int wavefront_run(pt_renderer &r, gpu::WfParams &p, bool batched,
-                  uint32_t trace_blocks, uint32_t iters_per_sync) {
-  hipStream_t s = r.stream;
-  HIP_TRY(hipMemsetAsync(p.st.meta, 0, size_t(p.n_slots) * sizeof(uint4), s));
-  HIP_TRY(hipMemsetAsync(p.group_counters, 0, sizeof(uint32_t) * gpu::kGroupCounterWords, s));
-  HIP_TRY(hipMemsetAsync(p.counters, 0, sizeof(uint32_t) * gpu::kNumCounters, s));
+int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_blocks,
+                  uint32_t iters_per_sync) {
+  hipStream_t s0 = r.stream;
+  for (int h = 0; h < w.n; ++h) {
+    const gpu::WfParams &p = w.p[h];
+    HIP_TRY(hipMemsetAsync(p.st.meta, 0, size_t(p.n_slots) * sizeof(uint4), s0));
+    HIP_TRY(hipMemsetAsync(p.group_counters, 0, sizeof(uint32_t) * gpu::kGroupCounterWords, s0));
+    HIP_TRY(hipMemsetAsync(p.counters, 0, sizeof(uint32_t) * gpu::kNumCounters, s0));
+  }
   if (!r.wf_host_counts)
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&r.wf_host_counts),
-                          sizeof(uint32_t) * gpu::kNumCounters, hipHostMallocDefault));
+                          sizeof(uint32_t) * kMaxPipes * gpu::kNumCounters,
+                          hipHostMallocDefault));
+  // the other pipelines start after everything enqueued so far on stream 0
+  // (scene upload, state clears)
+  HIP_TRY(hipEventRecord(r.ev_fork, s0));
+  for (int h = 1; h < w.n; ++h)
+    HIP_TRY(hipStreamWaitEvent(r.pipe_stream(h), r.ev_fork, 0));
   const bool prof = r.profiling;
   if (iters_per_sync == 0)
     iters_per_sync = 1;
   // Profiling needs per-iteration event reads, so it syncs every iteration.
   if (prof)
     iters_per_sync = 1;
-  hipGraphExec_t exec = nullptr;
-  bool use_graph = r.use_graphs && !prof;
-  if (use_graph) {
-    hipGraph_t graph = nullptr;
-    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (uint32_t k = 0; k < iters_per_sync && e == hipSuccess; ++k)
-      e = enqueue_iteration(r, p, batched, trace_blocks, false);
-    hipError_t e2 = hipStreamEndCapture(s, &graph);
-    if (e != hipSuccess || e2 != hipSuccess) {
+  const bool use_graph = r.use_graphs && !prof;
+  // Each pipeline runs its own host loop (pipeline 1 on a second host
+  // thread), so the pipelines drift apart and one's shade step overlaps the
+  // other's traversal instead of running in lockstep.
+  struct PipeResult {
+    hipError_t err = hipSuccess;
+    std::string what;
+    uint64_t launches = 0;
+    double ext_ms = 0.0, shadow_ms = 0.0;
+  };
+  PipeResult res[kMaxPipes];
+  auto run_pipe = [&](int h) {
+    PipeResult &out = res[h];
+    hipStream_t s = r.pipe_stream(h);
+    if ((out.err = hipSetDevice(r.device)) != hipSuccess) {
+      out.what = "hipSetDevice";
+      return;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      for (uint32_t k = 0; k < iters_per_sync && e == hipSuccess; ++k)
+        e = enqueue_iteration(r, w.p[h], h, batched, trace_blocks, false);
+      hipError_t e2 = hipStreamEndCapture(s, &graph);
+      if (e == hipSuccess && e2 == hipSuccess)
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph)
         (void)hipGraphDestroy(graph);
-      return set_error(PT_ERR_HIP, std::string("graph capture failed: ") +
-                                       hipGetErrorString(e != hipSuccess ? e : e2));
+      if (e != hipSuccess || e2 != hipSuccess) {
+        out.err = e != hipSuccess ? e : e2;
+        out.what = "graph capture";
+        if (exec)
+          (void)hipGraphExecDestroy(exec);
+        return;
+      }
     }
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess)
-      return set_error(PT_ERR_HIP, std::string("hipGraphInstantiate: ") +
-                                       hipGetErrorString(e));
+    for (;;) {
+      if (use_graph) {
+        out.err = hipGraphLaunch(exec, s);
+        out.launches += 2 * iters_per_sync;
+      } else {
+        for (uint32_t k = 0; k < iters_per_sync && out.err == hipSuccess; ++k) {
+          out.err = enqueue_iteration(r, w.p[h], h, batched, trace_blocks, prof);
+          out.launches += 2;
+        }
+      }
+      hipError_t e = hipStreamSynchronize(s);
+      if (out.err == hipSuccess)
+        out.err = e;
+      if (out.err != hipSuccess) {
+        out.what = "launch / sync";
+        break;
+      }
+      // counts of the last iteration's queues (pinned host copy in the stream)
+      const uint32_t *c = r.wf_host_counts + size_t(h) * gpu::kNumCounters;
+      const uint32_t ne = c[gpu::kExtCount], ns = c[gpu::kShadowCount];
+      if (prof && ne + ns) {
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, r.ev_prof[h][0], r.ev_prof[h][1]);
+        (void)hipEventElapsedTime(&b, r.ev_prof[h][1], r.ev_prof[h][2]);
+        out.ext_ms += a;
+        out.shadow_ms += b;
+      }
+      // a slot holds a path iff it queued at least one ray
+      if (ne + ns == 0)
+        break;
+    }
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+  };
+  {
+    std::thread second;
+    if (w.n > 1)
+      second = std::thread(run_pipe, 1);
+    run_pipe(0);
+    if (second.joinable())
+      second.join();
   }
   uint64_t launches = 0;
-  int rc = PT_OK;
-  for (;;) {
-    if (use_graph) {
-      hipError_t e = hipGraphLaunch(exec, s);
-      if (e != hipSuccess) {
-        rc = set_error(PT_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-        break;
-      }
-      launches += 2 * iters_per_sync;
-    } else {
-      for (uint32_t k = 0; k < iters_per_sync; ++k) {
-        hipError_t e = enqueue_iteration(r, p, batched, trace_blocks, prof);
-        if (e != hipSuccess) {
-          rc = set_error(PT_ERR_HIP, hipGetErrorString(e));
-          break;
-        }
-        launches += 2;
-      }
-      if (rc != PT_OK)
-        break;
-    }
-    hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-      rc = set_error(PT_ERR_HIP, std::string("sync: ") + hipGetErrorString(e));
-      break;
-    }
-    // counts of the last iteration's queues (pinned host copy in the stream)
-    uint32_t ne = r.wf_host_counts[gpu::kExtCount];
-    uint32_t ns = r.wf_host_counts[gpu::kShadowCount];
-    if (prof && ne + ns) {
-      float a = 0.f, b = 0.f;
-      (void)hipEventElapsedTime(&a, r.ev_prof[0], r.ev_prof[1]);
-      (void)hipEventElapsedTime(&b, r.ev_prof[1], r.ev_prof[2]);
-      r.stats.ext_ms += a;
-      r.stats.shadow_ms += b;
-    }
-    // a slot holds a path iff it queued at least one ray
-    if (ne + ns == 0)
-      break;
+  for (int h = 0; h < w.n; ++h) {
+    if (res[h].err != hipSuccess)
+      return set_error(PT_ERR_HIP, "wavefront pipeline " + std::to_string(h) + " " +
+                                       res[h].what + ": " + hipGetErrorString(res[h].err));
+    launches += res[h].launches;
+    r.stats.ext_ms += res[h].ext_ms;
+    r.stats.shadow_ms += res[h].shadow_ms;
   }
-  if (exec)
-    (void)hipGraphExecDestroy(exec);
-  if (rc != PT_OK)
-    return rc;
-  // the last iteration queued nothing, so the totals are complete
-  uint32_t c[gpu::kNumCounters];
-  HIP_TRY(hipMemcpyAsync(c, p.counters, sizeof(c), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  uint64_t ext = uint64_t(c[gpu::kExtTotal]) | (uint64_t(c[gpu::kExtTotal + 1]) << 32);
-  uint64_t shadow =
-      uint64_t(c[gpu::kShadowTotal]) | (uint64_t(c[gpu::kShadowTotal + 1]) << 32);
+  // every pipeline's last iteration queued nothing, so the totals are complete
+  uint32_t c[kMaxPipes * gpu::kNumCounters];
+  HIP_TRY(hipMemcpyAsync(c, r.wf_counters.ptr, sizeof(uint32_t) * w.n * gpu::kNumCounters,
+                         hipMemcpyDeviceToHost, s0));
+  HIP_TRY(hipStreamSynchronize(s0));
+  uint64_t ext = 0, shadow = 0;
+  uint32_t busy_iters = 0;
+  for (int h = 0; h < w.n; ++h) {
+    const uint32_t *ch = c + size_t(h) * gpu::kNumCounters;
+    ext += uint64_t(ch[gpu::kExtTotal]) | (uint64_t(ch[gpu::kExtTotal + 1]) << 32);
+    shadow += uint64_t(ch[gpu::kShadowTotal]) | (uint64_t(ch[gpu::kShadowTotal + 1]) << 32);
+    busy_iters += ch[gpu::kBusyIters];
+  }
   r.stats.ext_rays = ext;
   r.stats.shadow_rays = shadow;
-  r.stats.ext_launches = c[gpu::kBusyIters];
-  r.stats.shadow_launches = c[gpu::kBusyIters];
+  // launches of each traversal kernel over all pipelines
+  r.stats.ext_launches = busy_iters;
+  r.stats.shadow_launches = busy_iters;
   r.stats.trace_launches = launches;
-  return rc;
+  return PT_OK;
 }
 
 int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
@@ -215,11 +284,12 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
   std::memcpy(p.film_to_world, spec.film_to_world, sizeof(p.film_to_world));
   p.x_dim = x_dim;
   p.y_dim = y_dim;
-  int rc = wavefront_alloc(r, p, uint32_t(slots64), total, false, trace_blocks);
+  WfPipes w;
+  int rc = wavefront_alloc(r, w, p, uint32_t(slots64), total, false, trace_blocks);
   if (rc != PT_OK)
     return rc;
   HIP_TRY(r.rgb.ensure(size_t(wd.num_sample_blocks) * n_pixels * 3));
-  rc = wavefront_run(r, p, false, trace_blocks, r.iters_per_sync);
+  rc = wavefront_run(r, w, false, trace_blocks, r.iters_per_sync);
   if (rc != PT_OK)
     return rc;
   hipStream_t s = r.stream;

@@ -79,7 +79,7 @@ def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -
             "iters_per_sync": out[2], "graphs": bool(out[3])}
 
 
-def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 8,
+def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 6,
                       iters_per_sync: int = 1, use_graphs: bool = False) -> None:
     L.check(L.lib().pt_set_launch_params(renderer.handle, trace_blocks_per_cu,
                                          iters_per_sync, int(use_graphs)))
