@@ -90,10 +90,12 @@ def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
     s = _settings("Streaming")
     wf, _ = renderer.render(scene, ftw, size, size, spp, s)
     assert np.array_equal(mk.view(np.uint32), wf.view(np.uint32))
-    # a small path pool forces many regenerations per slot
-    s.max_num_samples_per_launch = 1000
-    wf2, _ = renderer.render(scene, ftw, size, size, spp, s)
-    assert np.array_equal(mk.view(np.uint32), wf2.view(np.uint32))
+    # small path pools force many regenerations per slot; below 512 slots the
+    # pool runs as one pipeline instead of two
+    for pool in (1000, 300):
+        s.max_num_samples_per_launch = pool
+        wf2, _ = renderer.render(scene, ftw, size, size, spp, s)
+        assert np.array_equal(mk.view(np.uint32), wf2.view(np.uint32)), pool
 
 
 def test_streaming_default_settings_multiplier_func(renderer, cornell):
