@@ -138,9 +138,33 @@ def bench_datagen(a, cfg):
     value = n * a.steps / elapsed
     total_px = sum(g.resolution ** 2 for g in gens)
 
+    # new scenes every batch: the trees must be built each time, on host
+    # threads (SAH, the default) or on the GPU (LBVH); generation excluded
+    from path_amd.datagen import last_bvh_build, set_bvh_builder
+    fresh = {}
+    for gpu_tree in (False, True, False, True):
+        batch = [generate_scene(seed) for seed in range(n)]
+        set_bvh_builder(r, gpu_tree)
+        sync()
+        t0 = time.perf_counter()
+        imgs_f, _, _ = render_batch(r, batch, spp, settings)
+        sync()
+        secs = time.perf_counter() - t0
+        key = "gpu_lbvh" if gpu_tree else "host_sah"
+        fresh.setdefault(key, []).append(secs)
+        if gpu_tree:
+            tree_ms, tree_levels = last_bvh_build(r)
+            fresh["gpu_tree_ms"] = tree_ms
+            fresh["gpu_tree_levels"] = tree_levels
+            fresh["gpu_tree_images_differing"] = sum(
+                not np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                for x, y in zip(imgs_f, images))
+        del batch
+    set_bvh_builder(r, False)
+
     # per-scene launches: the same scenes, one render call each
     single = settings.copy()
-    set_launch_params(r, 8, 1, False)
+    set_launch_params(r, 6, 1, False)
     for g in gens[:4]:
         r.render(g.scene, g.film_to_world, g.resolution, g.resolution, spp, single)
     sync()
@@ -190,6 +214,14 @@ def bench_datagen(a, cfg):
         "autotuned": tuned,
         "host_seconds": {"generate": round(gen_s, 3),
                          "first_batch_incl_bvh_builds": round(first_s, 3)},
+        "new_scenes_every_batch": {
+            "host_sah_trees_scenes_per_s": round(n / min(fresh["host_sah"]), 2),
+            "gpu_lbvh_trees_scenes_per_s": round(n / min(fresh["gpu_lbvh"]), 2),
+            "gpu_tree_build_ms": round(fresh["gpu_tree_ms"], 2),
+            "gpu_tree_levels": fresh["gpu_tree_levels"],
+            "images_differing_from_host_trees": fresh["gpu_tree_images_differing"],
+            "note": "a fresh set of 1024 generated scenes per batch (generation excluded): "
+                    "render_batch time including tree builds"},
     }
 
     counts = None

@@ -288,6 +288,18 @@ int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
  * checks; use_graphs: replay those iterations as one captured hipGraph. */
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
                          uint32_t iters_per_sync, int use_graphs);
+/* Tree builder of pt_render_batch (no reference counterpart: the reference
+ * builds its SBVH on the host, src/intersect/accel/sbvh/detail/generator.cu).
+ * PT_BVH_HOST_SAH (default): each scene's SAH tree is built on host threads
+ * and cached with the scene. PT_BVH_GPU_LBVH: all trees of a batch are built
+ * on the GPU in one pass (Morton-sorted LBVH collapsed to the same 4-wide
+ * quantised nodes); a batch with a tree deeper than the traversal stack
+ * allows falls back to the host builder. Images do not depend on the tree. */
+enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1 };
+int pt_set_bvh_builder(pt_renderer *r, int builder);
+/* Wall time (ms, host clock around the device work) and levels of the 4-wide
+ * trees of the last GPU tree build. */
+int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels);
 /* Time candidate (path pool, traversal grid, iterations per sync) triples on
  * the given items, keep the fastest on the renderer; out[4] = path pool,
  * blocks per CU, iterations per sync, graphs used. */

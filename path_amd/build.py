@@ -53,6 +53,7 @@ SOURCES = [
     "gpu/adjoint.hip",
     "mlp/mlp_kernels.hip",
     "gpu/nrc.hip",
+    "gpu/lbvh.hip",
 ]
 
 

@@ -418,6 +418,21 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
   return PT_OK;
 }
 
+int pt_set_bvh_builder(pt_renderer *r, int builder) {
+  if (!r || (builder != PT_BVH_HOST_SAH && builder != PT_BVH_GPU_LBVH))
+    return fail(PT_ERR_INVALID, "unknown BVH builder");
+  r->gpu_bvh = builder == PT_BVH_GPU_LBVH;
+  return PT_OK;
+}
+
+int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels) {
+  if (!r || !ms || !levels)
+    return fail(PT_ERR_INVALID, "null argument");
+  *ms = r->last_gpu_bvh_ms;
+  *levels = r->last_gpu_bvh_levels;
+  return PT_OK;
+}
+
 int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                       uint32_t samples_per, pt_settings *settings,
                       uint32_t out[4]) {

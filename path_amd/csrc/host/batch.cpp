@@ -49,13 +49,18 @@ template <class F> void parallel_for(size_t n, F f) {
 }
 
 // Flatten, build and pack the BVHs of the distinct scenes in parallel.
-void build_all(const pt_batch_item *items, uint32_t n) {
+void build_all(const pt_batch_item *items, uint32_t n, bool gpu_bvh) {
   std::vector<const pt_scene *> uniq;
   for (uint32_t i = 0; i < n; ++i)
     uniq.push_back(items[i].scene);
   std::sort(uniq.begin(), uniq.end());
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-  parallel_for(uniq.size(), [&](size_t k) { uniq[k]->packed(); });
+  parallel_for(uniq.size(), [&](size_t k) {
+    if (gpu_bvh)
+      uniq[k]->shading();
+    else
+      uniq[k]->packed();
+  });
 }
 
 // Byte layout of the concatenated scene arrays in the pinned staging buffer
@@ -68,19 +73,23 @@ struct Layout {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
+// gpu_bvh: the trees are built on the device after the upload; scene i's
+// root takes node slot (its rank among the scenes with triangles) and its
+// leaf triangles sit at its shading offset.
+int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, bool gpu_bvh,
          std::vector<gpu::SceneDesc> *desc, Layout *lay, uint64_t *n_pixels,
          uint64_t *total_samples) {
   desc->resize(n);
   std::vector<unsigned long long> begin(n + 1, 0);
   size_t c[Layout::kN] = {};
   uint64_t px_total = 0;
+  uint32_t roots = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const pt_batch_item &it = items[i];
     const pt_scene &sc = *it.scene;
     const FlatScene &f = sc.flat();
-    const PackedScene &pk = sc.packed();
-    if (pk.bvh.stack_need > uint32_t(gpu_stack_size()))
+    const PackedScene &pk = gpu_bvh ? sc.shading() : sc.packed();
+    if (!gpu_bvh && pk.bvh.stack_need > uint32_t(gpu_stack_size()))
       return set_error(PT_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
                                                ": BVH too deep for the traversal stack");
     gpu::SceneDesc d{};
@@ -94,10 +103,18 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
     d.y_dim = it.y_dim;
     d.pixel_off = uint32_t(px_total);
     d.has_nodes = pk.bvh.nodes.empty() ? 0u : 1u;
+    if (gpu_bvh) {
+      d.has_nodes = f.tris.empty() ? 0u : 1u;
+      d.node_off = roots;
+      d.tri_off = d.shade_off;
+      roots += d.has_nodes;
+    }
     std::memcpy(d.film_to_world, it.film_to_world, sizeof(d.film_to_world));
     (*desc)[i] = d;
-    c[Layout::kNodes] += pk.bvh.nodes.size();
-    c[Layout::kTris] += pk.bvh.tris.size();
+    if (!gpu_bvh) {
+      c[Layout::kNodes] += pk.bvh.nodes.size();
+      c[Layout::kTris] += pk.bvh.tris.size();
+    }
     c[Layout::kShade] += pk.shade.size();
     c[Layout::kMats] += pk.mats.size();
     c[Layout::kLightTris] += f.light_tris.size();
@@ -129,13 +146,16 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
   parallel_for(n, [&](size_t i) {
     const gpu::SceneDesc &d = (*desc)[i];
     const FlatScene &f = items[i].scene->flat();
-    const PackedScene &pk = items[i].scene->packed();
+    const PackedScene &pk = gpu_bvh ? items[i].scene->shading() : items[i].scene->packed();
     auto put = [&](int sec, size_t first, const void *src, size_t bytes) {
       if (bytes)
         std::memcpy(base + lay->off[sec] + first * elem[sec], src, bytes);
     };
-    put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(), pk.bvh.nodes.size() * sizeof(GpuNode));
-    put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
+    if (!gpu_bvh) {
+      put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(),
+          pk.bvh.nodes.size() * sizeof(GpuNode));
+      put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
+    }
     put(Layout::kShade, d.shade_off, pk.shade.data(), pk.shade.size() * sizeof(ShadeTri<float>));
     put(Layout::kMats, d.mat_off, pk.mats.data(), pk.mats.size() * sizeof(Material<float>));
     put(Layout::kLightTris, d.light_off, f.light_tris.data(),
@@ -144,6 +164,94 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
   });
   *n_pixels = px_total;
   *total_samples = begin[n];
+  return PT_OK;
+}
+
+// Trees of every scene of the batch on the GPU (LBVH, gpu/lbvh.h). Returns
+// PT_OK with *fallback set if some tree is too deep for the traversal stack
+// (or fails the quantisation check): the caller then builds on the host.
+int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
+                    const std::vector<gpu::SceneDesc> &desc, hipStream_t s, bool *fallback,
+                    uint32_t *levels) {
+  *fallback = false;
+  std::vector<uint32_t> tri_begin(n + 1, 0), int_begin(n + 1, 0);
+  std::vector<gpu::LbvhRoot> roots;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t nt = uint32_t(items[i].scene->flat().tris.size());
+    tri_begin[i + 1] = tri_begin[i] + nt;
+    int_begin[i + 1] = int_begin[i] + (nt >= 2 ? nt - 1 : 0);
+    if (desc[i].has_nodes != (nt > 0 ? 1u : 0u) || (nt && desc[i].tri_off != tri_begin[i]))
+      return set_error(PT_ERR_INVALID, "batch layout does not match the GPU tree build");
+    if (nt)
+      roots.push_back({nt >= 2 ? int32_t(int_begin[i]) : ~int32_t(tri_begin[i]),
+                       desc[i].node_off, i, desc[i].node_off});
+  }
+  const uint32_t nt = tri_begin[n], ni = int_begin[n];
+  *levels = 0;
+  if (nt == 0)
+    return PT_OK;
+  const size_t one = 1;
+  HIP_TRY(r.lb_tri_begin.upload(tri_begin.data(), n + 1, s));
+  HIP_TRY(r.lb_int_begin.upload(int_begin.data(), n + 1, s));
+  HIP_TRY(r.lb_scene_box.ensure(size_t(n) * 8));
+  HIP_TRY(r.lb_keys.ensure(nt));
+  HIP_TRY(r.lb_keys_sorted.ensure(nt));
+  HIP_TRY(r.lb_vals.ensure(nt));
+  HIP_TRY(r.lb_vals_sorted.ensure(nt));
+  HIP_TRY(r.lb_left.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_right.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_first.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_last.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_parent_int.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_visits.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_ibox.ensure(std::max(one, size_t(ni)) * 6));
+  HIP_TRY(r.lb_parent_leaf.ensure(nt));
+  HIP_TRY(r.lb_lbox.ensure(size_t(nt) * 6));
+  HIP_TRY(r.lb_counters.ensure(4));
+  // every 4-wide node is a root or an inner binary node
+  const size_t max_nodes = size_t(ni) + n;
+  HIP_TRY(r.lb_frontier[0].ensure(max_nodes));
+  HIP_TRY(r.lb_frontier[1].ensure(max_nodes));
+  HIP_TRY(r.b_nodes.ensure(max_nodes));
+  HIP_TRY(r.b_tris.ensure(nt));
+  size_t temp = 0;
+  HIP_TRY(gpu::lbvh_sort_temp_bytes(nt, n, &temp));
+  HIP_TRY(r.lb_sort_temp.ensure(std::max(one, temp)));
+  gpu::LbvhBuffers b{};
+  gpu::LbvhArgs &a = b.a;
+  a.shade = r.b_shade.ptr;
+  a.tri_begin = r.lb_tri_begin.ptr;
+  a.int_begin = r.lb_int_begin.ptr;
+  a.n_scenes = n;
+  a.n_tris = nt;
+  a.n_internal = ni;
+  a.scene_box = r.lb_scene_box.ptr;
+  a.keys = r.lb_keys.ptr;
+  a.keys_sorted = r.lb_keys_sorted.ptr;
+  a.vals = r.lb_vals.ptr;
+  a.vals_sorted = r.lb_vals_sorted.ptr;
+  a.left = r.lb_left.ptr;
+  a.right = r.lb_right.ptr;
+  a.first = r.lb_first.ptr;
+  a.last = r.lb_last.ptr;
+  a.parent_int = r.lb_parent_int.ptr;
+  a.parent_leaf = r.lb_parent_leaf.ptr;
+  a.ibox = r.lb_ibox.ptr;
+  a.lbox = r.lb_lbox.ptr;
+  a.visits = r.lb_visits.ptr;
+  a.nodes = r.b_nodes.ptr;
+  a.tris = r.b_tris.ptr;
+  a.counters = r.lb_counters.ptr;
+  b.frontier[0] = r.lb_frontier[0].ptr;
+  b.frontier[1] = r.lb_frontier[1].ptr;
+  b.sort_temp = r.lb_sort_temp.ptr;
+  b.sort_temp_bytes = temp;
+  const hipError_t e = gpu::build_lbvh(b, roots.data(), uint32_t(roots.size()), s, levels);
+  if (e == hipErrorInvalidValue) {
+    *fallback = true;
+    return PT_OK;
+  }
+  HIP_TRY(e);
   return PT_OK;
 }
 
@@ -164,11 +272,12 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     if (!items[i].scene || items[i].x_dim == 0 || items[i].y_dim == 0 || !items[i].out_rgb)
       return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
                                            ": null scene/output or empty image");
-  build_all(items, n);
+  const bool gpu_bvh = r.gpu_bvh;
+  build_all(items, n, gpu_bvh);
   std::vector<gpu::SceneDesc> desc;
   Layout lay;
   uint64_t n_pixels = 0, total = 0;
-  int rc = pack(r, items, n, spp, &desc, &lay, &n_pixels, &total);
+  int rc = pack(r, items, n, spp, gpu_bvh, &desc, &lay, &n_pixels, &total);
   if (rc != PT_OK)
     return rc;
   hipStream_t s = r.stream;
@@ -186,6 +295,25 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     HIP_TRY(up(r.b_light_cdf, Layout::kLightCdf));
     HIP_TRY(up(r.b_desc, Layout::kDesc));
     HIP_TRY(up(r.b_sample_begin, Layout::kBegin));
+  }
+  if (gpu_bvh) {
+    bool fallback = false;
+    uint32_t levels = 0;
+    const auto tb = std::chrono::steady_clock::now();
+    rc = build_gpu_trees(r, items, n, desc, s, &fallback, &levels);
+    if (rc != PT_OK)
+      return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    if (fallback) {
+      // a tree too deep for the traversal stack: build this batch on the host
+      r.gpu_bvh = false;
+      rc = render_batch(r, items, n, spp, settings, seconds);
+      r.gpu_bvh = true;
+      return rc;
+    }
+    r.last_gpu_bvh_ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - tb).count();
+    r.last_gpu_bvh_levels = levels;
   }
 
   gpu::RngParams rng{};

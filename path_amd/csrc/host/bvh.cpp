@@ -240,7 +240,6 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
     // child boxes (padded), then quantised to the node's grid
     float clo[4][3], chi[4][3];
     int32_t child[4];
-    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t k = 0; k < nk; ++k) {
       const BvhNode &n = bvh.nodes[kids[k]];
       const float lo[3] = {n.lo.x - pad, n.lo.y - pad, n.lo.z - pad};
@@ -248,8 +247,6 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
       for (int a = 0; a < 3; ++a) {
         clo[k][a] = lo[a];
         chi[k][a] = hi[a];
-        nlo[a] = std::min(nlo[a], lo[a]);
-        nhi[a] = std::max(nhi[a], hi[a]);
       }
       if (n.is_leaf) {
         child[k] = ~int32_t(n.a);
@@ -260,47 +257,9 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
         todo.push_back({kids[k], slot, it.depth + 1});
       }
     }
-    GpuNode o{};
-    uint8_t *qlo[3] = {o.lo_x, o.lo_y, o.lo_z}, *qhi[3] = {o.hi_x, o.hi_y, o.hi_z};
-    for (int a = 0; a < 3; ++a) {
-      // smallest power of two with 254 steps covering the extent (one step
-      // of slack so outward rounding never needs code 256)
-      const float extent = nhi[a] - nlo[a];
-      int e = -126;
-      if (extent > 0.f) {
-        int ex;
-        std::frexp(extent / 254.0f, &ex);
-        e = std::max(ex, -126);
-        while (std::ldexp(254.0f, e - 1) >= extent && e > -126)
-          --e;
-      }
-      const float scale = std::ldexp(1.0f, e);
-      o.origin[a] = nlo[a];
-      o.step[a] = scale;
-      for (uint32_t k = 0; k < 4; ++k) {
-        if (k >= nk) {
-          qlo[a][k] = 255;
-          qhi[a][k] = 0;
-          continue;
-        }
-        int ql = int(std::floor((clo[k][a] - nlo[a]) / scale));
-        int qh = int(std::ceil((chi[k][a] - nlo[a]) / scale));
-        ql = std::clamp(ql, 0, 255);
-        qh = std::clamp(qh, 0, 255);
-        // outward on the exact float decode
-        while (ql > 0 && std::fmaf(float(ql), scale, nlo[a]) > clo[k][a])
-          --ql;
-        while (qh < 255 && std::fmaf(float(qh), scale, nlo[a]) < chi[k][a])
-          ++qh;
-        if (std::fmaf(float(ql), scale, nlo[a]) > clo[k][a] ||
-            std::fmaf(float(qh), scale, nlo[a]) < chi[k][a])
-          throw std::runtime_error("BVH node quantisation is not conservative");
-        qlo[a][k] = uint8_t(ql);
-        qhi[a][k] = uint8_t(qh);
-      }
-    }
-    for (uint32_t k = 0; k < 4; ++k)
-      o.child[k] = k < nk ? child[k] : kEmptyChild;
+    GpuNode o;
+    if (!quantise_gpu_node(clo, chi, nk, child, &o))
+      throw std::runtime_error("BVH node quantisation is not conservative");
     g.nodes[size_t(it.slot)] = o;
     g.max_depth = std::max(g.max_depth, it.depth);
   }

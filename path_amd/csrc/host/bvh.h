@@ -3,6 +3,7 @@
 
 #include <cstddef>
 
+#include "../core/bvh_layout.h"
 #include "../core/geometry.h"
 
 #include <climits>
@@ -49,46 +50,10 @@ constexpr float kGpuSahTraversalCost = 0.5f;
 Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
                   const BvhSettings &settings = {});
 
-// GPU layout: a 4-wide BVH collapsed from the binary tree (each GPU node takes
-// the up-to-4 descendants obtained by repeatedly opening the child with the
-// largest surface area), 64 bytes per node with the four child boxes
-// quantised to 8 bits on a per-node grid:
-//   origin (x, y, z) and step (sx, sy, sz) floats, each step a power of two
-//     -> plane = fma(q, step, origin) per axis,
-//   child[4] (>= 0 inner node, < 0 ~(first triangle) of a leaf, kEmptyChild
-//     for an unused slot),
-//   q lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4] (uint8).
-// The grid codes round outward and are checked with the same float fma the
-// GPU evaluates, so each decoded box contains its child's padded box: boxes
-// only grow, and since the padded boxes are already conservative with
-// respect to the float triangle test, the closest hit still does not depend
-// on the tree.
-constexpr int32_t kEmptyChild = INT32_MIN;
-
-struct GpuNode {
-  float origin[3];
-  float step[3];
-  int32_t child[4];
-  uint8_t lo_x[4], hi_x[4], lo_y[4], hi_y[4];
-  uint8_t lo_z[4], hi_z[4];
-};
-static_assert(sizeof(GpuNode) == 64, "GPU BVH node is one 64-byte record");
-// gpu/trace.h reads the node as four float4: origin + step x, steps y z +
-// child 0 1, child 2 3 + lo_x hi_x, lo_y hi_y lo_z hi_z
-static_assert(offsetof(GpuNode, step) == 12 && offsetof(GpuNode, child) == 24 &&
-                  offsetof(GpuNode, lo_x) == 40 && offsetof(GpuNode, lo_y) == 48 &&
-                  offsetof(GpuNode, lo_z) == 56 && offsetof(GpuNode, hi_z) == 60,
-              "GpuNode layout read by the traversal");
-
-// Triangle in leaf order: (v0, idx bits), (e1, 0), (e2, 0).
-struct GpuTri {
-  float v0[3];
-  uint32_t idx;
-  float e1[3];
-  uint32_t pad0;
-  float e2[3];
-  uint32_t pad1;
-};
+// GPU layout (GpuNode, GpuTri, quantise_gpu_node): core/bvh_layout.h.
+using ::pt::GpuNode;
+using ::pt::GpuTri;
+using ::pt::kEmptyChild;
 
 struct GpuBvh {
   std::vector<GpuNode> nodes;

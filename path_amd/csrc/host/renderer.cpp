@@ -43,13 +43,29 @@ const Bvh &pt_scene::bvh() const {
   return bvh_;
 }
 
-const PackedScene &pt_scene::packed() const {
+const PackedScene &pt_scene::shading() const {
   const FlatScene &f = flat();
-  if (packed_gen_ != generation) {
-    packed_.bvh = pack_gpu_bvh(bvh(), f.tris);
+  if (shade_gen_ != generation) {
     packed_.shade.resize(f.tris.size());
     for (size_t i = 0; i < f.tris.size(); ++i)
       packed_.shade[i] = make_shade_tri(f.tris[i], &f.normals[3 * i], f.tri_mat[i]);
+    shade_gen_ = generation;
+  }
+  if (packed_mat_gen_ != mat_generation || packed_mats_of_gen_ != generation) {
+    packed_.mats.clear();
+    for (const auto &m : f.materials)
+      packed_.mats.push_back(m.to<float>());
+    packed_mat_gen_ = mat_generation;
+    packed_mats_of_gen_ = generation;
+  }
+  return packed_;
+}
+
+const PackedScene &pt_scene::packed() const {
+  const FlatScene &f = flat();
+  shading();
+  if (packed_gen_ != generation) {
+    packed_.bvh = pack_gpu_bvh(bvh(), f.tris);
     packed_gen_ = generation;
   }
   if (packed_mat_gen_ != mat_generation || packed_mats_of_gen_ != generation) {

@@ -3,6 +3,7 @@
 
 #include "../../../include/path_amd.h"
 #include "../gpu/kernels.h"
+#include "../gpu/lbvh.h"
 #include "../gpu/wavefront.h"
 #include "bvh.h"
 #include "scene.h"
@@ -50,10 +51,13 @@ struct pt_scene {
   const pt::host::FlatScene &flat() const;
   const pt::host::Bvh &bvh() const;
   const pt::host::PackedScene &packed() const;
+  // Shading records and materials only (the `bvh` member of the result is
+  // filled only once packed() ran): batches whose trees are built on the GPU.
+  const pt::host::PackedScene &shading() const;
 
 private:
   mutable uint64_t flat_gen_ = ~0ull, flat_mat_gen_ = ~0ull, bvh_gen_ = ~0ull,
-                   packed_gen_ = ~0ull, packed_mat_gen_ = ~0ull,
+                   shade_gen_ = ~0ull, packed_gen_ = ~0ull, packed_mat_gen_ = ~0ull,
                    packed_mats_of_gen_ = ~0ull;
   mutable pt::host::FlatScene flat_;
   mutable pt::host::Bvh bvh_;
@@ -211,6 +215,17 @@ struct pt_renderer {
   uint32_t trace_blocks_per_cu = 6;
   uint32_t iters_per_sync = 1;
   bool use_graphs = false;
+  bool gpu_bvh = false; // batched renders build their trees on the GPU (LBVH)
+  double last_gpu_bvh_ms = 0.0;     // host wall time of the last GPU tree build
+  uint32_t last_gpu_bvh_levels = 0; // levels of its deepest 4-wide tree
+  // GPU tree build scratch
+  pt::host::DeviceBuf<float> lb_scene_box, lb_ibox, lb_lbox;
+  pt::host::DeviceBuf<unsigned long long> lb_keys, lb_keys_sorted;
+  pt::host::DeviceBuf<uint32_t> lb_vals, lb_vals_sorted, lb_first, lb_last, lb_visits,
+      lb_counters, lb_tri_begin, lb_int_begin;
+  pt::host::DeviceBuf<int32_t> lb_left, lb_right, lb_parent_int, lb_parent_leaf;
+  pt::host::DeviceBuf<pt::gpu::LbvhRoot> lb_frontier[2];
+  pt::host::DeviceBuf<char> lb_sort_temp;
 
   ~pt_renderer();
   bool init(int device, std::string *err);

@@ -85,6 +85,19 @@ def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 6,
                                          iters_per_sync, int(use_graphs)))
 
 
+def set_bvh_builder(renderer: Renderer, gpu: bool) -> None:
+    """Trees of render_batch: GPU LBVH build of the whole batch (True) or the
+    host SAH builder, cached per scene (False, the default)."""
+    L.check(L.lib().pt_set_bvh_builder(renderer.handle, 1 if gpu else 0))
+
+
+def last_bvh_build(renderer: Renderer) -> tuple:
+    """(milliseconds, levels of the deepest 4-wide tree) of the last GPU build."""
+    ms, lv = C.c_double(), C.c_uint32()
+    L.check(L.lib().pt_last_bvh_build(renderer.handle, C.byref(ms), C.byref(lv)))
+    return float(ms.value), int(lv.value)
+
+
 def datagen_settings() -> Settings:
     """Streaming integrator, Sobol sampling, BSDF directions, NEE."""
     s = Settings()

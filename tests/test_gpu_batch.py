@@ -52,6 +52,27 @@ def test_batch_small_pool_and_graphs_bit_exact(renderer, gen_scenes):
     set_launch_params(renderer, 8, 1, False)
 
 
+def test_gpu_tree_build_bit_exact(renderer, gen_scenes):
+    """Trees built on the GPU (LBVH) render the same images and G-buffers as
+    the host SAH trees: closest hits do not depend on the tree."""
+    from path_amd.datagen import (datagen_settings, generate_scene, last_bvh_build,
+                                  render_batch, set_bvh_builder, set_launch_params)
+    set_launch_params(renderer, 6, 1, False)
+    st = datagen_settings()
+    scenes = gen_scenes + [generate_scene(s) for s in (40, 41, 42)]
+    host, host_g, _ = render_batch(renderer, scenes, SPP, st)
+    set_bvh_builder(renderer, True)
+    try:
+        gpu, gpu_g, _ = render_batch(renderer, scenes, SPP, st)
+        ms, levels = last_bvh_build(renderer)
+    finally:
+        set_bvh_builder(renderer, False)
+    assert 1 <= levels <= 21 and ms > 0
+    for a, b, ga, gb in zip(host, gpu, host_g, gpu_g):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+
+
 def test_gbuffer_first_hit(renderer, gen_scenes):
     """The G-buffer position is the closest hit along the camera ray through
     it, its normal is the hit triangle's (unit, facing the camera side), and
