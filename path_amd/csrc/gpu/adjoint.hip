@@ -12,6 +12,13 @@
 // Nothing is stored per vertex. Gradients accumulate in per-thread LDS slots
 // (no atomics), are summed per block in thread order and finally over blocks
 // in FP64, so the result is deterministic.
+//
+// Single pass (paths of at most kAdjointMaxVertices vertices, e.g. NIters):
+// dI (L - R_k) / albedo = dI L / albedo - dI R_k / albedo, so one replay adds
+// the second part at each vertex (R_k is known there), keeps the vertex's
+// material in LDS, and adds the first part for the kept materials once L is
+// known at the end of the path. A path longer than the list sets `overflow`
+// and the host reruns the two-replay kernel.
 #include "kernels.h"
 
 #include "../core/camera.h"
@@ -32,13 +39,19 @@ __device__ __forceinline__ R adjoint_rng(const RngParams &rp, uint32_t location,
     return make_hash_rng(sample + location * rp.samples_per);
 }
 
-// One replay of a sample. kBackprop = false: returns the radiance. kBackprop =
-// true: accumulates into acc[param * stride] (this thread's slots).
-template <class R, bool kBackprop>
+enum ReplayMode { kRadiance, kBackprop, kSinglePass };
+
+// One replay of a sample. kRadiance: returns the radiance. kBackprop:
+// accumulates into acc[param * stride] (this thread's slots) given the
+// sample's radiance. kSinglePass: accumulates without knowing the radiance
+// in advance; vlist (this thread's LDS column, stride kAdjointBlock) keeps
+// the vertex materials.
+template <class R, int kMode>
 __device__ __forceinline__ V3<float>
 replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
               V3<float> radiance, V3<float> d_i, float *acc, uint32_t stride,
-              LdsStack &stack) {
+              LdsStack &stack, uint8_t *vlist = nullptr) {
+  uint32_t n_vertices = 0;
   const uint32_t location = px + py * p.x_dim;
   R rng = adjoint_rng<R>(p.rng, location, sample);
   float xo = rng.next();
@@ -62,11 +75,11 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
     Ray<float> nl = light_ray, nn = next_ray;
     IterTape<float> tape;
     VertexInfo<float> *no_vertex = nullptr;
-    IterTape<float> *tape_ptr = kBackprop ? &tape : nullptr;
+    IterTape<float> *tape_ptr = kMode != kRadiance ? &tape : nullptr;
     bool done = rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
                                              p.settings, p.scene.view, &ns, &nl, &nn,
                                              no_vertex, tape_ptr);
-    if constexpr (kBackprop) {
+    if constexpr (kMode != kRadiance) {
       for (uint32_t j = 0; j < tape.n_emit; ++j) {
         float *e = acc + (n3 + 3 * tape.emit_mat[j]) * stride;
         e[0] += d_i.x * tape.emit_mult[j].x;
@@ -75,27 +88,58 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
       }
       if (tape.vertex_mat != kNoMaterial) {
         V3<float> rho = material_albedo(p.scene.view.materials[tape.vertex_mat]);
-        V3<float> rest = radiance - ns.total;
         float *a = acc + 3 * tape.vertex_mat * stride;
-        if (rho.x != 0.f)
-          a[0] += d_i.x * rest.x / rho.x;
-        if (rho.y != 0.f)
-          a[stride] += d_i.y * rest.y / rho.y;
-        if (rho.z != 0.f)
-          a[2 * stride] += d_i.z * rest.z / rho.z;
+        if constexpr (kMode == kBackprop) {
+          V3<float> rest = radiance - ns.total;
+          if (rho.x != 0.f)
+            a[0] += d_i.x * rest.x / rho.x;
+          if (rho.y != 0.f)
+            a[stride] += d_i.y * rest.y / rho.y;
+          if (rho.z != 0.f)
+            a[2 * stride] += d_i.z * rest.z / rho.z;
+        } else {
+          if (rho.x != 0.f)
+            a[0] -= d_i.x * ns.total.x / rho.x;
+          if (rho.y != 0.f)
+            a[stride] -= d_i.y * ns.total.y / rho.y;
+          if (rho.z != 0.f)
+            a[2 * stride] -= d_i.z * ns.total.z / rho.z;
+          if (n_vertices < kAdjointMaxVertices)
+            vlist[n_vertices * kAdjointBlock] = uint8_t(tape.vertex_mat);
+          else
+            *p.overflow = 1u;
+          ++n_vertices;
+        }
       }
     }
-    if (done)
+    if (done) {
+      if constexpr (kMode == kSinglePass) {
+        const V3<float> l = ns.total;
+        const uint32_t nv = n_vertices < kAdjointMaxVertices ? n_vertices : kAdjointMaxVertices;
+        for (uint32_t v = 0; v < nv; ++v) {
+          const uint32_t m = vlist[v * kAdjointBlock];
+          const V3<float> rho = material_albedo(p.scene.view.materials[m]);
+          float *a = acc + 3 * m * stride;
+          if (rho.x != 0.f)
+            a[0] += d_i.x * l.x / rho.x;
+          if (rho.y != 0.f)
+            a[stride] += d_i.y * l.y / rho.y;
+          if (rho.z != 0.f)
+            a[2 * stride] += d_i.z * l.z / rho.z;
+        }
+      }
       return ns.total;
+    }
     st = ns;
     light_ray = nl;
     next_ray = nn;
   }
 }
 
-template <class R>
+template <class R, bool kSingle>
 __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
   extern __shared__ float acc_all[];
+  __shared__ uint8_t vlist_all[kSingle ? kAdjointMaxVertices * kAdjointBlock : 1];
   __shared__ int stack_lds[kLdsStackInts * (kAdjointBlock / 64)];
   int spill[kOverflowDepth];
   LdsStack stack{lds_stack_column(stack_lds), spill, 1};
@@ -116,9 +160,14 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
                            p.d_image[3 * size_t(pixel) + 2]};
     for (uint32_t sample = s0; sample < s1; ++sample) {
       V3<float> zero = {0.f, 0.f, 0.f};
-      V3<float> radiance =
-          replay_sample<R, false>(p, px, py, sample, zero, zero, nullptr, stride, stack);
-      replay_sample<R, true>(p, px, py, sample, radiance, d_i, acc, stride, stack);
+      if constexpr (kSingle) {
+        replay_sample<R, kSinglePass>(p, px, py, sample, zero, d_i, acc, stride, stack,
+                                      vlist_all + threadIdx.x);
+      } else {
+        V3<float> radiance =
+            replay_sample<R, kRadiance>(p, px, py, sample, zero, zero, nullptr, stride, stack);
+        replay_sample<R, kBackprop>(p, px, py, sample, radiance, d_i, acc, stride, stack);
+      }
     }
   }
   __syncthreads();
@@ -134,10 +183,20 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s) {
   const uint32_t blocks = (p.n_threads + kAdjointBlock - 1) / kAdjointBlock;
   const size_t lds = size_t(6) * p.n_mats * (kAdjointBlock + 1) * sizeof(float);
-  if (p.rng.type == kRngSobel)
-    hipLaunchKernelGGL(k_adjoint<TableRng>, dim3(blocks), dim3(kAdjointBlock), lds, s, p);
-  else
-    hipLaunchKernelGGL(k_adjoint<HashRng>, dim3(blocks), dim3(kAdjointBlock), lds, s, p);
+  const bool sobol = p.rng.type == kRngSobel;
+  if (p.single_pass) {
+    if (sobol)
+      hipLaunchKernelGGL((k_adjoint<TableRng, true>), dim3(blocks), dim3(kAdjointBlock), lds, s, p);
+    else
+      hipLaunchKernelGGL((k_adjoint<HashRng, true>), dim3(blocks), dim3(kAdjointBlock), lds, s, p);
+  } else {
+    if (sobol)
+      hipLaunchKernelGGL((k_adjoint<TableRng, false>), dim3(blocks), dim3(kAdjointBlock), lds, s,
+                         p);
+    else
+      hipLaunchKernelGGL((k_adjoint<HashRng, false>), dim3(blocks), dim3(kAdjointBlock), lds, s,
+                         p);
+  }
   return hipGetLastError();
 }
 

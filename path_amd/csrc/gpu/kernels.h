@@ -64,7 +64,11 @@ struct AdjointParams {
   uint32_t n_mats;
   const float *d_image;        // dLoss/dI, (x + y * x_dim) * 3 + c
   float *partials;             // [block][6 * n_mats]: albedo then emission
+  uint32_t single_pass;        // one replay per path (paths of at most
+                               // kAdjointMaxVertices vertices)
+  uint32_t *overflow;          // set when a single-pass path was longer
 };
+constexpr uint32_t kAdjointMaxVertices = 16;
 
 // Radiance-cache estimator: the path is traced for one bounce and the
 // radiance reflected at the second vertex comes from a learned model. Sample i

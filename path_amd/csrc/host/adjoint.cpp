@@ -92,16 +92,32 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   HIP_TRY(r.adj_out.ensure(n_params));
   p.d_image = r.adj_d_image.ptr;
   p.partials = r.adj_partials.ptr;
+  // one replay per path when the path length is bounded (NIters: at most
+  // n_iters + 1 vertices), else radiance replay + back-propagation replay
+  p.single_pass = settings.term_prob == PT_TERM_N_ITERS &&
+                          settings.n_iters + 1 <= gpu::kAdjointMaxVertices
+                      ? 1u
+                      : 0u;
+  HIP_TRY(r.adj_overflow.ensure(1));
+  p.overflow = r.adj_overflow.ptr;
   r.stats = pt_render_stats{};
-  HIP_TRY(hipEventRecord(r.ev_start, s));
-  HIP_TRY(gpu::launch_adjoint(p, s));
-  HIP_TRY(gpu::launch_adjoint_finalize(r.adj_partials.ptr, n_blocks, n_params,
-                                       1.0 / double(spp), r.adj_out.ptr, s));
-  HIP_TRY(hipEventRecord(r.ev_stop, s));
   std::vector<float> out(n_params);
-  HIP_TRY(hipMemcpyAsync(out.data(), r.adj_out.ptr, n_params * sizeof(float),
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  for (;;) {
+    uint32_t overflow = 0;
+    HIP_TRY(hipMemsetAsync(p.overflow, 0, sizeof(uint32_t), s));
+    HIP_TRY(hipEventRecord(r.ev_start, s));
+    HIP_TRY(gpu::launch_adjoint(p, s));
+    HIP_TRY(gpu::launch_adjoint_finalize(r.adj_partials.ptr, n_blocks, n_params,
+                                         1.0 / double(spp), r.adj_out.ptr, s));
+    HIP_TRY(hipEventRecord(r.ev_stop, s));
+    HIP_TRY(hipMemcpyAsync(out.data(), r.adj_out.ptr, n_params * sizeof(float),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&overflow, p.overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!overflow || !p.single_pass)
+      break;
+    p.single_pass = 0; // a path outran the vertex list: replay twice
+  }
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
   r.stats.kernel_ms = ms;

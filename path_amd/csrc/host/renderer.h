@@ -219,6 +219,7 @@ struct pt_renderer {
   double last_gpu_bvh_ms = 0.0;     // host wall time of the last GPU tree build
   uint32_t last_gpu_bvh_levels = 0; // levels of its deepest 4-wide tree
   // GPU tree build scratch
+  pt::host::DeviceBuf<uint32_t> adj_overflow;
   pt::host::DeviceBuf<float> lb_scene_box, lb_ibox, lb_lbox;
   pt::host::DeviceBuf<unsigned long long> lb_keys, lb_keys_sorted;
   pt::host::DeviceBuf<uint32_t> lb_vals, lb_vals_sorted, lb_first, lb_last, lb_visits,

@@ -11,11 +11,11 @@ X = Y = 256
 SPP = 32
 
 
-def _settings():
+def _settings(n_iters=4):
     from path_amd.renderer import Settings
     s = Settings()
     s.term_prob = "NIters"
-    s.n_iters = 4
+    s.n_iters = n_iters
     return s
 
 
@@ -39,11 +39,14 @@ def _target_and_start(dr):
     return target, alb, em, (alb0.numpy(), em0.numpy())
 
 
-def test_adjoint_matches_fp64_oracle(renderer, dbox):
+# 4 iterations: one replay per path (at most 16 vertices); 20: radiance replay
+# + back-propagation replay
+@pytest.mark.parametrize("n_iters", [4, 20])
+def test_adjoint_matches_fp64_oracle(renderer, dbox, n_iters):
     from oracle import oracle as O
     from path_amd.diff import DiffRenderer
     scene, ftw = dbox
-    st = _settings()
+    st = _settings(n_iters)
     dr = DiffRenderer(renderer, scene, ftw, X, Y, SPP, st)
     target, alb, em, orig = _target_and_start(dr)
     dr.set_parameters(alb, em)
