@@ -604,6 +604,8 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (scale for the "
                          "profiler's FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pipelines", type=int, default=0,
+                    help="wavefront pipelines per render (default 2)")
     ap.add_argument("--mlp-forward", action="store_true",
                     help="only time the radiance MLP's fused inference forward (2^20 queries, "
                          "--steps launches)")
@@ -641,6 +643,9 @@ def main(argv=None):
         settings.max_num_samples_per_launch = a.max_paths
     n_tris = scene.counts()[0]
     r = Renderer(0)
+    if a.pipelines:
+        from path_amd.datagen import set_pipelines
+        set_pipelines(r, a.pipelines)
     if a.trace_blocks_per_cu or a.iters_per_sync:
         from path_amd.datagen import set_launch_params
         set_launch_params(r, a.trace_blocks_per_cu or 6, a.iters_per_sync or 1,

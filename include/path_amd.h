@@ -296,6 +296,10 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
  * quantised nodes); a batch with a tree deeper than the traversal stack
  * allows falls back to the host builder. Images do not depend on the tree. */
 enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1 };
+/* Independent wavefront pipelines a Streaming render splits its path pool
+ * into (1..4, default 2; each at least 256 paths), each on its own stream and
+ * host thread. Results never depend on it. */
+int pt_set_pipelines(pt_renderer *r, uint32_t n);
 int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */

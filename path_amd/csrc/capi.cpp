@@ -418,6 +418,13 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
   return PT_OK;
 }
 
+int pt_set_pipelines(pt_renderer *r, uint32_t n) {
+  if (!r || n == 0 || n > uint32_t(kMaxPipes))
+    return fail(PT_ERR_INVALID, "pipelines must be in [1, " + std::to_string(kMaxPipes) + "]");
+  r->n_pipes = n;
+  return PT_OK;
+}
+
 int pt_set_bvh_builder(pt_renderer *r, int builder) {
   if (!r || (builder != PT_BVH_HOST_SAH && builder != PT_BVH_GPU_LBVH))
     return fail(PT_ERR_INVALID, "unknown BVH builder");

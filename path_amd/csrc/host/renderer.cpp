@@ -136,8 +136,9 @@ pt_renderer::~pt_renderer() {
         (void)hipEventDestroy(e);
   if (wf_host_counts)
     (void)hipHostFree(wf_host_counts);
-  if (stream_b)
-    (void)hipStreamDestroy(stream_b);
+  for (hipStream_t ps : pipe_streams)
+    if (ps)
+      (void)hipStreamDestroy(ps);
   if (stream)
     (void)hipStreamDestroy(stream);
 }
@@ -147,8 +148,8 @@ bool pt_renderer::init(int dev, std::string *err) {
   hipError_t e = hipSetDevice(dev);
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-  if (e == hipSuccess)
-    e = hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking);
+  for (int h = 0; h + 1 < kMaxPipes && e == hipSuccess; ++h)
+    e = hipStreamCreateWithFlags(&pipe_streams[h], hipStreamNonBlocking);
   if (e == hipSuccess)
     e = hipEventCreate(&ev_start);
   if (e == hipSuccess)

@@ -143,17 +143,18 @@ struct PinnedBuf {
 } // namespace pt
 
 // Independent wavefront pipelines of one render (each its own stream).
-constexpr int kMaxPipes = 2;
+constexpr int kMaxPipes = 4;
 
 struct pt_renderer {
   int device = 0;
   hipStream_t stream = nullptr;   // everything, and wavefront pipeline 0
-  hipStream_t stream_b = nullptr; // wavefront pipeline 1
+  hipStream_t pipe_streams[kMaxPipes - 1] = {}; // wavefront pipelines 1..
+  uint32_t n_pipes = 2; // wavefront pipelines of a render (1..kMaxPipes)
   hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_fork = nullptr;
   pt_render_stats stats{};
   bool profiling = false;
   hipEvent_t ev_prof[kMaxPipes][3] = {};
-  hipStream_t pipe_stream(int pipe) const { return pipe == 0 ? stream : stream_b; }
+  hipStream_t pipe_stream(int pipe) const { return pipe == 0 ? stream : pipe_streams[pipe - 1]; }
 
   // scene cache
   const pt_scene *cached_scene = nullptr;

@@ -40,10 +40,10 @@ int wavefront_alloc(pt_renderer &r, WfPipes &w, const gpu::WfParams &base,
   if (batched)
     HIP_TRY(r.wf_scene.ensure(n_slots));
   HIP_TRY(r.wf_samples.ensure(size_t(total_samples) * 3));
-  // Two pipelines (half the slots each, their own stream, queues and
-  // counters) once there is at least one shade block for each: a memory-bound
-  // shade step of one overlaps the ALU-bound traversal of the other.
-  w.n = n_slots >= 2 * 256 ? kMaxPipes : 1;
+  // r.n_pipes pipelines (an equal share of the slots each, their own stream,
+  // queues and counters) while each gets at least one shade block: the
+  // memory-bound shade step of one overlaps the ALU-bound traversal of another.
+  w.n = int(std::max(1u, std::min({r.n_pipes, uint32_t(kMaxPipes), n_slots / 256})));
   uint32_t first[kMaxPipes + 1] = {0};
   for (int h = 0; h < w.n; ++h) {
     const uint32_t left = n_slots - first[h], pipes_left = uint32_t(w.n - h);
@@ -147,9 +147,9 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
   if (prof)
     iters_per_sync = 1;
   const bool use_graph = r.use_graphs && !prof;
-  // Each pipeline runs its own host loop (pipeline 1 on a second host
-  // thread), so the pipelines drift apart and one's shade step overlaps the
-  // other's traversal instead of running in lockstep.
+  // Each pipeline runs its own host loop (pipelines 1.. on host threads of
+  // their own), so the pipelines drift apart and one's shade step overlaps
+  // another's traversal instead of running in lockstep.
   struct PipeResult {
     hipError_t err = hipSuccess;
     std::string what;
@@ -218,12 +218,12 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
       (void)hipGraphExecDestroy(exec);
   };
   {
-    std::thread second;
-    if (w.n > 1)
-      second = std::thread(run_pipe, 1);
+    std::thread others[kMaxPipes - 1];
+    for (int h = 1; h < w.n; ++h)
+      others[h - 1] = std::thread(run_pipe, h);
     run_pipe(0);
-    if (second.joinable())
-      second.join();
+    for (int h = 1; h < w.n; ++h)
+      others[h - 1].join();
   }
   uint64_t launches = 0;
   for (int h = 0; h < w.n; ++h) {

@@ -85,6 +85,11 @@ def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 6,
                                          iters_per_sync, int(use_graphs)))
 
 
+def set_pipelines(renderer: Renderer, n: int) -> None:
+    """Wavefront pipelines of a Streaming render (1-4, default 2)."""
+    L.check(L.lib().pt_set_pipelines(renderer.handle, n))
+
+
 def set_bvh_builder(renderer: Renderer, gpu: bool) -> None:
     """Trees of render_batch: GPU LBVH build of the whole batch (True) or the
     host SAH builder, cached per scene (False, the default)."""
