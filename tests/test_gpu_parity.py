@@ -96,6 +96,16 @@ def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
         s.max_num_samples_per_launch = pool
         wf2, _ = renderer.render(scene, ftw, size, size, spp, s)
         assert np.array_equal(mk.view(np.uint32), wf2.view(np.uint32)), pool
+    # any number of pipelines
+    from path_amd.datagen import set_pipelines
+    s.max_num_samples_per_launch = 5000
+    try:
+        for n in (1, 3, 4):
+            set_pipelines(renderer, n)
+            wf3, _ = renderer.render(scene, ftw, size, size, spp, s)
+            assert np.array_equal(mk.view(np.uint32), wf3.view(np.uint32)), n
+    finally:
+        set_pipelines(renderer, 2)
 
 
 def test_streaming_default_settings_multiplier_func(renderer, cornell):
