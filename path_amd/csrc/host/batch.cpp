@@ -102,9 +102,8 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
     d.x_dim = it.x_dim;
     d.y_dim = it.y_dim;
     d.pixel_off = uint32_t(px_total);
-    d.has_nodes = pk.bvh.nodes.empty() ? 0u : 1u;
+    d.has_nodes = f.tris.empty() ? 0u : 1u;
     if (gpu_bvh) {
-      d.has_nodes = f.tris.empty() ? 0u : 1u;
       d.node_off = roots;
       d.tri_off = d.shade_off;
       roots += d.has_nodes;

@@ -193,8 +193,21 @@ GpuBvh pack_gpu_bvh(const Bvh &bvh, const std::vector<TriGeom<float>> &tris) {
   for (const BvhNode &n : bvh.nodes)
     if (n.is_leaf && n.b > n.a)
       g.tris[n.b - 1].idx |= 0x80000000u;
-  if (bvh.nodes.empty())
-    return g; // no triangles: callers treat every ray as a miss
+  if (bvh.nodes.empty()) {
+    // No triangles: one root with four empty slots, so every traversal visits
+    // it, hits nothing and ends, without a special case in the kernels.
+    GpuNode o{};
+    for (int a = 0; a < 3; ++a)
+      o.step[a] = 1.0f;
+    for (int k = 0; k < 4; ++k) {
+      o.child[k] = kEmptyChild;
+      o.lo_x[k] = o.lo_y[k] = o.lo_z[k] = 255;
+      o.hi_x[k] = o.hi_y[k] = o.hi_z[k] = 0;
+    }
+    g.nodes.push_back(o);
+    g.stack_need = 3;
+    return g;
+  }
   const float pad = g.pad;
   auto area = [&](uint32_t i) {
     const BvhNode &n = bvh.nodes[i];

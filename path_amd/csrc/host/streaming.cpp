@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <thread>
 
 using namespace pt;
@@ -218,11 +219,19 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
       (void)hipGraphExecDestroy(exec);
   };
   {
+    // A pipeline whose thread cannot be started runs on this thread after
+    // pipeline 0 (the pipelines are independent, only slower that way).
     std::thread others[kMaxPipes - 1];
-    for (int h = 1; h < w.n; ++h)
-      others[h - 1] = std::thread(run_pipe, h);
+    int started = 1;
+    try {
+      for (; started < w.n; ++started)
+        others[started - 1] = std::thread(run_pipe, started);
+    } catch (const std::system_error &) {
+    }
     run_pipe(0);
-    for (int h = 1; h < w.n; ++h)
+    for (int h = started; h < w.n; ++h)
+      run_pipe(h);
+    for (int h = 1; h < started; ++h)
       others[h - 1].join();
   }
   uint64_t launches = 0;
