@@ -714,6 +714,45 @@ struct XmlLoader {
     return true;
   }
 
+  // A <matrix> of four child elements (attributes a-d or v1-v4): the k-th
+  // element fills column k (CS123XmlSceneParser.cpp:236-262), and it composes
+  // as a full 4x4 product on the right (scenefile_loader.cpp:86-87).
+  bool compose_matrix(const XmlNode &e, Affine *t) {
+    static const char *const kNames[2][4] = {{"a", "b", "c", "d"}, {"v1", "v2", "v3", "v4"}};
+    float mat[4][4];
+    int col = 0;
+    for (auto &c : e.children) {
+      if (col == 4)
+        break;
+      int which = -1;
+      for (int w = 0; w < 2 && which < 0; ++w)
+        if (c->has(kNames[w][0]) && c->has(kNames[w][1]) && c->has(kNames[w][2]) &&
+            c->has(kNames[w][3]))
+          which = w;
+      if (which < 0) {
+        *err = "<matrix> element needs attributes a, b, c, d (or v1-v4)";
+        return false;
+      }
+      for (int i = 0; i < 4; ++i)
+        mat[i][col] = c->f(kNames[which][i]);
+      ++col;
+    }
+    if (col != 4) {
+      *err = "<matrix> needs four elements";
+      return false;
+    }
+    Affine r;
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 4; ++k)
+        r.m[i][k] = float(double(t->m[i][0]) * mat[0][k] + double(t->m[i][1]) * mat[1][k] +
+                          double(t->m[i][2]) * mat[2][k] + double(t->m[i][3]) * mat[3][k]);
+    *t = r;
+    return true;
+  }
+
+  // One scene-graph node per <transblock>: its transformations in order,
+  // then its primitives, then its subtrees and master references in order
+  // (scenefile_loader.cpp:70-102), so triangles are numbered as there.
   bool parse_tree(const XmlNode &obj, const Affine &parent, int depth) {
     if (depth > 64) {
       *err = "scene graph too deep (cycle?)";
@@ -732,36 +771,27 @@ struct XmlLoader {
           float ang = float(double(e->f("angle")) * M_PI / 180.0);
           t = t * Affine::rotate({e->f("x"), e->f("y"), e->f("z")}, ang);
         } else if (e->tag == "matrix") {
-          Affine m;
-          int r = 0;
-          for (auto &row : e->children) {
-            if (row->tag != "row" || r >= 4)
-              continue;
-            if (r < 3) {
-              m.m[r][0] = row->f("a");
-              m.m[r][1] = row->f("b");
-              m.m[r][2] = row->f("c");
-              m.m[r][3] = row->f("d");
-            }
-            ++r;
-          }
-          t = t * m;
+          if (!compose_matrix(*e, &t))
+            return false;
         }
+      }
+      for (auto &e : tb->children) {
+        if (e->tag != "object" || e->attrs["type"] != "primitive")
+          continue;
+        auto name = e->attrs["name"];
+        if (name != "mesh") {
+          *err = "only mesh primitives are supported (got '" + name + "')";
+          return false;
+        }
+        std::string file = e->has("meshfile") ? e->attrs["meshfile"] : e->attrs["filename"];
+        if (!add_mesh_file(file, t))
+          return false;
       }
       for (auto &e : tb->children) {
         if (e->tag != "object")
           continue;
         auto ty = e->attrs["type"];
-        if (ty == "primitive") {
-          auto name = e->attrs["name"];
-          if (name != "mesh") {
-            *err = "only mesh primitives are supported (got '" + name + "')";
-            return false;
-          }
-          std::string file = e->has("meshfile") ? e->attrs["meshfile"] : e->attrs["filename"];
-          if (!add_mesh_file(file, t))
-            return false;
-        } else if (ty == "tree") {
+        if (ty == "tree") {
           if (!parse_tree(*e, t, depth + 1))
             return false;
         } else if (ty == "master") {

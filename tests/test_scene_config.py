@@ -89,6 +89,91 @@ def test_ear_clipping_of_a_concave_polygon(tmp_path):
     assert abs(area - 3.0) < 1e-6
 
 
+def _affine(translate=(0, 0, 0), axis=None, angle_deg=0.0, scale=(1, 1, 1)):
+    """4x4 matrices of the CS123 transformations (Eigen Translation,
+    AngleAxis with the axis as given, Scaling)."""
+    m = np.eye(4)
+    if axis is not None:
+        a = np.asarray(axis, float)
+        th = np.radians(angle_deg)
+        k = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        m[:3, :3] = np.cos(th) * np.eye(3) + np.sin(th) * k + (1 - np.cos(th)) * np.outer(a, a)
+    m[:3, :3] = m[:3, :3] @ np.diag(scale)
+    m[:3, 3] = translate
+    return m
+
+
+def test_scene_graph_transforms_matrix_and_instancing(tmp_path):
+    """Nested transblocks compose on the right (scenefile_loader.cpp:70-102);
+    the k-th element of <matrix> is column k (CS123XmlSceneParser.cpp:236-262);
+    a node's primitives come before its subtrees; a mesh file used twice is
+    loaded once and instanced; master objects are reusable subtrees."""
+    (tmp_path / "m.mtl").write_text("newmtl a\n  illum 2\n  Kd 0.5 0.5 0.5\n")
+    (tmp_path / "a.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\n")
+    (tmp_path / "b.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 0 0 1\nv 1 1 0\nusemtl a\nf 1 2 3\n")
+    xml = """<scenefile>
+  <object type="tree" name="m">
+    <transblock>
+      <translate x="0" y="0" z="-1"/>
+      <object type="primitive" name="mesh" filename="b.obj"/>
+    </transblock>
+  </object>
+  <object type="tree" name="root">
+    <transblock>
+      <translate x="1" y="2" z="3"/>
+      <rotate x="0" y="0" z="1" angle="90"/>
+      <scale x="2" y="1" z="1"/>
+      <object type="tree">
+        <transblock>
+          <matrix>
+            <row a="0" b="1" c="0" d="0"/>
+            <row a="-1" b="0" c="0" d="0"/>
+            <row v1="0" v2="0" v3="1" v4="0"/>
+            <row a="5" b="6" c="7" d="1"/>
+          </matrix>
+          <object type="primitive" name="mesh" filename="a.obj"/>
+        </transblock>
+      </object>
+      <object type="primitive" name="mesh" filename="a.obj"/>
+    </transblock>
+    <transblock>
+      <scale x="1" y="1" z="2"/>
+      <object type="master" name="m"/>
+    </transblock>
+  </object>
+</scenefile>
+"""
+    (tmp_path / "g.xml").write_text(xml)
+    sc, _ = Scene.load(tmp_path / "g.xml", 1.0)
+    tris = sc.flat()["tris"].astype(np.float64)
+    world = np.stack([tris[:, 0:3], tris[:, 0:3] + tris[:, 3:6], tris[:, 0:3] + tris[:, 6:9]], 1)
+    a = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    b = np.array([[0, 0, 0], [0, 0, 1], [1, 1, 0]], float)
+    t1 = _affine(translate=(1, 2, 3)) @ _affine(axis=(0, 0, 1), angle_deg=90) @ _affine(scale=(2, 1, 1))
+    mat = np.array([[0, -1, 0, 5], [1, 0, 0, 6], [0, 0, 1, 7], [0, 0, 0, 1]], float)
+    t_master = _affine(scale=(1, 1, 2)) @ _affine(translate=(0, 0, -1))
+
+    def apply(m, v):
+        return v @ m[:3, :3].T + m[:3, 3]
+
+    # primitives of the first transblock, then its subtree, then the master
+    expected = np.stack([apply(t1, a), apply(t1 @ mat, a), apply(t_master, b)])
+    assert world.shape == expected.shape
+    assert np.allclose(world, expected, atol=1e-5)
+
+
+def test_matrix_with_three_rows_is_an_error(tmp_path):
+    (tmp_path / "m.mtl").write_text("newmtl a\n  illum 2\n  Kd 0.5 0.5 0.5\n")
+    (tmp_path / "a.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\n")
+    rows = "".join(f'<row a="{i}" b="0" c="0" d="0"/>' for i in range(3))
+    (tmp_path / "g.xml").write_text(
+        '<scenefile><object type="tree" name="root"><transblock><matrix>' + rows +
+        '</matrix><object type="primitive" name="mesh" filename="a.obj"/></transblock>'
+        "</object></scenefile>")
+    with pytest.raises(L.PathError, match="four"):
+        Scene.load(tmp_path / "g.xml", 1.0)
+
+
 def test_yaml_reference_config():
     s = Settings.from_yaml(GOLDEN / "configs" / "benchmark.yaml")
     assert s.kernel_approach == L.KERNEL_APPROACH["MegaKernel"]
