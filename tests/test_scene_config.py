@@ -75,6 +75,55 @@ def test_unsupported_illum_is_an_error(tmp_path):
     assert "illum" in str(e.value)
 
 
+@pytest.mark.parametrize("mtl,kind,diffuse,specular,weights", [
+    # illum 2, Kd only -> Diffuse
+    ("illum 2\nKd 0.2 0.4 0.6", "Diffuse", (0.2, 0.4, 0.6), None, None),
+    # Ks only, Ns <= 100 -> Glossy (Phong exponent Ns)
+    ("illum 2\nKs 0.3 0.3 0.3\nNs 40", "Glossy", None, (0.3, 0.3, 0.3), None),
+    # Ks only, Ns > 100 -> Mirror
+    ("illum 2\nKs 0.9 0.8 0.7\nNs 1000", "Mirror", None, (0.9, 0.8, 0.7), None),
+    # Kd + Ks: weights by channel sums, lobes divided by their weight
+    ("illum 2\nKd 0.3 0.3 0.3\nKs 0.1 0.1 0.1\nNs 20", "DiffuseGlossy",
+     (0.4, 0.4, 0.4), (0.4, 0.4, 0.4), (0.75, 0.25)),
+    ("illum 2\nKd 0.3 0.3 0.3\nKs 0.1 0.1 0.1\nNs 500", "DiffuseMirror",
+     (0.4, 0.4, 0.4), (0.4, 0.4, 0.4), (0.75, 0.25)),
+    # illum 5 -> dielectric, specular 1, ior Ni (Kd ignored)
+    ("illum 5\nKd 0.5 0.5 0.5\nNi 1.33", "DielectricRefractive", None, (1, 1, 1), None),
+])
+def test_mtl_conversion_branches(tmp_path, mtl, kind, diffuse, specular, weights):
+    """tinyobj_material_conversion.h:13-101, branch by branch."""
+    (tmp_path / "m.mtl").write_text("newmtl a\n" + mtl + "\nKe 1 2 3\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\n")
+    sc = Scene()
+    sc.add_obj(tmp_path / "m.obj")
+    m = sc.materials()[0]
+    assert m.type == L.BSDF[kind]
+    assert np.allclose(list(m.emission), [1, 2, 3])
+    if diffuse is not None:
+        assert np.allclose(list(m.diffuse), diffuse, rtol=1e-6)
+    if specular is not None:
+        assert np.allclose(list(m.specular), specular, rtol=1e-6)
+    if weights is not None:
+        assert np.allclose([m.diffuse_weight, m.glossy_weight], weights, rtol=1e-6)
+    if kind == "Glossy":
+        assert m.shininess == 40
+    if kind == "DielectricRefractive":
+        assert abs(m.ior - 1.33) < 1e-6
+
+
+@pytest.mark.parametrize("mtl,msg", [
+    ("illum 2\nKd 0.7 0.7 0.7\nKs 0.5 0.5 0.5", "exceeds 1"),   # always_assert
+    ("illum 2\nNs 1000", "mirror"),                             # mirror without Ks
+    ("illum 2", "unhandled material"),                          # neither Kd nor Ks
+    ("Kd 0.5 0.5 0.5", "illum"),                                # tinyobj default illum 0
+])
+def test_mtl_conversion_errors(tmp_path, mtl, msg):
+    (tmp_path / "m.mtl").write_text("newmtl a\n" + mtl + "\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\n")
+    with pytest.raises(L.PathError, match=msg):
+        Scene().add_obj(tmp_path / "m.obj")
+
+
 def test_ear_clipping_of_a_concave_polygon(tmp_path):
     (tmp_path / "m.mtl").write_text("newmtl a\n  illum 2\n  Kd 0.5 0.5 0.5\n")
     # L-shaped hexagon in the z=0 plane
