@@ -36,7 +36,7 @@ const char kUsage[] = R"(Path
       --bench-budget=<time>     Approx time in seconds for bench [default: 5.0]
       --config=<file_name>      Config file name. If no file is specified,
                                 default settings will be used.
-      -g --gpu                  Use gpu (always: this build renders on the GPU)
+      -g --gpu                  Use gpu (always: path_amd renders on the GPU)
 
       --bench                   Warm up and then run multiple times and report
                                 statistics

@@ -1,5 +1,5 @@
 """Training-data surface of the reference's `neural_render_generate_data_full_scene`
-module (src/generate_data/full_scene/python_module.cpp:9-46), on this build's
+module (src/generate_data/full_scene/python_module.cpp:9-46), on the path_amd
 renderer: `generate_data`, `generate_data_for_image`, `deinit_renderers`,
 `Constants`, and the `NetworkInputs` / `StandardData` / `ImageData` records
 with the reference's field names, tensor shapes and feature layouts
@@ -10,7 +10,7 @@ InitialIdxAndDir sample spec (each sample starts on a given triangle point
 with a given direction) and OutputPerStep output (one value per path step),
 as in the reference.
 
-Differences (each deliberate): scenes come from this build's seeded
+Differences (each deliberate): scenes come from path_amd's seeded
 box/sphere generator (`datagen.generate_scene`, seeds base_seed, base_seed+1,
 ...) instead of the reference's random-triangle generator; sample positions and
 directions are drawn with numpy's PCG64 seeded by base_seed (the reference's
