@@ -515,11 +515,24 @@ __global__ __launch_bounds__(1024) void k_wf_reduce(const float *__restrict__ sa
   ThreadInfo info = wd.get_thread_info(blockIdx.x, threadIdx.x);
   V3<float> total = {0.f, 0.f, 0.f};
   if (!info.exit) {
-    size_t base = size_t(info.x + info.y * wd.x_dim) * spp;
-    for (uint32_t s = info.start_sample; s < info.end_sample; ++s) {
-      const float *v = samples + 3 * (base + s);
-      total += V3<float>{v[0], v[1], v[2]};
+    const size_t first = size_t(info.x + info.y * wd.x_dim) * spp + info.start_sample;
+    const uint32_t count = info.end_sample - info.start_sample;
+    const float *v = samples + 3 * first;
+    uint32_t s = 0;
+    // four samples (three 16-byte loads) at a time when they are 16-byte
+    // aligned; the additions stay in sample order
+    if ((reinterpret_cast<uintptr_t>(v) & 15u) == 0) {
+      const float4 *v4 = reinterpret_cast<const float4 *>(v);
+      for (; s + 4 <= count; s += 4, v4 += 3) {
+        const float4 a = v4[0], b = v4[1], c = v4[2];
+        total += V3<float>{a.x, a.y, a.z};
+        total += V3<float>{a.w, b.x, b.y};
+        total += V3<float>{b.z, b.w, c.x};
+        total += V3<float>{c.y, c.z, c.w};
+      }
     }
+    for (; s < count; ++s)
+      total += V3<float>{v[3 * s], v[3 * s + 1], v[3 * s + 2]};
   }
   uint32_t sbs = wd.sample_block_size;
   uint32_t group = sbs < 64u ? sbs : 64u;
