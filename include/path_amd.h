@@ -280,11 +280,12 @@ int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                     double *seconds);
 
 /* Launch parameters of the wavefront integrator. Results never depend on
- * them. The path pool (max_num_samples_per_launch paths) runs as two
- * independent pipelines of half the pool each, on two streams driven by two
- * host threads, so one's shading overlaps the other's traversal.
+ * them. The path pool (max_num_samples_per_launch paths) runs as
+ * independent pipelines (pt_set_pipelines) of an equal share of the pool
+ * each, on their own streams driven by their own host threads, so one's
+ * shading overlaps another's traversal.
  * trace_blocks_per_cu: traversal grid per pipeline = CUs x this x 256
- * threads (default 6); iters_per_sync: iterations enqueued between host
+ * threads (default 4); iters_per_sync: iterations enqueued between host
  * checks; use_graphs: replay those iterations as one captured hipGraph. */
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
                          uint32_t iters_per_sync, int use_graphs);
@@ -297,7 +298,7 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
  * allows falls back to the host builder. Images do not depend on the tree. */
 enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1 };
 /* Independent wavefront pipelines a Streaming render splits its path pool
- * into (1..4, default 2; each at least 256 paths), each on its own stream and
+ * into (1..4, default 3; each at least 256 paths), each on its own stream and
  * host thread. Results never depend on it. */
 int pt_set_pipelines(pt_renderer *r, uint32_t n);
 int pt_set_bvh_builder(pt_renderer *r, int builder);
@@ -412,6 +413,9 @@ typedef struct pt_render_stats {
   double mega_ms;          /* mega-kernel launches */
   uint64_t mega_launches;
   uint32_t profiled;
+  uint32_t pipelines;      /* wavefront pipelines that ran at once (Streaming);
+                              their launches overlap, so a launch's duration
+                              covers its share of the rays under contention */
 } pt_render_stats;
 int pt_set_profiling(pt_renderer *r, int enable);
 int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out);

@@ -149,7 +149,7 @@ struct pt_renderer {
   int device = 0;
   hipStream_t stream = nullptr;   // everything, and wavefront pipeline 0
   hipStream_t pipe_streams[kMaxPipes - 1] = {}; // wavefront pipelines 1..
-  uint32_t n_pipes = 2; // wavefront pipelines of a render (1..kMaxPipes)
+  uint32_t n_pipes = 3; // wavefront pipelines of a render (1..kMaxPipes)
   hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_fork = nullptr;
   pt_render_stats stats{};
   bool profiling = false;
@@ -210,10 +210,10 @@ struct pt_renderer {
   pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
-  // persistent traversal waves; lanes refill from the wave's queue range. With
-  // two pipelines tracing at once, 4 / 6 / 8 measured 643 / 648 / 629
-  // Msamples/s on the 82k-triangle scene.
-  uint32_t trace_blocks_per_cu = 6;
+  // persistent traversal waves; lanes refill from the wave's queue range.
+  // Pipelines x blocks per CU on the 82k-triangle scene: 2 x 4 / 6 / 8
+  // measured 643 / 648 / 629 Msamples/s, 3 x 4 669.
+  uint32_t trace_blocks_per_cu = 4;
   uint32_t iters_per_sync = 1;
   bool use_graphs = false;
   bool gpu_bvh = false; // batched renders build their trees on the GPU (LBVH)

@@ -262,6 +262,7 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
   r.stats.ext_launches = busy_iters;
   r.stats.shadow_launches = busy_iters;
   r.stats.trace_launches = launches;
+  r.stats.pipelines = uint32_t(w.n);
   return PT_OK;
 }
 

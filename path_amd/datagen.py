@@ -79,14 +79,14 @@ def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -
             "iters_per_sync": out[2], "graphs": bool(out[3])}
 
 
-def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 6,
+def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 4,
                       iters_per_sync: int = 1, use_graphs: bool = False) -> None:
     L.check(L.lib().pt_set_launch_params(renderer.handle, trace_blocks_per_cu,
                                          iters_per_sync, int(use_graphs)))
 
 
 def set_pipelines(renderer: Renderer, n: int) -> None:
-    """Wavefront pipelines of a Streaming render (1-4, default 2)."""
+    """Wavefront pipelines of a Streaming render (1-4, default 3)."""
     L.check(L.lib().pt_set_pipelines(renderer.handle, n))
 
 

@@ -182,6 +182,7 @@ class RenderStats:
     mega_ms: float
     mega_launches: int
     profiled: bool
+    pipelines: int
 
 
 class Renderer:
@@ -286,7 +287,8 @@ class Renderer:
         L.check(L.lib().pt_last_render_stats(self.handle, C.byref(s)))
         return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches,
                            s.ext_ms, s.shadow_ms, s.ext_rays, s.shadow_rays, s.ext_launches,
-                           s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled))
+                           s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled),
+                           int(s.pipelines))
 
     def set_profiling(self, enable: bool) -> None:
         """Bracket every traversal / mega-kernel launch with HIP events."""

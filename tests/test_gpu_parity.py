@@ -100,12 +100,12 @@ def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
     from path_amd.datagen import set_pipelines
     s.max_num_samples_per_launch = 5000
     try:
-        for n in (1, 3, 4):
+        for n in (1, 2, 4):
             set_pipelines(renderer, n)
             wf3, _ = renderer.render(scene, ftw, size, size, spp, s)
             assert np.array_equal(mk.view(np.uint32), wf3.view(np.uint32)), n
     finally:
-        set_pipelines(renderer, 2)
+        set_pipelines(renderer, 3)
 
 
 def test_streaming_default_settings_multiplier_func(renderer, cornell):
