@@ -416,6 +416,8 @@ typedef struct pt_render_stats {
   uint32_t pipelines;      /* wavefront pipelines that ran at once (Streaming);
                               their launches overlap, so a launch's duration
                               covers its share of the rays under contention */
+  uint32_t replays;        /* pt_render_adjoint: replays of each path (1 when
+                              path lengths are bounded, else 2) */
 } pt_render_stats;
 int pt_set_profiling(pt_renderer *r, int enable);
 int pt_last_render_stats(const pt_renderer *r, pt_render_stats *out);

@@ -99,7 +99,7 @@ class PtRenderStats(C.Structure):
                 ("shadow_rays", C.c_uint64), ("ext_launches", C.c_uint64),
                 ("shadow_launches", C.c_uint64), ("mega_ms", C.c_double),
                 ("mega_launches", C.c_uint64), ("profiled", C.c_uint32),
-                ("pipelines", C.c_uint32)]
+                ("pipelines", C.c_uint32), ("replays", C.c_uint32)]
 
 
 class PtBatchItem(C.Structure):

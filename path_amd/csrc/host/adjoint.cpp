@@ -122,6 +122,7 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   HIP_TRY(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
   r.stats.kernel_ms = ms;
   r.stats.samples = total;
+  r.stats.replays = p.single_pass ? 1u : 2u;
   std::memcpy(grad_albedo, out.data(), 3 * n_mats * sizeof(float));
   std::memcpy(grad_emission, out.data() + 3 * n_mats, 3 * n_mats * sizeof(float));
   if (seconds)

@@ -183,6 +183,7 @@ class RenderStats:
     mega_launches: int
     profiled: bool
     pipelines: int
+    replays: int
 
 
 class Renderer:
@@ -288,7 +289,7 @@ class Renderer:
         return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches,
                            s.ext_ms, s.shadow_ms, s.ext_rays, s.shadow_rays, s.ext_launches,
                            s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled),
-                           int(s.pipelines))
+                           int(s.pipelines), int(s.replays))
 
     def set_profiling(self, enable: bool) -> None:
         """Bracket every traversal / mega-kernel launch with HIP events."""

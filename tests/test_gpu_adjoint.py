@@ -55,6 +55,8 @@ def test_adjoint_matches_fp64_oracle(renderer, dbox, n_iters):
     ga, ge = dr.adjoint(d_img)
     ga2, ge2 = dr.adjoint(d_img)
     assert np.array_equal(ga, ga2) and np.array_equal(ge, ge2)  # deterministic
+    # one replay per path while paths fit the vertex list (n_iters + 1 <= 16)
+    assert renderer.last_stats().replays == (1 if n_iters + 1 <= 16 else 2)
     tab = O.sobol_table(O.joe_kuo_directions(st.max_sample_size), st.max_sample_size, SPP)
     ra, re, img64, _ = O.adjoint(scene.flat(), ftw, st, SPP, X, Y, tab, d_img.astype(np.float64))
     # the FP32 image is the same estimator as the FP64 one (rare FP32/FP64
