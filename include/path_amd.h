@@ -301,6 +301,13 @@ enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1 };
  * into (1..4, default 3; each at least 256 paths), each on its own stream and
  * host thread. Results never depend on it. */
 int pt_set_pipelines(pt_renderer *r, uint32_t n);
+/* MegaKernel launch shape (no reference counterpart; the reference runs one
+ * lane per work-division thread, kernel_launch_impl_gpu.cuh:12-33). enable
+ * (default 1): where a pixel's threads fit one wave and there is no per-step
+ * output, four lanes share each work-division thread's samples and the
+ * thread's sum is formed in sample order afterwards. Results never depend on
+ * it. */
+int pt_set_megakernel_split(pt_renderer *r, int enable);
 int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */

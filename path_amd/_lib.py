@@ -154,6 +154,7 @@ SIGNATURES = {
     "pt_mlp_set_forward_kernel": (C.c_int, [_P, C.c_int]),
     "pt_set_bvh_builder": (C.c_int, [_P, C.c_int]),
     "pt_set_pipelines": (C.c_int, [_P, C.c_uint32]),
+    "pt_set_megakernel_split": (C.c_int, [_P, C.c_int]),
     "pt_last_bvh_build": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "pt_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_uint32, _F, _F]),
     "pt_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_float, _F]),

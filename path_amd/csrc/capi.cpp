@@ -425,6 +425,13 @@ int pt_set_pipelines(pt_renderer *r, uint32_t n) {
   return PT_OK;
 }
 
+int pt_set_megakernel_split(pt_renderer *r, int enable) {
+  if (!r)
+    return fail(PT_ERR_INVALID, "null renderer");
+  r->mega_split = enable != 0;
+  return PT_OK;
+}
+
 int pt_set_bvh_builder(pt_renderer *r, int builder) {
   if (!r || (builder != PT_BVH_HOST_SAH && builder != PT_BVH_GPU_LBVH))
     return fail(PT_ERR_INVALID, "unknown BVH builder");

@@ -94,8 +94,9 @@ hipError_t launch_sobol_table(const uint32_t *dirs, uint32_t dim_bound,
 hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
                             uint32_t n, float *t, uint32_t *idx, uint32_t *back,
                             hipStream_t s);
+// split: run the sample-parallel kernel where it applies (same image bits).
 hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
-                             uint32_t end_block, hipStream_t s);
+                             uint32_t end_block, bool split, hipStream_t s);
 // Sum of the num_sample_blocks partials per pixel (in sample-block order),
 // divided by samples_per.
 hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,

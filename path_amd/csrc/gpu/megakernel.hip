@@ -88,75 +88,86 @@ hipError_t launch_intersect(const DeviceScene &sc, const float *rays,
 // radiance added by iteration k of a path goes to steps[k]; a path stops after
 // n_steps iterations.
 template <class R, bool kPerStep>
+__device__ __forceinline__ V3<float> sample_path(const MegaParams &p, const ThreadInfo &info,
+                                                 uint32_t location, uint32_t sample,
+                                                 uint32_t *n_rays_traced, V3<float> *steps,
+                                                 LdsStack &stack) {
+  R rng;
+  if constexpr (std::is_same_v<R, TableRng>) {
+    rng = TableRng{p.rng.vals, sample,
+                   initial_dimension(location, p.rng.initial_dim_bound),
+                   p.rng.dim_bound, p.rng.samples_per};
+  } else {
+    rng = make_hash_rng(sample + location * p.rng.samples_per);
+  }
+  Ray<float> next_ray, light_ray;
+  Hit<float> given = {0.f, kNoHit, 0};
+  if (p.initial_rays) {
+    const float *r = p.initial_rays + 6 * size_t(info.x);
+    next_ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+    // InitialIdxAndDir: the first intersection is given, at distance 1 and
+    // front-facing by convention (initial_ray_sample.h:58-75)
+    if (p.initial_idx)
+      given = {1.f, p.initial_idx[info.x], 0};
+  } else {
+    float xo = rng.next();
+    float yo = rng.next();
+    next_ray = camera_ray(p.film_to_world, float(info.x) + xo,
+                          float(info.y) + yo, p.wd.x_dim, p.wd.y_dim);
+  }
+  light_ray = next_ray;
+  PathState<float> st = initial_path_state<float>();
+  for (uint32_t step = 0;; ++step) {
+    if constexpr (kPerStep) {
+      if (step >= p.n_steps)
+        break;
+      st.total = {0.f, 0.f, 0.f};
+    }
+    Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
+    if (st.n_light) {
+      light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
+                                st.light_target + 2e-6f,
+                                st.light_target - 2e-6f, stack);
+      ++*n_rays_traced;
+    }
+    if (st.has_next) {
+      if (step == 0 && p.initial_idx) {
+        next_hit = given;
+      } else {
+        next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
+                                 3.4028235e38f, -1.0f, stack);
+        ++*n_rays_traced;
+      }
+    }
+    PathState<float> ns;
+    Ray<float> nl = light_ray, nn = next_ray;
+    bool done = rendering_equation_iteration(st, rng, next_ray, light_hit,
+                                             next_hit, p.settings,
+                                             p.scene.view, &ns, &nl, &nn);
+    if constexpr (kPerStep)
+      steps[step] += ns.total;
+    if (done)
+      return ns.total;
+    st = ns;
+    light_ray = nl;
+    next_ray = nn;
+  }
+  return {0.f, 0.f, 0.f};
+}
+
+// The thread's samples in order; the sum adds each finished path's radiance
+// in sample order (the reference's per-thread float_rgb_total).
+template <class R, bool kPerStep>
 __device__ __forceinline__ V3<float>
 integrate_samples(const MegaParams &p, const ThreadInfo &info, uint32_t location,
                   uint32_t *rays_traced, V3<float> *steps, LdsStack &stack) {
   V3<float> total = {0.f, 0.f, 0.f};
   uint32_t n_rays_traced = 0;
   for (uint32_t sample = info.start_sample; sample < info.end_sample; ++sample) {
-    R rng;
-    if constexpr (std::is_same_v<R, TableRng>) {
-      rng = TableRng{p.rng.vals, sample,
-                     initial_dimension(location, p.rng.initial_dim_bound),
-                     p.rng.dim_bound, p.rng.samples_per};
-    } else {
-      rng = make_hash_rng(sample + location * p.rng.samples_per);
-    }
-    Ray<float> next_ray, light_ray;
-    Hit<float> given = {0.f, kNoHit, 0};
-    if (p.initial_rays) {
-      const float *r = p.initial_rays + 6 * size_t(info.x);
-      next_ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
-      // InitialIdxAndDir: the first intersection is given, at distance 1 and
-      // front-facing by convention (initial_ray_sample.h:58-75)
-      if (p.initial_idx)
-        given = {1.f, p.initial_idx[info.x], 0};
-    } else {
-      float xo = rng.next();
-      float yo = rng.next();
-      next_ray = camera_ray(p.film_to_world, float(info.x) + xo,
-                            float(info.y) + yo, p.wd.x_dim, p.wd.y_dim);
-    }
-    light_ray = next_ray;
-    PathState<float> st = initial_path_state<float>();
-    for (uint32_t step = 0;; ++step) {
-      if constexpr (kPerStep) {
-        if (step >= p.n_steps)
-          break;
-        st.total = {0.f, 0.f, 0.f};
-      }
-      Hit<float> light_hit = {0.f, kNoHit, 0}, next_hit = {0.f, kNoHit, 0};
-      if (st.n_light) {
-        light_hit = trace_closest(p.scene.nodes, p.scene.tris, light_ray,
-                                  st.light_target + 2e-6f,
-                                  st.light_target - 2e-6f, stack);
-        ++n_rays_traced;
-      }
-      if (st.has_next) {
-        if (step == 0 && p.initial_idx) {
-          next_hit = given;
-        } else {
-          next_hit = trace_closest(p.scene.nodes, p.scene.tris, next_ray,
-                                   3.4028235e38f, -1.0f, stack);
-          ++n_rays_traced;
-        }
-      }
-      PathState<float> ns;
-      Ray<float> nl = light_ray, nn = next_ray;
-      bool done = rendering_equation_iteration(st, rng, next_ray, light_hit,
-                                               next_hit, p.settings,
-                                               p.scene.view, &ns, &nl, &nn);
-      if constexpr (kPerStep)
-        steps[step] += ns.total;
-      if (done) {
-        if constexpr (!kPerStep)
-          total += ns.total;
-        break;
-      }
-      st = ns;
-      light_ray = nl;
-      next_ray = nn;
-    }
+    const V3<float> v = sample_path<R, kPerStep>(p, info, location, sample, &n_rays_traced,
+                                                 steps, stack);
+    if constexpr (!kPerStep)
+      total += v;
   }
   *rays_traced = n_rays_traced;
   return total;
@@ -211,6 +222,25 @@ __device__ __forceinline__ void reduce_store(const MegaParams &p, const ThreadIn
   }
 }
 
+// Rays traced by the block, added to the render's counter with one device
+// atomic per block (atomics on one address serialise).
+__device__ __forceinline__ void count_rays(const MegaParams &p, uint32_t traced) {
+  if (!p.ray_counter)
+    return;
+  __shared__ uint32_t block_rays;
+  if (threadIdx.x == 0)
+    block_rays = 0;
+  __syncthreads();
+  uint32_t lane_total = traced;
+  for (int off = 32; off >= 1; off >>= 1)
+    lane_total += __shfl_xor(lane_total, off, 64);
+  if ((threadIdx.x & 63u) == 0)
+    atomicAdd(&block_rays, lane_total);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(p.ray_counter, (unsigned long long)block_rays);
+}
+
 // Blocks of up to 1024 threads (16 waves); dynamic LDS holds the traversal
 // stacks (kLdsStackInts per wave), deeper entries go to a private array.
 template <bool kPerStep>
@@ -241,35 +271,103 @@ __global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
   } else {
     reduce_store(p, info, block, total, p.out_rgb);
   }
-  if (p.ray_counter) {
-    // one device atomic per block: atomics on one address serialise
-    __shared__ uint32_t block_rays;
-    if (threadIdx.x == 0)
-      block_rays = 0;
-    __syncthreads();
-    uint32_t lane_total = traced;
-    for (int off = 32; off >= 1; off >>= 1)
-      lane_total += __shfl_xor(lane_total, off, 64);
-    if ((threadIdx.x & 63u) == 0)
-      atomicAdd(&block_rays, lane_total);
-    __syncthreads();
-    if (threadIdx.x == 0)
-      atomicAdd(p.ray_counter, (unsigned long long)block_rays);
+  count_rays(p, traced);
+}
+
+// Sample-parallel mega-kernel. The work division gives a thread a pixel and a
+// run of consecutive samples, and the one-lane-per-thread kernel above leaves
+// few waves per SIMD when there are few threads (65,536 pixels x 2 threads
+// for 256 x 256 at 16 spp: 2 waves per SIMD). Here kSplit lanes share a
+// thread: lane j traces samples start + j, start + j + kSplit, ..., the path
+// radiances go to LDS, and one lane per thread then adds them in sample order
+// and the pixel's threads are reduced on consecutive lanes of wave 0 exactly
+// as above, so the image is the same bit for bit. Used when the pixel's
+// threads fit one wave (sample_block_size <= 64) and there is no per-step
+// output.
+constexpr uint32_t kSplit = 4;
+constexpr uint32_t kSplitBlock = 256;                  // lanes per block
+constexpr uint32_t kSplitThreads = kSplitBlock / kSplit; // work-division threads per block
+
+__global__ __launch_bounds__(kSplitBlock) void k_megakernel_split(MegaParams p,
+                                                                  uint32_t start_block,
+                                                                  uint32_t max_spt) {
+  extern __shared__ int mk_lds[];
+  // traversal stacks (kLdsStackInts per wave), then [kSplitThreads][max_spt] radiances
+  int spill[kOverflowDepth];
+  LdsStack stack{lds_stack_column(mk_lds), spill, 1};
+  float *rad = reinterpret_cast<float *>(mk_lds + (kSplitBlock / 64) * kLdsStackInts);
+  const uint32_t parts = p.wd.block_size / kSplitThreads;
+  const uint32_t block = start_block + blockIdx.x / parts;
+  const uint32_t first = (blockIdx.x % parts) * kSplitThreads;
+  const uint32_t t = threadIdx.x / kSplit, j = threadIdx.x % kSplit;
+  const ThreadInfo info = p.wd.get_thread_info(block, first + t);
+  uint32_t traced = 0;
+  if (!info.exit) {
+    const uint32_t location = info.x + info.y * p.wd.x_dim;
+    for (uint32_t sample = info.start_sample + j; sample < info.end_sample; sample += kSplit) {
+      const V3<float> v =
+          p.rng.type == kRngSobel
+              ? sample_path<TableRng, false>(p, info, location, sample, &traced, nullptr, stack)
+              : sample_path<HashRng, false>(p, info, location, sample, &traced, nullptr, stack);
+      float *d = rad + 3 * (size_t(t) * max_spt + (sample - info.start_sample));
+      d[0] = v.x;
+      d[1] = v.y;
+      d[2] = v.z;
+    }
   }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // wave 0: lane i is thread first + i (kSplitThreads == 64)
+    const ThreadInfo own = p.wd.get_thread_info(block, first + threadIdx.x);
+    V3<float> total = {0.f, 0.f, 0.f};
+    if (!own.exit) {
+      const float *src = rad + 3 * size_t(threadIdx.x) * max_spt;
+      for (uint32_t k = 0; k < own.end_sample - own.start_sample; ++k)
+        total += V3<float>{src[3 * k], src[3 * k + 1], src[3 * k + 2]};
+    }
+    const uint32_t sbs = p.wd.sample_block_size;
+    total.x = group_reduce_sum(total.x, sbs);
+    total.y = group_reduce_sum(total.y, sbs);
+    total.z = group_reduce_sum(total.z, sbs);
+    if ((threadIdx.x % sbs) == 0 && !own.exit) {
+      if (p.divide_by_spp) {
+        const float spp = float(p.rng.samples_per);
+        total = {total.x / spp, total.y / spp, total.z / spp};
+      }
+      const size_t idx = size_t(p.wd.sample_block_idx(block)) +
+                         size_t(p.wd.num_sample_blocks) *
+                             (size_t(own.x) + size_t(own.y) * p.wd.x_dim);
+      p.out_rgb[3 * idx + 0] = total.x;
+      p.out_rgb[3 * idx + 1] = total.y;
+      p.out_rgb[3 * idx + 2] = total.z;
+    }
+  }
+  count_rays(p, traced);
 }
 
 hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
-                             uint32_t end_block, hipStream_t s) {
+                             uint32_t end_block, bool split, hipStream_t s) {
   if (end_block <= start_block)
     return hipSuccess;
   if (p.wd.block_size == 0 || p.wd.block_size > 1024 || p.wd.block_size % 64 != 0)
     return hipErrorInvalidConfiguration;
-  const size_t lds = size_t(p.wd.block_size / 64) * kLdsStackInts * sizeof(int);
   int dev = 0, lds_max = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
           hipSuccess)
     return hipErrorInvalidDevice;
+  const uint32_t max_spt =
+      p.wd.base_samples_per_thread + (p.wd.n_threads_per_unit_extra ? 1u : 0u);
+  const size_t split_lds = size_t(kSplitBlock / 64) * kLdsStackInts * sizeof(int) +
+                           size_t(kSplitThreads) * max_spt * 3 * sizeof(float);
+  if (split && !p.n_steps && p.wd.sample_block_size <= 64 && max_spt >= 2 &&
+      split_lds + 256 <= size_t(lds_max)) {
+    const uint32_t parts = p.wd.block_size / kSplitThreads;
+    hipLaunchKernelGGL(k_megakernel_split, dim3((end_block - start_block) * parts),
+                       dim3(kSplitBlock), split_lds, s, p, start_block, max_spt);
+    return hipGetLastError();
+  }
+  const size_t lds = size_t(p.wd.block_size / 64) * kLdsStackInts * sizeof(int);
   if (lds + 256 > size_t(lds_max)) // 256: the reduction's static LDS
     return hipErrorInvalidConfiguration;
   if (p.n_steps)

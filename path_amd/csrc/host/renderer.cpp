@@ -362,7 +362,7 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
       uint32_t end = start + chunk < total ? start + chunk : total;
       if (profiling)
         HIP_TRY(hipEventRecord(ev_prof[0][0], stream));
-      HIP_TRY(gpu::launch_megakernel(p, start, end, stream));
+      HIP_TRY(gpu::launch_megakernel(p, start, end, mega_split, stream));
       if (profiling) {
         HIP_TRY(hipEventRecord(ev_prof[0][1], stream));
         HIP_TRY(hipEventSynchronize(ev_prof[0][1]));

@@ -214,6 +214,8 @@ struct pt_renderer {
   // Pipelines x blocks per CU on the 82k-triangle scene: 2 x 4 / 6 / 8
   // measured 643 / 648 / 629 Msamples/s, 3 x 4 669.
   uint32_t trace_blocks_per_cu = 4;
+  // mega-kernel: several lanes per work-division thread (same image bits)
+  bool mega_split = true;
   uint32_t iters_per_sync = 1;
   bool use_graphs = false;
   bool gpu_bvh = false; // batched renders build their trees on the GPU (LBVH)

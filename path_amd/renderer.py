@@ -291,6 +291,11 @@ class Renderer:
                            s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled),
                            int(s.pipelines), int(s.replays))
 
+    def set_megakernel_split(self, enable: bool) -> None:
+        """Several lanes per work-division thread in the MegaKernel
+        integrator (default on; the image does not depend on it)."""
+        L.check(L.lib().pt_set_megakernel_split(self.handle, int(enable)))
+
     def set_profiling(self, enable: bool) -> None:
         """Bracket every traversal / mega-kernel launch with HIP events."""
         L.check(L.lib().pt_set_profiling(self.handle, int(enable)))

@@ -81,6 +81,37 @@ def test_megakernel_block_sizes_bit_exact(renderer, cornell_sphere, block_size, 
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("block_size,per_thread,spp,x,y", [
+    (256, 8, 16, 64, 64),    # the default shape: 2 threads per pixel, 8 samples each
+    (256, 3, 13, 37, 19),    # uneven samples per thread, ragged image
+    (128, 32, 64, 40, 24),   # 32 samples per thread (the largest run), 2 blocks per 256 lanes
+    (1024, 5, 7, 33, 17),    # one thread per pixel, 7 samples over 4 lanes, 16-wave blocks
+    (64, 2, 64, 16, 16),     # 32 threads per pixel: the reduction spans half a wave
+])
+def test_megakernel_split_bit_exact(renderer, cornell_sphere, block_size, per_thread, spp, x, y):
+    """Sample-parallel mega-kernel (four lanes per work-division thread) equals
+    the one-lane-per-thread kernel and the CPU checker bit for bit."""
+    from oracle import oracle as O
+    scene, ftw = cornell_sphere
+    s = _settings()
+    wd = s.render_work_division
+    wd.block_size = block_size
+    wd.force_target_samples = 1
+    wd.forced_target_samples_per_thread = per_thread
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp)
+    ref, _ = O.render(scene.flat(), ftw, s, spp, x, y, tab, precision=32)
+    try:
+        renderer.set_megakernel_split(False)
+        one, _ = renderer.render(scene, ftw, x, y, spp, s)
+        rays_one = renderer.last_stats().rays
+    finally:
+        renderer.set_megakernel_split(True)
+    split, _ = renderer.render(scene, ftw, x, y, spp, s)
+    assert renderer.last_stats().rays == rays_one
+    assert np.array_equal(one.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(split.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("which,spp,size", [("cornell", 16, 64), ("cornell_sphere", 8, 48)])
 def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
     """The wavefront integrator produces the mega-kernel's image bit for bit
