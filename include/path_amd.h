@@ -200,11 +200,15 @@ enum pt_output_type {
   PT_OUTPUT_BGRA = 0,     /* uint32 per pixel, memory order B,G,R,0 */
   PT_OUTPUT_FLOAT_RGB = 1, /* float[3] per pixel */
   PT_OUTPUT_PER_STEP = 2,  /* n_steps float[3] images, one per path vertex */
+  PT_OUTPUT_DEVICE_FLOAT_RGB = 3, /* float[3] per pixel in device memory of the
+                                     renderer's GPU (no reference counterpart:
+                                     the image stays in HBM for GPU consumers) */
 };
 
 typedef struct pt_output {
   uint32_t type;
-  void *data;         /* BGRA / FLOAT_RGB: host buffer of x_dim*y_dim pixels */
+  void *data;         /* BGRA / FLOAT_RGB: host buffer of x_dim*y_dim pixels;
+                         DEVICE_FLOAT_RGB: device buffer of as many pixels */
   float **per_step;   /* PER_STEP: n_steps host buffers of float[3] pixels */
   uint32_t n_steps;
 } pt_output;
@@ -308,6 +312,11 @@ int pt_set_pipelines(pt_renderer *r, uint32_t n);
  * thread's sum is formed in sample order afterwards. Results never depend on
  * it. */
 int pt_set_megakernel_split(pt_renderer *r, int enable);
+/* Device memory on the renderer's GPU for PT_OUTPUT_DEVICE_FLOAT_RGB (no
+ * reference counterpart): allocate, free, and copy back to the host. */
+int pt_device_alloc(pt_renderer *r, size_t bytes, void **out);
+int pt_device_free(pt_renderer *r, void *ptr);
+int pt_device_read(pt_renderer *r, const void *src, void *host_dst, size_t bytes);
 int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */

@@ -29,7 +29,7 @@ DIR_SAMPLER = {"Uniform": 0, "BSDF": 1}
 TERM_PROB = {"Constant": 0, "NIters": 1, "MultiplierFunc": 2, "Normalize": 3}
 RNG = {"Uniform": 0, "Sobel": 1}
 SAMPLE_SPEC = {"SquareImage": 0, "InitialRays": 1, "InitialIdxAndDir": 2}
-OUTPUT = {"BGRA": 0, "FloatRGB": 1, "OutputPerStep": 2}
+OUTPUT = {"BGRA": 0, "FloatRGB": 1, "OutputPerStep": 2, "DeviceFloatRGB": 3}
 
 
 class PtMaterial(C.Structure):
@@ -155,6 +155,9 @@ SIGNATURES = {
     "pt_set_bvh_builder": (C.c_int, [_P, C.c_int]),
     "pt_set_pipelines": (C.c_int, [_P, C.c_uint32]),
     "pt_set_megakernel_split": (C.c_int, [_P, C.c_int]),
+    "pt_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "pt_device_free": (C.c_int, [_P, _P]),
+    "pt_device_read": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "pt_last_bvh_build": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "pt_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_uint32, _F, _F]),
     "pt_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_float, _F]),

@@ -425,6 +425,41 @@ int pt_set_pipelines(pt_renderer *r, uint32_t n) {
   return PT_OK;
 }
 
+// ------------------------------ device buffers ----------------------------
+
+int pt_device_alloc(pt_renderer *r, size_t bytes, void **out) {
+  if (!r || !out || bytes == 0)
+    return fail(PT_ERR_INVALID, "device buffer needs a renderer, an output and a size");
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = hipMalloc(out, bytes);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("device buffer: ") + hipGetErrorString(e));
+  return PT_OK;
+}
+
+int pt_device_free(pt_renderer *r, void *ptr) {
+  if (!r)
+    return fail(PT_ERR_INVALID, "null renderer");
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess && ptr)
+    e = hipFree(ptr);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("device buffer free: ") + hipGetErrorString(e));
+  return PT_OK;
+}
+
+int pt_device_read(pt_renderer *r, const void *src, void *host_dst, size_t bytes) {
+  if (!r || !src || !host_dst)
+    return fail(PT_ERR_INVALID, "null argument");
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = hipMemcpy(host_dst, src, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("device buffer read: ") + hipGetErrorString(e));
+  return PT_OK;
+}
+
 int pt_set_megakernel_split(pt_renderer *r, int enable) {
   if (!r)
     return fail(PT_ERR_INVALID, "null renderer");

@@ -288,7 +288,10 @@ constexpr uint32_t kSplit = 4;
 constexpr uint32_t kSplitBlock = 256;                  // lanes per block
 constexpr uint32_t kSplitThreads = kSplitBlock / kSplit; // work-division threads per block
 
-__global__ __launch_bounds__(kSplitBlock) void k_megakernel_split(MegaParams p,
+// Six waves per SIMD (80 VGPRs, a few spills) measured 3-5% faster than the
+// compiler's four (121 VGPRs) on the 36-triangle box.
+__global__ __launch_bounds__(kSplitBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void
+k_megakernel_split(MegaParams p,
                                                                   uint32_t start_block,
                                                                   uint32_t max_spt) {
   extern __shared__ int mk_lds[];

@@ -284,6 +284,16 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
         return set_error(PT_ERR_INVALID, "null per-step output buffer");
   } else if (!out.data) {
     return set_error(PT_ERR_INVALID, "null output buffer");
+  } else if (out.type == PT_OUTPUT_DEVICE_FLOAT_RGB) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, out.data) != hipSuccess ||
+        attr.type != hipMemoryTypeDevice || attr.device != device) {
+      (void)hipGetLastError();
+      return set_error(PT_ERR_INVALID,
+                       "DEVICE_FLOAT_RGB output must be device memory of the renderer's GPU");
+    }
+  } else if (out.type != PT_OUTPUT_BGRA && out.type != PT_OUTPUT_FLOAT_RGB) {
+    return set_error(PT_ERR_INVALID, "unknown output type");
   }
   if ((per_step || spec.type != PT_SAMPLE_SQUARE_IMAGE) &&
       settings.kernel_approach != PT_MEGA_KERNEL &&
@@ -397,6 +407,9 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   if (out.type == PT_OUTPUT_BGRA)
     HIP_TRY(hipMemcpyAsync(out.data, bgra.ptr, size_t(n_pixels) * 4,
                            hipMemcpyDeviceToHost, stream));
+  else if (out.type == PT_OUTPUT_DEVICE_FLOAT_RGB)
+    HIP_TRY(hipMemcpyAsync(out.data, final_rgb, size_t(n_pixels) * 12,
+                           hipMemcpyDeviceToDevice, stream));
   else if (per_step)
     for (uint32_t i = 0; i < out.n_steps; ++i)
       HIP_TRY(hipMemcpyAsync(out.per_step[i], final_rgb + size_t(i) * n_pixels * 3,

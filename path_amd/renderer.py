@@ -186,6 +186,33 @@ class RenderStats:
     replays: int
 
 
+class DeviceBuffer:
+    """Device memory on a renderer's GPU (pt_device_alloc), e.g. the target
+    of a DeviceFloatRGB render."""
+
+    def __init__(self, renderer: "Renderer", nbytes: int):
+        self.renderer = renderer
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        L.check(L.lib().pt_device_alloc(renderer.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def numpy(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        L.check(L.lib().pt_device_read(self.renderer.handle, C.c_void_p(self.ptr),
+                                       out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            L.lib().pt_device_free(self.renderer.handle, C.c_void_p(self.ptr))
+            self.ptr = None
+
+
 class Renderer:
     """render::Renderer on one GPU (ExecutionModel::GPU)."""
 
@@ -198,12 +225,15 @@ class Renderer:
                settings: Settings | None = None, output: str = "FloatRGB",
                show_progress: bool = False, show_times: bool = False,
                rays: np.ndarray | None = None, initial_idx: np.ndarray | None = None,
-               n_steps: int = 8):
+               n_steps: int = 8, device_out=None):
         """Renderer::render. Sample spec: SquareImage, or InitialRays if `rays`
         (n, 6) is given, or InitialIdxAndDir if `initial_idx` (n,) is given
         too (the first intersection is that triangle at distance 1 along each
         ray). Output "BGRA", "FloatRGB" or "OutputPerStep" (n_steps images,
-        shape (n_steps, y, x, 3); MegaKernel). Returns (image, seconds)."""
+        shape (n_steps, y, x, 3); MegaKernel) or "DeviceFloatRGB" (into
+        `device_out`: a DeviceBuffer of this renderer, or a contiguous float32
+        torch tensor of y*x*3 elements on its GPU; the image never leaves
+        HBM). Returns (image, seconds)."""
         settings = settings or Settings()
         spec = L.PtSampleSpec()
         keep = []
@@ -229,6 +259,19 @@ class Renderer:
         if output == "BGRA":
             img = np.zeros((y_dim, x_dim), np.uint32)
             out.data = img.ctypes.data
+        elif output == "DeviceFloatRGB":
+            if isinstance(device_out, DeviceBuffer):
+                ok = device_out.nbytes >= y_dim * x_dim * 12
+            else:  # a torch tensor on the renderer's GPU
+                ok = (device_out is not None and getattr(device_out, "is_cuda", False)
+                      and device_out.is_contiguous() and device_out.numel() == y_dim * x_dim * 3
+                      and str(device_out.dtype) == "torch.float32")
+            if not ok:
+                raise ValueError("DeviceFloatRGB needs device_out: a DeviceBuffer of at least "
+                                 "y_dim * x_dim * 12 bytes or a contiguous float32 CUDA tensor "
+                                 "of y_dim * x_dim * 3 elements")
+            img = device_out
+            out.data = device_out.data_ptr()
         elif output == "OutputPerStep":
             img = np.zeros((n_steps, y_dim, x_dim, 3), np.float32)
             ptrs = (C.POINTER(C.c_float) * n_steps)(*[L.fptr(img[i]) for i in range(n_steps)])

@@ -183,3 +183,37 @@ def test_argument_errors(renderer, cornell):
     # the renderer still works after the errors
     img, _ = renderer.render(scene, ftw, 8, 8, 1, _settings())
     assert img.any()
+
+
+def test_device_output(renderer, cornell):
+    """DEVICE_FLOAT_RGB leaves the image in HBM: the same bits as the host
+    output, for both integrators; host memory and short buffers are rejected."""
+    import ctypes as C
+    from path_amd import _lib as L
+    from path_amd.renderer import DeviceBuffer
+    scene, ftw = cornell
+    dev = DeviceBuffer(renderer, 24 * 40 * 12)
+    for approach in ("MegaKernel", "Streaming"):
+        s = _settings(approach)
+        host, _ = renderer.render(scene, ftw, 40, 24, 4, s)
+        out, _ = renderer.render(scene, ftw, 40, 24, 4, s, output="DeviceFloatRGB",
+                                 device_out=dev)
+        assert out is dev
+        got = dev.numpy((24, 40, 3))
+        assert np.array_equal(got.view(np.uint32), host.view(np.uint32)), approach
+    with pytest.raises(ValueError):
+        renderer.render(scene, ftw, 48, 48, 1, _settings(), output="DeviceFloatRGB",
+                        device_out=dev)
+    # the C ABI checks the pointer itself
+    spec = L.PtSampleSpec()
+    spec.type = L.SAMPLE_SPEC["SquareImage"]
+    spec.x_dim = spec.y_dim = 8
+    spec.film_to_world[:] = [float(v) for v in np.asarray(ftw).ravel()]
+    host_buf = np.zeros((8, 8, 3), np.float32)
+    out = L.PtOutput()
+    out.type = L.OUTPUT["DeviceFloatRGB"]
+    out.data = host_buf.ctypes.data
+    secs = C.c_double()
+    rc = L.lib().pt_render(renderer.handle, C.byref(spec), C.byref(out), scene.handle, 1,
+                           C.byref(_settings().raw), 0, 0, C.byref(secs))
+    assert rc != 0 and b"device memory" in L.lib().pt_last_error()
