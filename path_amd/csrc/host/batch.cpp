@@ -271,15 +271,56 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     if (!items[i].scene || items[i].x_dim == 0 || items[i].y_dim == 0 || !items[i].out_rgb)
       return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
                                            ": null scene/output or empty image");
+  // outputs: all host memory, or all device memory of this GPU
+  bool device_out = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    for (const void *ptr : {static_cast<const void *>(items[i].out_rgb),
+                            static_cast<const void *>(items[i].out_gbuffer)}) {
+      if (!ptr)
+        continue;
+      hipPointerAttribute_t attr{};
+      bool dev = hipPointerGetAttributes(&attr, ptr) == hipSuccess &&
+                 attr.type == hipMemoryTypeDevice;
+      if (!dev)
+        (void)hipGetLastError();
+      if (dev && attr.device != r.device)
+        return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                             ": output on another GPU");
+      if (i == 0 && ptr == items[0].out_rgb)
+        device_out = dev;
+      else if (dev != device_out)
+        return set_error(PT_ERR_INVALID, "batch outputs must be all host or all device memory");
+    }
+  }
   const bool gpu_bvh = r.gpu_bvh;
-  build_all(items, n, gpu_bvh);
+  // The scenes of the last batch stay on the GPU: the same scenes (by
+  // generation), sizes, spp and tree builder skip the host packing, the upload
+  // and the tree builds.
+  std::vector<uint64_t> key;
+  key.reserve(3 * size_t(n) + 2);
+  key.push_back(spp);
+  key.push_back(gpu_bvh ? 1u : 0u);
+  for (uint32_t i = 0; i < n; ++i) {
+    key.push_back(items[i].scene->generation);
+    key.push_back(items[i].scene->mat_generation);
+    key.push_back(uint64_t(items[i].x_dim) | (uint64_t(items[i].y_dim) << 32));
+  }
+  const bool resident = key == r.batch_key;
   std::vector<gpu::SceneDesc> desc;
   Layout lay;
   uint64_t n_pixels = 0, total = 0;
-  int rc = pack(r, items, n, spp, gpu_bvh, &desc, &lay, &n_pixels, &total);
+  int rc = PT_OK;
+  hipStream_t s = r.stream;
+  if (resident) {
+    desc = r.batch_desc;
+    n_pixels = r.batch_pixels;
+    total = r.batch_samples;
+  } else {
+  r.batch_key.clear();
+  build_all(items, n, gpu_bvh);
+  rc = pack(r, items, n, spp, gpu_bvh, &desc, &lay, &n_pixels, &total);
   if (rc != PT_OK)
     return rc;
-  hipStream_t s = r.stream;
   {
     const char *base = static_cast<const char *>(r.b_stage_in.ptr);
     auto up = [&](auto &buf, int sec) {
@@ -313,6 +354,12 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     r.last_gpu_bvh_ms = std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now() - tb).count();
     r.last_gpu_bvh_levels = levels;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  r.batch_key = std::move(key);
+  r.batch_desc = desc;
+  r.batch_pixels = n_pixels;
+  r.batch_samples = total;
   }
 
   gpu::RngParams rng{};
@@ -384,9 +431,36 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     }
   }
   HIP_TRY(hipEventRecord(r.ev_stop, s));
-  // results: one copy each into pinned staging, then a parallel scatter
   const size_t rgb_bytes = size_t(n_pixels) * 3 * sizeof(float);
   const size_t gb_bytes = want_gbuffer ? size_t(n_pixels) * 12 * sizeof(float) : 0;
+  if (device_out) {
+    // results stay in HBM: one copy per array when the caller's buffers are
+    // laid out in batch order, else one per scene
+    bool packed = true;
+    for (uint32_t i = 0; i < n && packed; ++i) {
+      const size_t off = desc[i].pixel_off;
+      packed = items[i].out_rgb == items[0].out_rgb + 3 * off &&
+               (!want_gbuffer || items[i].out_gbuffer == items[0].out_gbuffer + 12 * off);
+    }
+    if (packed) {
+      HIP_TRY(hipMemcpyAsync(items[0].out_rgb, r.rgb.ptr, rgb_bytes, hipMemcpyDeviceToDevice, s));
+      if (want_gbuffer)
+        HIP_TRY(hipMemcpyAsync(items[0].out_gbuffer, r.b_gbuffer.ptr, gb_bytes,
+                               hipMemcpyDeviceToDevice, s));
+    } else {
+      for (uint32_t i = 0; i < n; ++i) {
+        const size_t off = desc[i].pixel_off;
+        const size_t px = size_t(items[i].x_dim) * items[i].y_dim;
+        HIP_TRY(hipMemcpyAsync(items[i].out_rgb, r.rgb.ptr + 3 * off, px * 12,
+                               hipMemcpyDeviceToDevice, s));
+        if (items[i].out_gbuffer)
+          HIP_TRY(hipMemcpyAsync(items[i].out_gbuffer, r.b_gbuffer.ptr + 12 * off, px * 48,
+                                 hipMemcpyDeviceToDevice, s));
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+  } else {
+  // results: one copy each into pinned staging, then a parallel scatter
   HIP_TRY(r.b_stage_out.ensure(rgb_bytes + gb_bytes));
   char *stage = static_cast<char *>(r.b_stage_out.ptr);
   HIP_TRY(hipMemcpyAsync(stage, r.rgb.ptr, rgb_bytes, hipMemcpyDeviceToHost, s));
@@ -401,6 +475,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     if (items[i].out_gbuffer)
       std::memcpy(items[i].out_gbuffer, stage + rgb_bytes + off * 48, px * 48);
   });
+  }
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
   r.stats.kernel_ms = ms;

@@ -198,6 +198,11 @@ struct pt_renderer {
   pt::host::DeviceBuf<unsigned long long> b_sample_begin;
   pt::host::DeviceBuf<float> b_gbuffer;
   pt::host::PinnedBuf b_stage_in, b_stage_out;
+  // the batch whose scenes are on the GPU: (spp, tree builder, then scene
+  // generation, material generation and size per item)
+  std::vector<uint64_t> batch_key;
+  std::vector<pt::gpu::SceneDesc> batch_desc;
+  uint64_t batch_pixels = 0, batch_samples = 0;
 
   // adjoint buffers
   pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;

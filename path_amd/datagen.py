@@ -66,6 +66,42 @@ def render_batch(renderer: Renderer, scenes, samples_per: int,
     return images, gbufs, secs.value
 
 
+def render_batch_device(renderer: Renderer, scenes, samples_per: int,
+                        settings: Settings | None = None, gbuffer: bool = True, out=None):
+    """render_batch with the results left in HBM: the images (and G-buffers)
+    of all scenes packed in batch order in one device buffer each, scene i
+    from pixel offsets[i]. `out` (a previous return value) is reused when it
+    is large enough. Returns (rgb, gbuffer or None, offsets, seconds)."""
+    from .renderer import DeviceBuffer
+    settings = settings or datagen_settings()
+    px = [g.resolution * g.resolution for g in scenes]
+    offsets = np.concatenate([[0], np.cumsum(px)]).astype(np.int64)
+    total = int(offsets[-1])
+    rgb = gb = None
+    if out is not None:
+        rgb, gb = out[0], out[1]
+        if rgb.nbytes < total * 12 or (gbuffer and (gb is None or gb.nbytes < total * 48)):
+            rgb = gb = None
+    if rgb is None:
+        rgb = DeviceBuffer(renderer, max(total, 1) * 12)
+        gb = DeviceBuffer(renderer, max(total, 1) * 48) if gbuffer else None
+    items = (L.PtBatchItem * len(scenes))()
+    fp = C.POINTER(C.c_float)
+    for i, g in enumerate(scenes):
+        r = g.resolution
+        items[i].scene = g.scene.handle
+        items[i].x_dim = r
+        items[i].y_dim = r
+        items[i].film_to_world[:] = [float(v) for v in g.film_to_world]
+        items[i].out_rgb = C.cast(C.c_void_p(rgb.ptr + 12 * int(offsets[i])), fp)
+        items[i].out_gbuffer = (C.cast(C.c_void_p(gb.ptr + 48 * int(offsets[i])), fp)
+                                if gbuffer else None)
+    secs = C.c_double()
+    L.check(L.lib().pt_render_batch(renderer.handle, items, len(scenes), samples_per,
+                                    C.byref(settings.raw), C.byref(secs)))
+    return rgb, (gb if gbuffer else None), offsets[:-1], secs.value
+
+
 def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -> dict:
     """Pick the fastest (path pool, traversal blocks per CU, iterations per
     host sync) for this workload; the renderer and `settings` keep it."""

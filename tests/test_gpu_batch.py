@@ -104,3 +104,41 @@ def test_gbuffer_first_hit(renderer, gen_scenes):
         assert agree.mean() > 0.97
         assert np.allclose(emi[hit][ok][agree], want_emi[agree])
         assert (gb[~hit.reshape(gb.shape[:2])] == 0).all()
+
+
+def test_batch_device_output_and_resident_scenes(renderer, gen_scenes):
+    """Images and G-buffers left in HBM equal the host output bit for bit; a
+    repeated batch reuses the scenes already on the GPU, and a material change
+    invalidates them."""
+    from path_amd.datagen import (datagen_settings, generate_scene, render_batch,
+                                  render_batch_device, set_launch_params)
+    set_launch_params(renderer, 6, 1, False)
+    st = datagen_settings()
+    scenes = gen_scenes[:3] + [generate_scene(50)]
+    host, host_g, _ = render_batch(renderer, scenes, SPP, st)
+    total = sum(g.resolution ** 2 for g in scenes)
+
+    def check(dev, imgs, gbs):
+        rgb = dev[0].numpy((total, 3))
+        gb = dev[1].numpy((total, 12))
+        for i, g in enumerate(scenes):
+            o, px = int(dev[2][i]), g.resolution ** 2
+            assert np.array_equal(rgb[o:o + px].view(np.uint32),
+                                  imgs[i].reshape(-1, 3).view(np.uint32)), i
+            assert np.array_equal(gb[o:o + px].view(np.uint32),
+                                  gbs[i].reshape(-1, 12).view(np.uint32)), i
+
+    dev = render_batch_device(renderer, scenes, SPP, st)
+    check(dev, host, host_g)
+    dev = render_batch_device(renderer, scenes, SPP, st, out=dev)  # resident scenes
+    check(dev, host, host_g)
+    # a material change must reach the GPU copy of the batch
+    from path_amd import _lib as L
+    mats = scenes[3].scene.materials()
+    k = next(i for i, m in enumerate(mats) if m.type == L.BSDF["Diffuse"] and any(m.diffuse))
+    mats[k].diffuse[:] = [0.5 * v for v in mats[k].diffuse]
+    scenes[3].scene.set_material(k, mats[k])
+    dev = render_batch_device(renderer, scenes, SPP, st, out=dev)
+    host2, host2_g, _ = render_batch(renderer, [generate_scene(0)] + scenes, SPP, st)
+    check(dev, host2[1:], host2_g[1:])
+    assert not np.array_equal(host2[4], host[3])
