@@ -326,23 +326,44 @@ def bench_adjoint(a, cfg):
         d_img = (2.0 * (img.astype(np.float64) - target)).astype(np.float32)
         return img, d_img, dr.adjoint(d_img)
 
+    # the gradient step in HBM: render into a device buffer, dL/dI = 2 (I - T)
+    # on the GPU, adjoint from it; only the 48 gradient floats come back
+    from path_amd.renderer import DeviceBuffer
+    n_f = size * size * 3
+    dev_img, dev_tgt, dev_d = (DeviceBuffer(r, n_f * 4) for _ in range(3))
+    dev_tgt.write(target.astype(np.float32))
+
+    def gradient_device():
+        r.render(scene, ftw, size, size, spp, settings, output="DeviceFloatRGB",
+                 device_out=dev_img)
+        fwd = r.last_stats().kernel_ms
+        r.l2_loss_gradient(dev_img, dev_tgt, n_f, dev_d)
+        ga, ge, _ = r.adjoint(scene, ftw, size, size, spp, dev_d, settings)
+        return fwd, ga, ge
+
     for _ in range(a.warmup):
         gradient()
+        gradient_device()
     sync = _sync_fn()
     sync()
     t0 = time.perf_counter()
     fwd_ms = adj_ms = 0.0
     replays = 1
     for _ in range(a.steps):
-        img = dr.render()
-        fwd_ms += r.last_stats().kernel_ms
-        d_img = (2.0 * (img.astype(np.float64) - target)).astype(np.float32)
-        ga, ge = dr.adjoint(d_img)
+        f_ms, ga, ge = gradient_device()
+        fwd_ms += f_ms
         adj_ms += r.last_stats().kernel_ms
         replays = r.last_stats().replays
     sync()
     elapsed = time.perf_counter() - t0
     samples = size * size * spp
+    # the same step through host arrays (DiffRenderer / the autograd Function)
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        img, d_img, (ga_h, ge_h) = gradient()
+    host_elapsed = time.perf_counter() - t1
+    assert np.array_equal(ga_h, ga) and np.array_equal(ge_h, ge), \
+        "device and host gradient steps differ"
 
     # agreement: central differences (h = 1e-3) of the loss through the GPU's
     # FP32 renders and, with the CPU leg, through FP64 renders of the same
@@ -391,6 +412,8 @@ def bench_adjoint(a, cfg):
         "forward_ms": round(fwd_ms / a.steps, 3),
         "adjoint_ms": round(adj_ms / a.steps, 3),
         "max_rel_err_vs_central_differences_fp32_renders": fd_err,
+        "output": "image, its loss gradient and the adjoint input stay in HBM",
+        "host_arrays_msamples_per_s": round(samples * a.steps / host_elapsed / 1e6, 3),
     }
     counts = None
     if not a.no_cpu_baseline:

@@ -317,6 +317,13 @@ int pt_set_megakernel_split(pt_renderer *r, int enable);
 int pt_device_alloc(pt_renderer *r, size_t bytes, void **out);
 int pt_device_free(pt_renderer *r, void *ptr);
 int pt_device_read(pt_renderer *r, const void *src, void *host_dst, size_t bytes);
+int pt_device_write(pt_renderer *r, void *dst, const void *host_src, size_t bytes);
+/* d_image = 2 (image - target) for n floats, all in device memory of the
+ * renderer's GPU: the image gradient of an L2 loss, ready for
+ * pt_render_adjoint, whose d_image may be device memory too (no reference
+ * counterpart; the gradient step stays in HBM). */
+int pt_l2_loss_gradient(pt_renderer *r, const float *image, const float *target, uint64_t n,
+                        float *d_image);
 int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */

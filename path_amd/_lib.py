@@ -158,6 +158,8 @@ SIGNATURES = {
     "pt_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "pt_device_free": (C.c_int, [_P, _P]),
     "pt_device_read": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "pt_device_write": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "pt_l2_loss_gradient": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
     "pt_last_bvh_build": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "pt_mlp_gradient": (C.c_int, [_P, _F, _F, C.c_uint32, _F, _F]),
     "pt_mlp_train_step": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_float, _F]),

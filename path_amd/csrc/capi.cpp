@@ -460,6 +460,40 @@ int pt_device_read(pt_renderer *r, const void *src, void *host_dst, size_t bytes
   return PT_OK;
 }
 
+int pt_device_write(pt_renderer *r, void *dst, const void *host_src, size_t bytes) {
+  if (!r || !dst || !host_src)
+    return fail(PT_ERR_INVALID, "null argument");
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = hipMemcpy(dst, host_src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("device buffer write: ") + hipGetErrorString(e));
+  return PT_OK;
+}
+
+int pt_l2_loss_gradient(pt_renderer *r, const float *image, const float *target, uint64_t n,
+                        float *d_image) {
+  if (!r || !image || !target || !d_image)
+    return fail(PT_ERR_INVALID, "null argument");
+  for (const void *ptr : {static_cast<const void *>(image), static_cast<const void *>(target),
+                          static_cast<const void *>(d_image)}) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess || attr.type != hipMemoryTypeDevice ||
+        attr.device != r->device) {
+      (void)hipGetLastError();
+      return fail(PT_ERR_INVALID, "pt_l2_loss_gradient takes device memory of the renderer's GPU");
+    }
+  }
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = pt::gpu::launch_l2_gradient(image, target, n, d_image, r->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("l2 gradient: ") + hipGetErrorString(e));
+  return PT_OK;
+}
+
 int pt_set_megakernel_split(pt_renderer *r, int enable) {
   if (!r)
     return fail(PT_ERR_INVALID, "null renderer");

@@ -230,5 +230,23 @@ hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
   return hipGetLastError();
 }
 
+// d = 2 (image - target), each difference in double and rounded once to
+// float: the gradient of sum((image - target)^2).
+__global__ void k_l2_gradient(const float *__restrict__ image, const float *__restrict__ target,
+                              uint64_t n, float *__restrict__ d) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    d[i] = float(2.0 * (double(image[i]) - double(target[i])));
+}
+
+hipError_t launch_l2_gradient(const float *image, const float *target, uint64_t n, float *d,
+                              hipStream_t s) {
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_l2_gradient, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, image,
+                     target, n, d);
+  return hipGetLastError();
+}
+
 } // namespace gpu
 } // namespace pt

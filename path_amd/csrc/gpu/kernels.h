@@ -116,6 +116,9 @@ hipError_t launch_nrc_combine(const float *base, const float *weight, const floa
 hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
                                    uint32_t n_params, double scale, float *out,
                                    hipStream_t s);
+// d[i] = 2 (image[i] - target[i]) (device arrays of n floats).
+hipError_t launch_l2_gradient(const float *image, const float *target, uint64_t n, float *d,
+                              hipStream_t s);
 
 } // namespace gpu
 } // namespace pt

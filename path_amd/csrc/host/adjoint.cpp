@@ -87,10 +87,16 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   const uint32_t n_blocks = uint32_t((n_threads + gpu::kAdjointBlock - 1) / gpu::kAdjointBlock);
   const uint32_t n_params = 6 * n_mats;
   hipStream_t s = r.stream;
-  HIP_TRY(r.adj_d_image.upload(d_image, size_t(n_pixels) * 3, s));
+  hipPointerAttribute_t attr{};
+  const bool d_on_device = hipPointerGetAttributes(&attr, d_image) == hipSuccess &&
+                           attr.type == hipMemoryTypeDevice && attr.device == r.device;
+  if (!d_on_device) {
+    (void)hipGetLastError();
+    HIP_TRY(r.adj_d_image.upload(d_image, size_t(n_pixels) * 3, s));
+  }
   HIP_TRY(r.adj_partials.ensure(size_t(n_blocks) * n_params));
   HIP_TRY(r.adj_out.ensure(n_params));
-  p.d_image = r.adj_d_image.ptr;
+  p.d_image = d_on_device ? d_image : r.adj_d_image.ptr;
   p.partials = r.adj_partials.ptr;
   // one replay per path when the path length is bounded (NIters: at most
   // n_iters + 1 vertices), else radiance replay + back-propagation replay

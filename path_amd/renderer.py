@@ -207,6 +207,12 @@ class DeviceBuffer:
                                        out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    def write(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        L.check(L.lib().pt_device_write(self.renderer.handle, C.c_void_p(self.ptr),
+                                        a.ctypes.data_as(C.c_void_p), a.nbytes))
+
     def __del__(self):
         if getattr(self, "ptr", None):
             L.lib().pt_device_free(self.renderer.handle, C.c_void_p(self.ptr))
@@ -310,14 +316,19 @@ class Renderer:
         material's albedo color and emission. Returns (grad_albedo (n, 3),
         grad_emission (n, 3), seconds)."""
         settings = settings or Settings()
-        d = np.ascontiguousarray(d_image, np.float32).reshape(y_dim, x_dim, 3)
+        if isinstance(d_image, DeviceBuffer):  # already in HBM
+            assert d_image.nbytes >= y_dim * x_dim * 12
+            d_ptr = C.cast(C.c_void_p(d_image.ptr), C.POINTER(C.c_float))
+        else:
+            d = np.ascontiguousarray(d_image, np.float32).reshape(y_dim, x_dim, 3)
+            d_ptr = L.fptr(d)
         nm = scene.counts()[1]
         ga = np.zeros((nm, 3), np.float32)
         ge = np.zeros((nm, 3), np.float32)
         ftw = np.ascontiguousarray(np.asarray(film_to_world).ravel(), np.float32)
         secs = C.c_double()
         L.check(L.lib().pt_render_adjoint(self.handle, scene.handle, L.fptr(ftw), x_dim, y_dim,
-                                          samples_per, C.byref(settings.raw), L.fptr(d),
+                                          samples_per, C.byref(settings.raw), d_ptr,
                                           L.fptr(ga), L.fptr(ge), C.byref(secs)))
         return ga, ge, secs.value
 
@@ -333,6 +344,15 @@ class Renderer:
                            s.ext_ms, s.shadow_ms, s.ext_rays, s.shadow_rays, s.ext_launches,
                            s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled),
                            int(s.pipelines), int(s.replays))
+
+    def l2_loss_gradient(self, image: DeviceBuffer, target: DeviceBuffer, n_floats: int,
+                         out: DeviceBuffer) -> DeviceBuffer:
+        """out = 2 (image - target) on the GPU (device buffers of n_floats)."""
+        assert min(image.nbytes, target.nbytes, out.nbytes) >= 4 * n_floats
+        L.check(L.lib().pt_l2_loss_gradient(self.handle, C.c_void_p(image.ptr),
+                                            C.c_void_p(target.ptr), n_floats,
+                                            C.c_void_p(out.ptr)))
+        return out
 
     def set_megakernel_split(self, enable: bool) -> None:
         """Several lanes per work-division thread in the MegaKernel

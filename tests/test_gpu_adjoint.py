@@ -166,3 +166,26 @@ def test_adjoint_rejects_unsupported(renderer, dbox):
     s.term_prob = "MultiplierFunc"
     with pytest.raises(PathError):
         renderer.adjoint(scene, ftw, 16, 16, 4, np.zeros((16, 16, 3), np.float32), s)
+
+
+def test_gradient_step_in_hbm(renderer, dbox):
+    """Render into device memory, dL/dI = 2 (I - T) on the GPU, adjoint from
+    the device gradient: the same gradients as the host-array step."""
+    from path_amd.diff import DiffRenderer
+    from path_amd.renderer import DeviceBuffer
+    scene, ftw = dbox
+    st = _settings(4)
+    dr = DiffRenderer(renderer, scene, ftw, X, Y, SPP, st)
+    target, alb, em, _ = _target_and_start(dr)
+    dr.set_parameters(alb, em)
+    img = dr.render().astype(np.float64)
+    d_host = (2.0 * (img - target)).astype(np.float32)
+    ga, ge = dr.adjoint(d_host)
+    n = X * Y * 3
+    dev_img, dev_tgt, dev_d = (DeviceBuffer(renderer, n * 4) for _ in range(3))
+    dev_tgt.write(np.asarray(target, np.float32))
+    renderer.render(scene, ftw, X, Y, SPP, st, output="DeviceFloatRGB", device_out=dev_img)
+    renderer.l2_loss_gradient(dev_img, dev_tgt, n, dev_d)
+    assert np.array_equal(dev_d.numpy((Y, X, 3)).view(np.uint32), d_host.view(np.uint32))
+    gd, ged, _ = renderer.adjoint(scene, ftw, X, Y, SPP, dev_d, st)
+    assert np.array_equal(gd, ga) and np.array_equal(ged, ge)
