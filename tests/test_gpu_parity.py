@@ -172,7 +172,8 @@ def test_output_per_step_bit_exact(renderer, cornell, n_steps):
         assert np.allclose(gpu.sum(0), full, rtol=1e-5, atol=1e-6)
 
 
-def test_initial_rays_and_idx_bit_exact(renderer, cornell_sphere):
+@pytest.mark.parametrize("spp", [8, 64])  # 64: several samples per thread (split kernel)
+def test_initial_rays_and_idx_bit_exact(renderer, cornell_sphere, spp):
     from oracle import oracle as O
     scene, _ = cornell_sphere
     flat = scene.flat()
@@ -185,10 +186,10 @@ def test_initial_rays_and_idx_bit_exact(renderer, cornell_sphere):
     n /= np.linalg.norm(n, axis=1, keepdims=True)
     rays = np.concatenate([c + n, -n], 1).astype(np.float32)
     s = _settings("MegaKernel")
-    tab = O.sobol_table(O.joe_kuo_directions(256), 256, 8)
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp)
     for idx in (None, pick):
-        gpu, _ = renderer.render(scene, None, 0, 0, 8, s, rays=rays, initial_idx=idx)
-        cpu, _ = O.render(flat, np.zeros(12, np.float32), s, 8, len(rays), 1, tab, 32, 0,
+        gpu, _ = renderer.render(scene, None, 0, 0, spp, s, rays=rays, initial_idx=idx)
+        cpu, _ = O.render(flat, np.zeros(12, np.float32), s, spp, len(rays), 1, tab, 32, 0,
                           initial_rays=rays, initial_idx=idx)
         assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), idx is None
 
