@@ -328,12 +328,12 @@ int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */
 int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels);
-/* Time candidate (path pool, traversal grid, iterations per sync) triples on
- * the given items, keep the fastest on the renderer; out[4] = path pool,
- * blocks per CU, iterations per sync, graphs used. */
+/* Time candidate (path pool, traversal grid, iterations per sync, pipelines)
+ * on the given items, keep the fastest on the renderer; out[5] = path pool,
+ * blocks per CU, iterations per sync, graphs used, pipelines. */
 int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                       uint32_t samples_per, pt_settings *settings,
-                      uint32_t out[4]);
+                      uint32_t out[5]);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiable rendering. No reference counterpart (the reference has no  */

@@ -518,7 +518,7 @@ int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels) {
 
 int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                       uint32_t samples_per, pt_settings *settings,
-                      uint32_t out[4]) {
+                      uint32_t out[5]) {
   PT_GUARD_BEGIN
   if (!r || !items || !n || !settings || !out)
     return fail(PT_ERR_INVALID, "null argument");

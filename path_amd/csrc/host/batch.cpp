@@ -487,45 +487,52 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
 }
 
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
-                   uint32_t spp, pt_settings &settings, uint32_t out[4]) {
+                   uint32_t spp, pt_settings &settings, uint32_t out[5]) {
   const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
   const uint32_t blocks[] = {4, 6, 8, 16};
   const uint32_t iters[] = {1, 4};
+  const uint32_t pipes[] = {2, 3, 4};
   double best = 1e30;
   uint32_t bp = settings.max_num_samples_per_launch, bb = r.trace_blocks_per_cu,
-           bi = r.iters_per_sync;
+           bi = r.iters_per_sync, bn = r.n_pipes;
   bool saved_graphs = r.use_graphs;
   r.use_graphs = true;
   // warm-up (uploads, Sobol table, code objects)
   int rc = render_batch(r, items, n, spp, settings, nullptr);
   if (rc != PT_OK)
     return rc;
-  for (uint32_t pool : pools)
-    for (uint32_t b : blocks)
-      for (uint32_t it : iters) {
-        settings.max_num_samples_per_launch = pool;
-        r.trace_blocks_per_cu = b;
-        r.iters_per_sync = it;
-        rc = render_batch(r, items, n, spp, settings, nullptr);
-        if (rc != PT_OK)
-          return rc;
-        double t = r.stats.kernel_ms;
-        if (t < best) {
-          best = t;
-          bp = pool;
-          bb = b;
-          bi = it;
+  for (uint32_t np : pipes)
+    for (uint32_t pool : pools)
+      for (uint32_t b : blocks)
+        for (uint32_t it : iters) {
+          settings.max_num_samples_per_launch = pool;
+          r.trace_blocks_per_cu = b;
+          r.iters_per_sync = it;
+          r.n_pipes = np;
+          rc = render_batch(r, items, n, spp, settings, nullptr);
+          if (rc != PT_OK) {
+            r.use_graphs = saved_graphs;
+            return rc;
+          }
+          double t = r.stats.kernel_ms;
+          if (t < best) {
+            best = t;
+            bp = pool;
+            bb = b;
+            bi = it;
+            bn = np;
+          }
         }
-      }
   settings.max_num_samples_per_launch = bp;
   r.trace_blocks_per_cu = bb;
   r.iters_per_sync = bi;
-  r.use_graphs = true;
-  (void)saved_graphs;
+  r.n_pipes = bn;
+  r.use_graphs = true; // every candidate ran with graphs
   out[0] = bp;
   out[1] = bb;
   out[2] = bi;
   out[3] = 1;
+  out[4] = bn;
   return PT_OK;
 }
 

@@ -266,7 +266,7 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
 int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                  uint32_t spp, const pt_settings &settings, double *seconds);
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
-                   uint32_t spp, pt_settings &settings, uint32_t out[4]);
+                   uint32_t spp, pt_settings &settings, uint32_t out[5]);
 int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_world[12],
                    uint32_t x_dim, uint32_t y_dim, uint32_t spp,
                    const pt_settings &settings, const float *d_image, float *grad_albedo,

@@ -107,12 +107,12 @@ def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -
     host sync) for this workload; the renderer and `settings` keep it."""
     # the output arrays must outlive the call: the renders write into them
     items, images, _ = _items(scenes, False)
-    out = (C.c_uint32 * 4)()
+    out = (C.c_uint32 * 5)()
     L.check(L.lib().pt_autotune_batch(renderer.handle, items, len(scenes), samples_per,
                                       C.byref(settings.raw), out))
     del images
     return {"max_num_samples_per_launch": out[0], "trace_blocks_per_cu": out[1],
-            "iters_per_sync": out[2], "graphs": bool(out[3])}
+            "iters_per_sync": out[2], "graphs": bool(out[3]), "pipelines": out[4]}
 
 
 def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 4,
