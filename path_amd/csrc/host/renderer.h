@@ -198,6 +198,7 @@ struct pt_renderer {
   pt::host::DeviceBuf<unsigned long long> b_sample_begin;
   pt::host::DeviceBuf<float> b_gbuffer;
   pt::host::PinnedBuf b_stage_in, b_stage_out;
+  pt::host::PinnedBuf out_stage; // host outputs of pt_render
   // the batch whose scenes are on the GPU: (spp, tree builder, then scene
   // generation, material generation and size per item)
   std::vector<uint64_t> batch_key;
