@@ -92,8 +92,13 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   const bool d_on_device = hipPointerGetAttributes(&attr, d_image) == hipSuccess &&
                            attr.type == hipMemoryTypeDevice && attr.device == r.device;
   if (!d_on_device) {
+    // host d_image: one host memcpy into pinned staging, then one DMA copy
     (void)hipGetLastError();
-    HIP_TRY(r.adj_d_image.upload(d_image, size_t(n_pixels) * 3, s));
+    const size_t bytes = size_t(n_pixels) * 3 * sizeof(float);
+    HIP_TRY(r.out_stage.ensure(bytes));
+    std::memcpy(r.out_stage.ptr, d_image, bytes);
+    HIP_TRY(r.adj_d_image.ensure(size_t(n_pixels) * 3));
+    HIP_TRY(hipMemcpyAsync(r.adj_d_image.ptr, r.out_stage.ptr, bytes, hipMemcpyHostToDevice, s));
   }
   HIP_TRY(r.adj_partials.ensure(size_t(n_blocks) * n_params));
   HIP_TRY(r.adj_out.ensure(n_params));
