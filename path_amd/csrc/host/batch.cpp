@@ -315,6 +315,22 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     desc = r.batch_desc;
     n_pixels = r.batch_pixels;
     total = r.batch_samples;
+    // The cameras are not part of the key: a resident batch rendered from
+    // new viewpoints re-uploads only the per-scene descriptors (96 B each).
+    bool moved = false;
+    for (uint32_t i = 0; i < n; ++i)
+      if (std::memcmp(desc[i].film_to_world, items[i].film_to_world,
+                      sizeof(desc[i].film_to_world)) != 0) {
+        std::memcpy(desc[i].film_to_world, items[i].film_to_world,
+                    sizeof(desc[i].film_to_world));
+        moved = true;
+      }
+    if (moved) {
+      HIP_TRY(hipMemcpyAsync(r.b_desc.ptr, desc.data(), n * sizeof(gpu::SceneDesc),
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      r.batch_desc = desc;
+    }
   } else {
   r.batch_key.clear();
   build_all(items, n, gpu_bvh);
@@ -495,7 +511,24 @@ int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   double best = 1e30;
   uint32_t bp = settings.max_num_samples_per_launch, bb = r.trace_blocks_per_cu,
            bi = r.iters_per_sync, bn = r.n_pipes;
-  bool saved_graphs = r.use_graphs;
+  // Every error return leaves the renderer and the caller's settings as they
+  // were before the search (the five tuned values).
+  struct Restore {
+    pt_renderer &r;
+    pt_settings &s;
+    uint32_t pool, blocks, iters, pipes;
+    bool graphs;
+    bool armed = true;
+    ~Restore() {
+      if (!armed)
+        return;
+      s.max_num_samples_per_launch = pool;
+      r.trace_blocks_per_cu = blocks;
+      r.iters_per_sync = iters;
+      r.n_pipes = pipes;
+      r.use_graphs = graphs;
+    }
+  } restore{r, settings, bp, bb, bi, bn, r.use_graphs};
   r.use_graphs = true;
   // warm-up (uploads, Sobol table, code objects)
   int rc = render_batch(r, items, n, spp, settings, nullptr);
@@ -510,10 +543,8 @@ int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
           r.iters_per_sync = it;
           r.n_pipes = np;
           rc = render_batch(r, items, n, spp, settings, nullptr);
-          if (rc != PT_OK) {
-            r.use_graphs = saved_graphs;
+          if (rc != PT_OK)
             return rc;
-          }
           double t = r.stats.kernel_ms;
           if (t < best) {
             best = t;
@@ -523,6 +554,7 @@ int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
             bn = np;
           }
         }
+  restore.armed = false;
   settings.max_num_samples_per_launch = bp;
   r.trace_blocks_per_cu = bb;
   r.iters_per_sync = bi;

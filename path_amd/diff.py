@@ -70,15 +70,30 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, albedo, emission, dr: DiffRenderer):
-        dr.set_parameters(albedo.detach().cpu().numpy(), emission.detach().cpu().numpy())
+        alb = albedo.detach().to("cpu", torch.float32).numpy().copy()
+        em = emission.detach().to("cpu", torch.float32).numpy().copy()
+        dr.set_parameters(alb, em)
         img = torch.from_numpy(dr.render())
         ctx.dr = dr
+        # The adjoint depends on the parameters the image was rendered with;
+        # another forward through the same DiffRenderer before this backward
+        # changes the scene, so keep this call's values.
+        ctx.params = (alb, em)
         ctx.devices = (albedo.device, emission.device)
         return img.to(albedo.device)
 
     @staticmethod
     def backward(ctx, grad_image):
         dr = ctx.dr
-        ga, ge = dr.adjoint(grad_image.detach().to("cpu", torch.float32).contiguous().numpy())
+        cur_alb, cur_em = (t.numpy() for t in dr.parameters())
+        alb, em = ctx.params
+        swap = not (np.array_equal(cur_alb, alb) and np.array_equal(cur_em, em))
+        if swap:
+            dr.set_parameters(alb, em)
+        try:
+            ga, ge = dr.adjoint(grad_image.detach().to("cpu", torch.float32).contiguous().numpy())
+        finally:
+            if swap:
+                dr.set_parameters(cur_alb, cur_em)
         return (torch.from_numpy(ga).to(ctx.devices[0]), torch.from_numpy(ge).to(ctx.devices[1]),
                 None)

@@ -278,6 +278,13 @@ class Renderer:
                                  "of y_dim * x_dim * 3 elements")
             img = device_out
             out.data = device_out.data_ptr()
+            if not isinstance(device_out, DeviceBuffer):
+                # pt_render writes on the renderer's own stream: torch work
+                # still reading this tensor must finish first. pt_render
+                # returns after its stream synchronised, so the image is
+                # complete for any torch stream afterwards.
+                import torch
+                torch.cuda.current_stream(device_out.device).synchronize()
         elif output == "OutputPerStep":
             img = np.zeros((n_steps, y_dim, x_dim, 3), np.float32)
             ptrs = (C.POINTER(C.c_float) * n_steps)(*[L.fptr(img[i]) for i in range(n_steps)])
@@ -331,6 +338,17 @@ class Renderer:
                                           samples_per, C.byref(settings.raw), d_ptr,
                                           L.fptr(ga), L.fptr(ge), C.byref(secs)))
         return ga, ge, secs.value
+
+    def reduce_samples(self, wds: L.PtWorkDivisionSettings, values: np.ndarray,
+                       num_locations: int, samples_per: int) -> np.ndarray:
+        """The renderer's per-pixel reduction on the GPU applied to
+        values[sample + location * samples_per] (one sample block)."""
+        v = np.ascontiguousarray(values, np.float32).ravel()
+        assert v.size == num_locations * samples_per
+        out = np.zeros(num_locations, np.float32)
+        L.check(L.lib().pt_reduce_samples(self.handle, C.byref(wds), L.fptr(v), num_locations,
+                                          samples_per, L.fptr(out)))
+        return out
 
     def sobol_table(self, dim_bound: int, count: int) -> np.ndarray:
         out = np.zeros((count, dim_bound), np.float32)

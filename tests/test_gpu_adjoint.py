@@ -159,6 +159,38 @@ def test_autograd_function(renderer, dbox):
     dr.set_parameters(alb0.numpy(), em0.numpy())
 
 
+def test_autograd_two_forwards_one_backward(renderer, dbox):
+    """Two forwards through one DiffRenderer at different parameters, then one
+    backward of the summed loss: each forward's gradient is the adjoint at its
+    own parameters (the second forward must not leak into the first's
+    backward), and the scene keeps the latest parameters afterwards."""
+    import torch
+    from path_amd.diff import DiffRenderer
+    scene, ftw = dbox
+    dr = DiffRenderer(renderer, scene, ftw, 64, 64, 8, _settings())
+    alb0, em0 = dr.parameters()
+    target = torch.from_numpy(dr.render())
+    a1 = (alb0 * 0.9).requires_grad_(True)
+    a2 = (alb0 * 0.6).requires_grad_(True)
+    e1 = (em0 * 1.1).requires_grad_(True)
+    e2 = (em0 * 0.7).requires_grad_(True)
+    img1 = dr(a1, e1)
+    img2 = dr(a2, e2)
+    (((img1 - target) ** 2).sum() + ((img2 - target) ** 2).sum()).backward()
+    cur_a, cur_e = dr.parameters()
+    assert torch.equal(cur_a, a2.detach()) and torch.equal(cur_e, e2.detach())
+    # each against a single forward + adjoint at that forward's parameters
+    for a, e, img in ((a1, e1, img1), (a2, e2, img2)):
+        dr.set_parameters(a.detach().numpy(), e.detach().numpy())
+        ref = dr.render()
+        assert np.array_equal(ref.view(np.uint32), img.detach().numpy().view(np.uint32))
+        ga, ge = dr.adjoint((2.0 * (ref - target.numpy())).astype(np.float32))
+        assert torch.equal(a.grad, torch.from_numpy(ga))
+        assert torch.equal(e.grad, torch.from_numpy(ge))
+    assert not torch.equal(a1.grad, a2.grad)
+    dr.set_parameters(alb0.numpy(), em0.numpy())
+
+
 def test_adjoint_rejects_unsupported(renderer, dbox):
     from path_amd._lib import PathError
     scene, ftw = dbox

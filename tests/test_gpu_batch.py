@@ -142,3 +142,29 @@ def test_batch_device_output_and_resident_scenes(renderer, gen_scenes):
     host2, host2_g, _ = render_batch(renderer, [generate_scene(0)] + scenes, SPP, st)
     check(dev, host2[1:], host2_g[1:])
     assert not np.array_equal(host2[4], host[3])
+
+
+def test_resident_batch_with_moved_cameras(renderer, gen_scenes):
+    """The same scenes rendered again from other viewpoints: the batch stays
+    resident (no repacking), the new cameras reach the GPU, and every image
+    equals rendering that scene alone from its new camera."""
+    import dataclasses
+    from path_amd.datagen import datagen_settings, render_batch, set_launch_params
+    set_launch_params(renderer, 6, 1, False)
+    st = datagen_settings()
+    scenes = gen_scenes[:4]
+    first, _, _ = render_batch(renderer, scenes, SPP, st)
+    moved = []
+    for k, g in enumerate(scenes):
+        ftw = g.film_to_world.copy()
+        ftw[[3, 7, 11]] += np.float32(0.05 * (k + 1))  # translate the camera
+        moved.append(dataclasses.replace(g, film_to_world=ftw))
+    again, _, _ = render_batch(renderer, moved, SPP, st)
+    for g, a, b in zip(moved, first, again):
+        ref = _single(renderer, g, SPP, st)
+        assert np.array_equal(b.view(np.uint32), ref.view(np.uint32)), g.seed
+        assert not np.array_equal(a.view(np.uint32), b.view(np.uint32)), g.seed
+    # and back to the original cameras
+    back, _, _ = render_batch(renderer, scenes, SPP, st)
+    for a, b in zip(first, back):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
