@@ -169,6 +169,10 @@ template <class T> struct RenderArgs {
   // OutputPerStep (rendering_equation.h:56-123): radiance added by iteration
   // k goes to step image k; paths stop after n_steps iterations
   uint32_t n_steps = 0;
+  // First-hit G-buffer of sample 0 per pixel (12 floats: position, shading
+  // normal, albedo summary, emission; zeros on a miss), as the batched
+  // datagen renders write it.
+  T *gbuffer = nullptr;
 };
 
 template <class T> struct TableRngT {
@@ -235,8 +239,19 @@ V3<T> integrate_thread(const RenderArgs<T> &a, const ThreadInfo &info,
       }
       PathState<T> ns;
       Ray<T> nl = light_ray, nn = next_ray;
+      VertexInfo<T> vi{};
+      const bool want_g = a.gbuffer && sample == 0 && step == 0;
       const bool done = rendering_equation_iteration(st, rng, next_ray, light_hit, next_hit,
-                                                     a.settings, view, &ns, &nl, &nn);
+                                                     a.settings, view, &ns, &nl, &nn,
+                                                     want_g ? &vi : nullptr);
+      if (want_g && vi.valid) {
+        const Material<T> &m = view.materials[vi.material];
+        const V3<T> alb = material_albedo(m);
+        T *g = a.gbuffer + 12 * size_t(location);
+        const V3<T> vals[4] = {vi.p, vi.normal, alb, m.emission};
+        for (int k = 0; k < 4; ++k)
+          g[3 * k] = vals[k].x, g[3 * k + 1] = vals[k].y, g[3 * k + 2] = vals[k].z;
+      }
       if (per_step)
         steps[step] += ns.total;
       if (done) {
@@ -683,7 +698,7 @@ int oracle_adjoint(const float *tris, const float *normals, const uint32_t *tri_
 // out64 (double), x_dim*y_dim*3. Counts per render: rays traced, BVH nodes
 // visited and triangles tested by the reference traversal.
 int oracle_render_ex(const float *initial_rays, const uint32_t *initial_idx,
-                     uint32_t n_steps, const float *tris, const float *normals, const uint32_t *tri_mat,
+                     uint32_t n_steps, float *gbuffer, const float *tris, const float *normals, const uint32_t *tri_mat,
                   uint32_t n_tris, const pt_material *mats, uint32_t n_mats,
                   const float *light_tris, const float *light_cdf,
                   uint32_t n_lights, const float *film_to_world,
@@ -728,7 +743,8 @@ int oracle_render_ex(const float *initial_rays, const uint32_t *initial_idx,
     fill_scene(s, tris, normals, tri_mat, n_tris, mats, n_mats, light_tris,
                light_cdf, n_lights);
     RenderArgs<float> a{&s, is, wd, {}, sobol, dim_bound, init_bound,
-                        samples_per, settings->rng, initial_rays, initial_idx, n_steps};
+                        samples_per, settings->rng, initial_rays, initial_idx, n_steps,
+                        gbuffer};
     for (int i = 0; i < 12; ++i)
       a.film_to_world[i] = film_to_world[i];
     t0 = omp_get_wtime();
@@ -753,13 +769,151 @@ int oracle_render(const float *tris, const float *normals, const uint32_t *tri_m
                   uint64_t *counts /* 6: per kind (camera/continuation,
                                       light): rays, nodes, tris */,
                   double *seconds /* render time, scene setup excluded */) {
-  return oracle_render_ex(nullptr, nullptr, 0, tris, normals, tri_mat, n_tris, mats, n_mats,
+  return oracle_render_ex(nullptr, nullptr, 0, nullptr, tris, normals, tri_mat, n_tris, mats, n_mats,
                           light_tris, light_cdf, n_lights, film_to_world, settings,
                           samples_per, x_dim, y_dim, sobol, precision, n_threads,
                           pixel_begin, pixel_end, out32, out64, counts, seconds);
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+namespace {
+WorkDivisionSettings wd_settings(const pt_work_division_settings &w) {
+  WorkDivisionSettings ws;
+  ws.block_size = w.block_size;
+  ws.target_x_block_size = w.target_x_block_size;
+  ws.force_target_samples = w.force_target_samples;
+  ws.forced_target_samples_per_thread = w.forced_target_samples_per_thread;
+  ws.base_num_threads = w.base_num_threads;
+  ws.samples_per_thread_scaling_power = w.samples_per_thread_scaling_power;
+  ws.max_samples_per_thread = w.max_samples_per_thread;
+  return ws;
+}
+} // namespace
+
+// The reduction of the reference's reduce test (tests/src/reduce.cu:49-158):
+// every work-division thread sums in[sample + location * samples_per] over
+// its sample range in order, then the threads of a location are combined by
+// the lane reduction tree. out[location] (num_locations = x_dim, y_dim = 1).
+// Returns -1 on invalid settings or more than one sample block.
+int oracle_reduce_samples(const pt_work_division_settings *s, const float *in,
+                          uint32_t num_locations, uint32_t samples_per, float *out) {
+  WorkDivision wd;
+  if (!WorkDivision::make(wd_settings(*s), samples_per, num_locations, 1, &wd) ||
+      wd.num_sample_blocks != 1)
+    return -1;
+  std::vector<float> vals(wd.block_size);
+  for (uint32_t b = 0; b < wd.total_num_blocks(); ++b) {
+    for (uint32_t t = 0; t < wd.block_size; ++t) {
+      ThreadInfo info = wd.get_thread_info(b, t);
+      float total = 0.f;
+      if (!info.exit)
+        for (uint32_t i = info.start_sample; i < info.end_sample; ++i)
+          total += in[i + size_t(info.x) * samples_per];
+      vals[t] = total;
+    }
+    for (uint32_t t = 0; t < wd.block_size; t += wd.sample_block_size) {
+      ThreadInfo info = wd.get_thread_info(b, t);
+      if (!info.exit)
+        out[info.x] = tree_reduce(vals.data() + t, wd.sample_block_size);
+    }
+  }
+  return 0;
+}
+
+// The coverage walk of the reference's work-division test
+// (tests/src/work_division.cpp:24-80): threads in launch order must advance
+// through samples, then x within an x block, then y within a y block, then
+// the next block, with exit set exactly outside the image, and the dimension
+// check of :82-91. Returns 0, or the 1-based index of the first failing
+// check * 2^32 + thread ordinal (0 for the end-of-walk checks).
+uint64_t oracle_work_division_walk(const pt_work_division_settings *s, uint32_t samples_per,
+                                   uint32_t x_dim, uint32_t y_dim) {
+  WorkDivision d;
+  if (!WorkDivision::make(wd_settings(*s), samples_per, x_dim, y_dim, &d))
+    return uint64_t(100) << 32;
+  const uint32_t spp = d.num_sample_blocks * d.sample_block_size * d.base_samples_per_thread +
+                       d.n_threads_per_unit_extra;
+  const uint32_t ex = d.x_block_size * d.num_x_blocks, ey = d.y_block_size * d.num_y_blocks;
+  if (spp != samples_per)
+    return uint64_t(101) << 32;
+  if (!(ex >= x_dim && ey >= y_dim && ex < x_dim + d.x_block_size &&
+        ey < y_dim + d.y_block_size))
+    return uint64_t(102) << 32;
+  uint32_t samples_up_to = 0, x_up_to = 0, y_up_to = 0;
+  uint64_t ordinal = 0;
+  auto fail = [&](uint64_t code) { return (code << 32) | ordinal; };
+  for (uint32_t b = 0; b < d.total_num_blocks(); ++b) {
+    for (uint32_t t = 0; t < d.block_size; ++t, ++ordinal) {
+      const ThreadInfo info = d.get_thread_info(b, t);
+      if (bool(info.exit) != (info.x >= x_dim || info.y >= y_dim))
+        return fail(1);
+      if (info.start_sample != samples_up_to)
+        return fail(2);
+      if (info.x != x_up_to)
+        return fail(3);
+      if (info.y != y_up_to)
+        return fail(4);
+      samples_up_to = info.end_sample;
+      if (samples_up_to > spp)
+        return fail(5);
+      if (samples_up_to == spp && (t + 1) % d.sample_block_size == 0) {
+        ++x_up_to;
+        samples_up_to = 0;
+        if (x_up_to % d.x_block_size == 0) {
+          if (t == d.block_size - 1) {
+            if (x_up_to == ex) {
+              if ((b + 1) % d.num_x_blocks != 0)
+                return fail(6);
+              ++y_up_to;
+              x_up_to = 0;
+            } else {
+              if (y_up_to < d.y_block_size - 1)
+                return fail(7);
+              y_up_to -= d.y_block_size - 1;
+            }
+          } else {
+            ++y_up_to;
+            if (x_up_to < d.x_block_size)
+              return fail(8);
+            x_up_to -= d.x_block_size;
+          }
+        }
+      }
+      if (x_up_to > ex || y_up_to > ey)
+        return fail(9);
+    }
+  }
+  if (samples_up_to != 0 || x_up_to != 0 || y_up_to != ey)
+    return uint64_t(10) << 32;
+  return 0;
+}
+
+// FloatRGB -> BGRA32, restated from include/lib/bgra_32.h:12-31 and
+// include/lib/tone_map.h:7-10: per channel v / (v + 1) in float, times 255.0f,
+// converted to uint8 by truncation; bytes in memory order B, G, R, 0.
+void oracle_to_bgra(const float *rgb, uint32_t n, uint8_t *out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    uint8_t c[3];
+    for (int k = 0; k < 3; ++k) {
+      const float v = rgb[3 * size_t(i) + k];
+      const float mapped = v / (v + 1.0f);
+      const float scaled = mapped * 255.0f;
+      c[k] = static_cast<uint8_t>(scaled);
+    }
+    out[4 * size_t(i) + 0] = c[2];
+    out[4 * size_t(i) + 1] = c[1];
+    out[4 * size_t(i) + 2] = c[0];
+    out[4 * size_t(i) + 3] = 0;
+  }
+}
+
+// Values of the Uniform-rng stand-in for one seed (sample + location * spp).
+void oracle_hash_rng(uint32_t seed, uint32_t n, float *out) {
+  HashRng r = make_hash_rng(seed);
+  for (uint32_t i = 0; i < n; ++i)
+    out[i] = r.next();
+}
 
 namespace {
 struct ArrayRng {

@@ -37,7 +37,7 @@ def lib():
                                     _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                     C.c_uint32, C.c_uint32, _F, C.c_int, C.c_int, C.c_uint32,
                                     C.c_uint32, _F, _D, _U64, _D]
-        l.oracle_render_ex.argtypes = [_F, _U, C.c_uint32] + l.oracle_render.argtypes
+        l.oracle_render_ex.argtypes = [_F, _U, C.c_uint32, _F] + l.oracle_render.argtypes
         l.oracle_adjoint.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
                                      _F, _F, C.c_uint32, _F, C.POINTER(L.PtSettings), C.c_uint32,
                                      C.c_uint32, C.c_uint32, _F, _D, C.c_int, _D, _D, _D, _D]
@@ -51,6 +51,15 @@ def lib():
         l.oracle_term_prob.restype = C.c_double
         l.oracle_detmath.argtypes = [C.c_int, _F, _F, _F, C.c_uint32]
         l.oracle_detmath.restype = None
+        l.oracle_reduce_samples.argtypes = [C.POINTER(L.PtWorkDivisionSettings), _F, C.c_uint32,
+                                            C.c_uint32, _F]
+        l.oracle_work_division_walk.argtypes = [C.POINTER(L.PtWorkDivisionSettings), C.c_uint32,
+                                                C.c_uint32, C.c_uint32]
+        l.oracle_work_division_walk.restype = C.c_uint64
+        l.oracle_to_bgra.argtypes = [_F, C.c_uint32, C.POINTER(C.c_uint8)]
+        l.oracle_to_bgra.restype = None
+        l.oracle_hash_rng.argtypes = [C.c_uint32, C.c_uint32, _F]
+        l.oracle_hash_rng.restype = None
         _lib = l
     return _lib
 
@@ -107,13 +116,15 @@ def thread_infos(wds: L.PtWorkDivisionSettings, spp: int, x: int, y: int, max_th
 def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
            sobol: np.ndarray | None, precision: int = 32, threads: int = 0,
            pixel_begin: int = 0, pixel_end: int = 0, initial_rays=None, initial_idx=None,
-           n_steps: int = 0):
+           n_steps: int = 0, gbuffer: bool = False):
     """Returns (image (y, x, 3), info) with info = {"counts": [[rays, nodes,
     tris] for continuation/camera rays, same for light rays], "seconds":
     render time without scene setup}. With initial_rays ((n, 6): one ray per
     "pixel", x_dim = n, y_dim = 1; with initial_idx the first intersection
     is that triangle at distance 1) and / or n_steps (per-step images,
-    shape (n_steps, y, x, 3))."""
+    shape (n_steps, y, x, 3)). With gbuffer (float renders), info["gbuffer"]
+    holds the first hit of sample 0 per pixel, (y, x, 12): position, shading
+    normal, albedo summary, emission; zeros where the camera ray misses."""
     raw = settings.raw if hasattr(settings, "raw") else settings
     ftw = np.ascontiguousarray(film_to_world, np.float32)
     sob = np.zeros(1, np.float32) if sobol is None else np.ascontiguousarray(sobol, np.float32)
@@ -124,9 +135,11 @@ def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_
     out64 = np.zeros((n_img, y_dim, x_dim, 3), np.float64)
     rays = None if initial_rays is None else np.ascontiguousarray(initial_rays, np.float32)
     idx = None if initial_idx is None else np.ascontiguousarray(initial_idx, np.uint32)
+    gb = np.zeros((y_dim, x_dim, 12), np.float32) if gbuffer and precision == 32 else None
     rc = lib().oracle_render_ex(
         _f(rays) if rays is not None else None, _u(idx) if idx is not None else None, n_steps,
-        _f(flat["tris"]), _f(flat["normals"]), _u(flat["tri_mat"]), flat["tris"].shape[0],
+        _f(gb) if gb is not None else None, _f(flat["tris"]), _f(flat["normals"]),
+        _u(flat["tri_mat"]), flat["tris"].shape[0],
         flat["materials"], flat["n_materials"], _f(flat["light_tris"]), _f(flat["light_cdf"]),
         flat["light_tris"].shape[0], _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
         precision, threads, pixel_begin, pixel_end, _f(out32), out64.ctypes.data_as(_D),
@@ -134,6 +147,8 @@ def render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_
     if rc != 0:
         raise ValueError("oracle render failed")
     info = {"counts": counts.reshape(2, 3), "seconds": secs.value}
+    if gb is not None:
+        info["gbuffer"] = gb
     img = out64 if precision == 64 else out32
     return (img if n_steps else img[0]), info
 
@@ -162,6 +177,39 @@ def adjoint(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y
     if rc != 0:
         raise ValueError(f"oracle adjoint failed ({rc})")
     return ga, ge, img, secs.value
+
+
+def reduce_samples(wds: L.PtWorkDivisionSettings, values: np.ndarray, num_locations: int,
+                   samples_per: int) -> np.ndarray:
+    """Work-division reduction of values[sample + location * samples_per]
+    (one sample block): per-thread sequential sums, then the lane tree."""
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    assert v.size == num_locations * samples_per
+    out = np.zeros(num_locations, np.float32)
+    if lib().oracle_reduce_samples(C.byref(wds), _f(v), num_locations, samples_per, _f(out)):
+        raise ValueError("invalid work division or more than one sample block")
+    return out
+
+
+def work_division_walk(wds: L.PtWorkDivisionSettings, spp: int, x: int, y: int) -> tuple:
+    """The reference test's coverage walk: (failed check, thread ordinal),
+    (0, 0) if every check holds."""
+    r = int(lib().oracle_work_division_walk(C.byref(wds), spp, x, y))
+    return r >> 32, r & 0xFFFFFFFF
+
+
+def to_bgra(rgb: np.ndarray) -> np.ndarray:
+    """FloatRGB (..., 3) -> bytes (..., 4) in memory order B, G, R, 0."""
+    v = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(v.shape[:-1] + (4,), np.uint8)
+    lib().oracle_to_bgra(_f(v), v.size // 3, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def hash_rng(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().oracle_hash_rng(seed, n, _f(out))
+    return out
 
 
 def max_threads() -> int:

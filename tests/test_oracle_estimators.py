@@ -242,3 +242,102 @@ def test_direct_light_against_quadrature(case):
                      [material("Diffuse", diffuse=(0, 0, 0), emission=(le,) * 3)])
     img2, _ = O.render(sc2.flat(), ftw, s, 64, 1, 1, tab[:64], precision=64)
     assert img2.max() == 0.0
+
+
+FURNACE_CASES = [
+    # (term prob, light sampler, rng, dir sampler)
+    ("MultiplierFunc", "RandomTriangle", "Sobel", "BSDF"),
+    ("MultiplierFunc", "NoLightSampling", "Sobel", "BSDF"),
+    ("Constant", "RandomTriangle", "Sobel", "BSDF"),
+    ("Constant", "NoLightSampling", "Sobel", "BSDF"),
+    ("Normalize", "RandomTriangle", "Sobel", "BSDF"),
+    ("Normalize", "NoLightSampling", "Sobel", "BSDF"),
+    ("MultiplierFunc", "RandomTriangle", "Uniform", "BSDF"),
+    ("MultiplierFunc", "RandomTriangle", "Sobel", "Uniform"),
+    ("Constant", "NoLightSampling", "Uniform", "Uniform"),
+]
+
+
+@pytest.mark.parametrize("term,light,rng,dirs", FURNACE_CASES)
+def test_white_furnace_unbounded_depth(term, light, rng, dirs):
+    """Russian roulette at unbounded depth (rendering_equation_iteration.h:
+    130-150: continue iff u > p, multiplier /= 1 - p) with each termination
+    rule of include/integrate/term_prob/*: inside a closed emissive diffuse
+    box the expected radiance is the Neumann series Le / (1 - rho), with or
+    without light sampling, for Sobol or Uniform draws and BSDF or uniform
+    hemisphere directions. Dropping the 1 / (1 - p) reweight would give
+    Le / (1 - rho (1 - p)) instead (2.17 for MultiplierFunc's p = 0.1, 1.43
+    for Constant 0.5, against 2.5). FP64 oracle."""
+    from path_amd.renderer import camera_from_look
+    rho, le = 0.6, 1.0
+    sc = _furnace_scene(rho, le)
+    ftw = camera_from_look([0, 0, -1], [0, 1, 0], [0, 0, 0.2], 60.0, 1.0)
+    s = Settings()
+    s.light_sampler = light
+    s.term_prob = term
+    s.constant_prob = 0.5
+    s.rng = rng
+    s.dir_sampler = dirs
+    spp = 2048
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp) if rng == "Sobel" else None
+    img, _ = O.render(sc.flat(), ftw, s, spp, 8, 8, tab, precision=64)
+    expected = le / (1.0 - rho)
+    assert abs(img.mean() - expected) / expected < 0.01, (img.mean(), expected)
+
+
+def test_multiplier_func_reweight_matters():
+    """The unbounded furnace separates the reweighted estimator from one
+    without the 1 / (1 - p) factor: with p = 0.1 constant the two expected
+    values differ by 13%, far outside the furnace bound."""
+    rho, p = 0.6, 0.1
+    with_reweight = 1.0 / (1.0 - rho)
+    without = 1.0 / (1.0 - rho * (1.0 - p))
+    assert abs(without - with_reweight) / with_reweight > 0.1
+
+
+def test_normalize_term_prob():
+    """Normalize (normalize.h:18-23): 1 - 1 / mean(multiplier), clamped to
+    [min_prob, 1]."""
+    s = Settings()
+    s.term_prob = "Normalize"
+    s.normalize_min_prob = 0.1
+    assert abs(O.term_prob(s, 0, [2.0, 4.0, 6.0]) - 0.75) < 1e-7   # mean 4
+    assert O.term_prob(s, 0, [0.5, 0.5, 0.5]) == pytest.approx(0.1)  # 1 - 2 < min
+    assert O.term_prob(s, 3, [1.0, 1.0, 1.0]) == pytest.approx(0.1)  # 0 < min
+    assert O.term_prob(s, 0, [1e9, 1e9, 1e9]) == pytest.approx(1.0, abs=1e-6)
+
+
+def test_bgra_tone_map_closed_form():
+    """bgra_32.h:12-31 + tone_map.h:7-10: channel = trunc(255 v / (v + 1)),
+    bytes B, G, R, 0."""
+    rgb = np.array([[0.0, 1.0, 3.0],       # 0, 127.5 -> 127, 191.25 -> 191
+                    [0.25, 0.5, 9.0],      # 51, 85, 229.5 -> 229
+                    [1e30, 0.0, 0.01]],    # 255, 0, 2.52 -> 2
+                   np.float32)
+    out = O.to_bgra(rgb)
+    assert out.tolist() == [[191, 127, 0, 0], [229, 85, 51, 0], [2, 0, 255, 0]]
+    # monotone in v and never above 255
+    v = np.linspace(0, 50, 20001, dtype=np.float32)
+    g = O.to_bgra(np.stack([v, v, v], 1))[:, 1].astype(int)
+    assert (np.diff(g) >= 0).all() and g.max() <= 255 and g[0] == 0
+
+
+def test_uniform_rng_statistics():
+    """The Uniform rng stand-in (the reference's cuRAND XORWOW stream cannot
+    be reproduced; parity of its values is unpinned): values lie in [0, 1),
+    are uniform (Kolmogorov-Smirnov), uncorrelated between consecutive draws
+    and between neighbouring seeds, and each seed gives its own stream."""
+    from scipy import stats
+    n_seeds, n = 512, 64
+    v = np.stack([O.hash_rng(seed, n) for seed in range(n_seeds)])
+    assert v.min() >= 0.0 and v.max() < 1.0
+    assert stats.kstest(v.ravel(), "uniform").pvalue > 1e-3
+    # first draws of consecutive seeds (adjacent samples of a pixel)
+    assert stats.kstest(v[:, 0], "uniform").pvalue > 1e-3
+    for a, b in ((v[:, :-1].ravel(), v[:, 1:].ravel()), (v[:-1].ravel(), v[1:].ravel())):
+        assert abs(np.corrcoef(a, b)[0, 1]) < 0.02
+    # 2-D stratification of consecutive pairs (chi-square over 16 x 16 cells)
+    h, _, _ = np.histogram2d(v[:, 0::2].ravel(), v[:, 1::2].ravel(), bins=16,
+                             range=[[0, 1], [0, 1]])
+    assert stats.chisquare(h.ravel()).pvalue > 1e-3
+    assert len({tuple(r[:4]) for r in v}) == n_seeds

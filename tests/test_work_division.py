@@ -60,3 +60,25 @@ def test_default_settings():
     assert wd["num_x_blocks"] * wd["num_y_blocks"] * wd["num_sample_blocks"] == 512
     wd = work_division(settings(), 64, 512, 512)
     assert wd["sample_block_size"] == 2 and wd["base_samples_per_thread"] == 32
+
+
+@pytest.mark.parametrize("x", [1, 3, 17, 72, 128])
+@pytest.mark.parametrize("y", [1, 3, 17, 72, 128])
+def test_reference_coverage_walk(x, y):
+    """The reference's own walk (tests/src/work_division.cpp:24-80, grid
+    :93-105): threads in launch order advance through a pixel's samples, then
+    x inside the x block, then y inside the y block, then the next block;
+    exit is set exactly outside the image; the division covers samples_per
+    and the image with less than one block of padding (:82-91)."""
+    s = settings(block_size=64, target_x_block_size=16, force_target_samples=1,
+                 forced_target_samples_per_thread=2)
+    for spp in (1, 3, 17, 72, 128, 2048):
+        assert O.work_division_walk(s, spp, x, y) == (0, 0), (spp, x, y)
+
+
+def test_reference_coverage_walk_default_settings():
+    assert O.work_division_walk(settings(), 2048, 8, 8) == (0, 0)
+    # the render configurations of the benchmark workloads
+    for spp, size in ((16, 256), (64, 512), (128, 128), (128, 32), (32, 256)):
+        assert O.work_division_walk(settings(), spp, size, size) == (0, 0), (spp, size)
+
