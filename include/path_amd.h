@@ -244,6 +244,17 @@ int pt_sobol_direction_numbers(uint32_t dim_bound, uint32_t *out);
  * num_x_blocks, num_y_blocks, x_dim, y_dim. */
 int pt_work_division(const pt_work_division_settings *s, uint32_t samples_per,
                      uint32_t x_dim, uint32_t y_dim, uint32_t out[11]);
+/* The renderer's per-pixel sample reduction on given values: each
+ * work-division thread sums values[sample + location * samples_per] over its
+ * samples in order, then a location's threads are combined by the same lane
+ * reduction the mega-kernel uses; out[location] for num_locations x 1
+ * "pixels". The division must have one sample block. Replaces the reduction
+ * the reference's reduce test drives (RuntimeConstantsReducer,
+ * include/kernel/runtime_constants_reducer.h:24-133, test
+ * tests/src/reduce.cu:49-158). */
+int pt_reduce_samples(pt_renderer *r, const pt_work_division_settings *s,
+                      const float *values, uint32_t num_locations, uint32_t samples_per,
+                      float *out);
 /* Host SAH BVH of the flattened scene in the reference node layout
  * (node.h:22-126): per node lo[3], hi[3] (float) and is_leaf, a, b (uint32);
  * perm[n_tris]. This is the binary tree the GPU's 4-wide tree is collapsed

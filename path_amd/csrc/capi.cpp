@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 using namespace pt;
 using namespace pt::host;
@@ -376,6 +377,43 @@ int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
   if (!r || !out)
     return fail(PT_ERR_INVALID, "null argument");
   return r->sobol_table_to_host(dim_bound, count, out);
+  PT_GUARD_END
+}
+
+int pt_reduce_samples(pt_renderer *r, const pt_work_division_settings *s,
+                      const float *values, uint32_t num_locations, uint32_t samples_per,
+                      float *out) {
+  PT_GUARD_BEGIN
+  if (!r || !s || !values || !out)
+    return fail(PT_ERR_INVALID, "null argument");
+  if (num_locations == 0 || samples_per == 0)
+    return fail(PT_ERR_INVALID, "empty reduction");
+  WorkDivision d;
+  if (!WorkDivision::make(work_division_settings(*s), samples_per, num_locations, 1, &d))
+    return fail(PT_ERR_INVALID, "invalid work division settings");
+  if (d.num_sample_blocks != 1)
+    return fail(PT_ERR_INVALID, "the reduction needs one sample block per location");
+  if (d.block_size > 1024)
+    return fail(PT_ERR_INVALID, "block_size above 1024");
+  const size_t n = size_t(num_locations) * samples_per;
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = r->scratch_t.upload(values, n, r->stream);
+  if (e == hipSuccess)
+    e = r->rgb.ensure(size_t(num_locations) * 3);
+  if (e == hipSuccess)
+    e = gpu::launch_reduce_samples(d, r->scratch_t.ptr, samples_per, r->rgb.ptr, r->stream);
+  std::vector<float> rgb(size_t(num_locations) * 3);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rgb.data(), r->rgb.ptr, rgb.size() * sizeof(float),
+                       hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("reduce: ") + hipGetErrorString(e));
+  for (uint32_t i = 0; i < num_locations; ++i)
+    out[i] = rgb[3 * size_t(i)];
+  return PT_OK;
   PT_GUARD_END
 }
 

@@ -102,6 +102,10 @@ hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
 hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
                                        uint32_t n_pixels, uint32_t samples_per,
                                        float *out, hipStream_t s);
+// Per-location reduction of per-sample values over the work division (one
+// sample block): out_rgb[3 * (location * num_sample_blocks + block)].x
+hipError_t launch_reduce_samples(const WorkDivision &wd, const float *in, uint32_t samples_per,
+                                 float *out_rgb, hipStream_t s);
 hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
                           hipStream_t s);
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s);

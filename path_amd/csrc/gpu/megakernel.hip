@@ -382,6 +382,40 @@ hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
   return hipGetLastError();
 }
 
+// The mega-kernel's per-pixel reduction applied to given per-sample values
+// (the reduction of the reference's reduce test, tests/src/reduce.cu:49-158):
+// each work-division thread sums in[sample + location * samples_per] over its
+// samples in order, then reduce_store combines a location's threads.
+__global__ __launch_bounds__(1024) void k_reduce_samples(MegaParams p,
+                                                         const float *__restrict__ in,
+                                                         uint32_t start_block) {
+  const uint32_t block = blockIdx.x + start_block;
+  const ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
+  V3<float> total = {0.f, 0.f, 0.f};
+  if (!info.exit) {
+    const size_t base = (size_t(info.x) + size_t(info.y) * p.wd.x_dim) * p.rng.samples_per;
+    for (uint32_t i = info.start_sample; i < info.end_sample; ++i)
+      total.x += in[base + i];
+  }
+  reduce_store(p, info, block, total, p.out_rgb);
+}
+
+hipError_t launch_reduce_samples(const WorkDivision &wd, const float *in, uint32_t samples_per,
+                                 float *out_rgb, hipStream_t s) {
+  if (wd.block_size == 0 || wd.block_size > 1024 || wd.block_size % 64 != 0)
+    return hipErrorInvalidConfiguration;
+  MegaParams p{};
+  p.wd = wd;
+  p.rng.samples_per = samples_per;
+  p.out_rgb = out_rgb;
+  p.divide_by_spp = false;
+  const uint32_t total = wd.total_num_blocks();
+  if (total == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_reduce_samples, dim3(total), dim3(wd.block_size), 0, s, p, in, 0u);
+  return hipGetLastError();
+}
+
 __global__ void k_reduce_sample_blocks(const float *__restrict__ in, uint32_t nsb,
                                        uint32_t n_pixels, uint32_t spp,
                                        float *__restrict__ out) {

@@ -1,0 +1,216 @@
+"""GPU parity on the benchmark workloads themselves, through the C ABI.
+
+* The 82k-triangle scene (Cornell box + level-6 icosphere, seed 0; diffuse,
+  Phong n = 50 and glass by triangle index mod 3) at the depth it is
+  benchmarked with (NIters 7 = 8 vertices): closest hits against brute force
+  and images of both integrators against the float CPU checker, bit for bit.
+* Batched datagen scenes (generated from seeds, including the one with the
+  most triangles of the first 1024 and the smallest) at 128 spp and NIters 5,
+  rendered in one batch: every image and first-hit G-buffer against the CPU
+  checker's, bit for bit.
+* The reference's own configs/benchmark.yaml (Uniform rng, MultiplierFunc
+  roulette) on the reference's scene file.
+* BGRA output, launch chunking and the per-pixel reduction's known answers.
+
+reference: include/integrate/rendering_equation.h:27-129 (path loop),
+src/generate_data/full_scene/generate_data.cpp:280-294 (one render per scene).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import random_rays, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def l6_scene(tmp_path_factory):
+    from path_amd import scenes
+    from path_amd.renderer import Scene
+    d = tmp_path_factory.mktemp("l6")
+    scene, ftw = Scene.load(scenes.cornell_icosphere(d, level=6, seed=0), 1.0)
+    assert scene.counts()[0] == 81956
+    return scene, ftw
+
+
+def _sobol(spp, dims=256):
+    from oracle import oracle as O
+    return O.sobol_table(O.joe_kuo_directions(dims), dims, spp)
+
+
+def test_l6_intersect_matches_loopall(renderer, l6_scene):
+    from oracle import oracle as O
+    scene, _ = l6_scene
+    flat = scene.flat()
+    rays = random_rays(20000, -1.2, 2.2, seed=17)
+    # half of the rays start on the sphere, pointing outward or inward, so
+    # grazing and inside-the-mesh cases are covered
+    tris = flat["tris"].reshape(-1, 3, 3)
+    pick = np.random.default_rng(5).choice(len(tris), 10000, replace=False)
+    c = tris[pick, 0] + (tris[pick, 1] + tris[pick, 2]) / 3.0
+    rays[:10000, :3] = c + 1e-3 * rays[:10000, 3:]
+    t, idx, back = renderer.intersect(scene, rays)
+    rt, ridx, rback = O.loopall(flat["tris"], rays)
+    assert np.array_equal(idx, ridx)
+    hit = ridx != 0xFFFFFFFF
+    assert hit.sum() > 15000
+    assert np.array_equal(t[hit].view(np.uint32), rt[hit].view(np.uint32))
+    assert np.array_equal(back[hit], rback[hit])
+
+
+@pytest.mark.parametrize("approach", ["MegaKernel", "Streaming"])
+def test_l6_images_bit_exact_at_benchmark_depth(renderer, l6_scene, approach):
+    """The benchmarked scene and depth (NIters 7), 64x64 at 8 spp: long
+    glass and glossy paths, traversal stacks deeper than the LDS part."""
+    from oracle import oracle as O
+    from path_amd.renderer import Settings
+    scene, ftw = l6_scene
+    s = Settings()
+    s.kernel_approach = approach
+    s.term_prob = "NIters"
+    s.n_iters = 7
+    size, spp = 64, 8
+    img, _ = renderer.render(scene, ftw, size, size, spp, s)
+    tab = _sobol(spp)
+    ref32, info = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=32)
+    diff = np.abs(img.astype(np.float64) - ref32)
+    assert np.array_equal(img.view(np.uint32), ref32.view(np.uint32)), (
+        f"max |diff| {diff.max()} at {np.unravel_index(diff.argmax(), diff.shape)}")
+    assert img.mean() > 0
+    # paths reach the full depth: more continuation rays than 3 per sample
+    assert info["counts"][0][0] > 3 * size * size * spp
+    ref64, _ = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=64)
+    assert rel_rmse(img, ref64) < 2e-2
+
+
+def test_l6_streaming_equals_megakernel_full_size(renderer, l6_scene):
+    """At the benchmark's full size (512x512, 64 spp, NIters 7) the
+    wavefront image equals the mega-kernel image bit for bit (the CPU checker
+    is too slow for the whole image; the test above pins both against it on
+    the same scene and depth)."""
+    from path_amd.renderer import Settings
+    scene, ftw = l6_scene
+    s = Settings()
+    s.term_prob = "NIters"
+    s.n_iters = 7
+    s.kernel_approach = "Streaming"
+    wf, _ = renderer.render(scene, ftw, 512, 512, 64, s)
+    s.kernel_approach = "MegaKernel"
+    mk, _ = renderer.render(scene, ftw, 512, 512, 64, s)
+    assert np.array_equal(wf.view(np.uint32), mk.view(np.uint32))
+
+
+# 348: most triangles of seeds 0-1023 (19,942); 890: 19,338 triangles at
+# 128x128; 22: fewest triangles (26); the rest ordinary
+C3_SEEDS = [348, 890, 90, 22, 319, 0]
+
+
+def test_batched_datagen_scenes_bit_exact(renderer):
+    """Datagen workload settings (Streaming, 128 spp, NIters 5 = depth 6),
+    six scenes in one batch: images and G-buffers equal the CPU checker's."""
+    from oracle import oracle as O
+    from path_amd.datagen import (datagen_settings, generate_scene, render_batch,
+                                  set_launch_params)
+    gens = [generate_scene(s) for s in C3_SEEDS]
+    assert max(g.scene.counts()[0] for g in gens) == 19942
+    st = datagen_settings()
+    st.term_prob = "NIters"
+    st.n_iters = 5
+    spp = 128
+    set_launch_params(renderer, 4, 1, False)
+    images, gbufs, _ = render_batch(renderer, gens, spp, st)
+    tab = _sobol(spp)
+    for g, img, gb in zip(gens, images, gbufs):
+        r = g.resolution
+        ref, info = O.render(g.scene.flat(), g.film_to_world, st, spp, r, r, tab, 32,
+                             gbuffer=True)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), g.seed
+        assert np.array_equal(gb.view(np.uint32), info["gbuffer"].view(np.uint32)), g.seed
+        assert img.mean() > 0 and (gb[..., 3:6] != 0).any()
+
+
+def test_reference_benchmark_yaml_bit_exact(renderer):
+    """The reference's configs/benchmark.yaml (MegaKernel, SBVH, RandomTriangle,
+    BSDF, MultiplierFunc exp 50 / min 0.1, Uniform rng, back-face culled
+    emission) on the reference's cornell-diffuse scene: both integrators
+    equal the CPU checker bit for bit."""
+    from oracle import oracle as O
+    from path_amd.renderer import Scene, Settings
+    s = Settings.from_yaml(GOLDEN / "configs" / "benchmark.yaml")
+    assert s.rng == 0 and s.term_prob == 2 and s.kernel_approach == 0
+    scene, ftw = Scene.load(GOLDEN / "scenes" / "cornell-diffuse.xml", 1.0)
+    size, spp = 48, 16
+    ref, _ = O.render(scene.flat(), ftw, s, spp, size, size, None, precision=32)
+    mk, _ = renderer.render(scene, ftw, size, size, spp, s)
+    assert np.array_equal(mk.view(np.uint32), ref.view(np.uint32))
+    s.kernel_approach = "Streaming"
+    wf, _ = renderer.render(scene, ftw, size, size, spp, s)
+    assert np.array_equal(wf.view(np.uint32), ref.view(np.uint32))
+    assert ref.mean() > 0
+
+
+@pytest.mark.parametrize("approach", ["MegaKernel", "Streaming"])
+def test_bgra_output_is_tone_mapped_float_rgb(renderer, cornell, approach):
+    """PT_OUTPUT_BGRA equals trunc(255 v / (v + 1)) of the FloatRGB render,
+    byte for byte, B, G, R, 0 in memory (bgra_32.h:12-31, tone_map.h:7-10)."""
+    from oracle import oracle as O
+    from path_amd.renderer import Settings
+    scene, ftw = cornell
+    s = Settings()
+    s.kernel_approach = approach
+    rgb, _ = renderer.render(scene, ftw, 40, 24, 16, s)
+    bgra, _ = renderer.render(scene, ftw, 40, 24, 16, s, output="BGRA")
+    got = np.ascontiguousarray(bgra).view(np.uint8).reshape(24, 40, 4)
+    assert np.array_equal(got, O.to_bgra(rgb))
+    assert got[..., 3].max() == 0 and got[..., :3].max() > 100
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 7])
+def test_megakernel_launch_chunks(renderer, cornell_sphere, chunk):
+    """max_blocks_per_launch_gpu smaller than the grid (the render runs as
+    several launches, kernel_launch_impl_gpu.cuh:12-33): same image as one
+    launch and as the CPU checker."""
+    from oracle import oracle as O
+    from path_amd.renderer import Settings
+    scene, ftw = cornell_sphere
+    s = Settings()
+    s.term_prob = "NIters"
+    s.n_iters = 4
+    size, spp = 40, 8
+    whole, _ = renderer.render(scene, ftw, size, size, spp, s)
+    launches_whole = renderer.last_stats().mega_launches
+    s.max_blocks_per_launch_gpu = chunk
+    parts, _ = renderer.render(scene, ftw, size, size, spp, s)
+    assert renderer.last_stats().mega_launches > launches_whole
+    ref, _ = O.render(scene.flat(), ftw, s, spp, size, size, _sobol(spp), precision=32)
+    assert np.array_equal(parts.view(np.uint32), whole.view(np.uint32))
+    assert np.array_equal(parts.view(np.uint32), ref.view(np.uint32))
+
+
+def test_reduce_known_answers_on_gpu(renderer):
+    """The reference's reduce test (tests/src/reduce.cu:49-158) on the GPU
+    reduction: integer values sum exactly, uniform values to 4 float ulps
+    of the brute-force sum, and the GPU equals the CPU checker bit for bit."""
+    from oracle import oracle as O
+    from tests.test_oracle_reduce import cases, one_sample_block, ulps
+    rng = np.random.default_rng(99)
+    n = 0
+    for n_loc, spp, bs, base in cases():
+        s = one_sample_block(bs, spp, n_loc, base)
+        for kind in ("int", "uniform"):
+            if kind == "int":
+                vals = rng.integers(-100, 101, n_loc * spp).astype(np.float32)
+            else:
+                vals = rng.uniform(0, 1, n_loc * spp).astype(np.float32)
+            got = renderer.reduce_samples(s, vals, n_loc, spp)
+            expect = vals.astype(np.float64).reshape(n_loc, spp).sum(1).astype(np.float32)
+            if kind == "int":
+                assert np.array_equal(got, expect), (n_loc, spp, bs, base)
+            else:
+                assert ulps(got, expect).max() <= 4, (n_loc, spp, bs, base)
+            assert np.array_equal(got.view(np.uint32),
+                                  O.reduce_samples(s, vals, n_loc, spp).view(np.uint32))
+            n += 1
+    assert n > 600
