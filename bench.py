@@ -64,40 +64,58 @@ CONFIGS = {
 MLP_MACS = 225 * 256 + 5 * 256 * 256 + 256 * 3   # per query, valid (unpadded) dims
 
 
-def _roofline(name, ms, launches, n_rays, bpr, config):
+def _roofline(name, ms, launches, n_rays, bpr, config, step_bytes=None, step_ms=None,
+              pipelines=1):
+    """Roofline of the dominant traversal kernel.
+
+    Top level (`achieved`, `frac`): per GPU - the algorithmic bytes of every
+    traversal launch of one step (step_bytes) over the measured step time, so
+    overlapping launches cannot count the same time twice. `per_launch`: the
+    dominant kernel's algorithmic bytes per launch over its average launch
+    duration (HIP events on the launch's stream), the figure the rocprofv3
+    kernel trace in profiles/ checks. `traffic`: HBM bytes per launch of that
+    kernel from the FETCH_SIZE / WRITE_SIZE counters, and `binding`: the
+    counters that show what limits it when the algorithmic figure exceeds the
+    bandwidth (it is not a bound then)."""
     avg_s = ms / 1e3 / max(launches, 1)
     bytes_per_launch = bpr * n_rays / max(launches, 1)
-    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    traffic = None
+    per_launch_gbps = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    if step_bytes is None or not step_ms:
+        step_bytes, step_ms = bytes_per_launch, avg_s * 1e3
+    achieved = step_bytes / (step_ms / 1e3) / 1e9
+    prof = {}
     tf = ROOT / "profiles" / f"{name}_{config}_traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_dispatch")
+        prof = json.loads(tf.read_text())
+    traffic = prof.get("bytes_per_dispatch")
     out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
            "traffic": traffic, "kernel": name,
-           "avg_launch_ms": round(avg_s * 1e3, 4),
-           "algorithmic_bytes_per_ray": round(bpr, 1),
-           "rays_per_launch": round(n_rays / max(launches, 1), 1)}
+           "scope": "per GPU: algorithmic bytes of every traversal launch of one step / ms_per_step",
+           "algorithmic_bytes_per_step": round(step_bytes),
+           "per_launch": {"avg_launch_ms": round(avg_s * 1e3, 4),
+                          "algorithmic_bytes_per_ray": round(bpr, 1),
+                          "rays_per_launch": round(n_rays / max(launches, 1), 1),
+                          "achieved": round(per_launch_gbps, 2),
+                          "frac": round(per_launch_gbps / HBM_PEAK_GBPS, 5),
+                          "concurrent_pipelines": pipelines}}
+    if pipelines > 1:
+        out["per_launch"]["note"] = (
+            f"{pipelines} pipelines run at once and their launches overlap; a launch's "
+            "duration covers its share of the rays while the others run")
     if traffic and avg_s > 0:
         # what the memory system actually moved (scaled FETCH + WRITE counters)
         out["measured_traffic_gbps"] = round(traffic / avg_s / 1e9, 2)
         out["measured_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBPS, 5)
+    if prof.get("counters"):
+        out["binding"] = prof["counters"]
     if out["frac"] > 1.0:
-        out["note"] = ("algorithmic bytes follow the reference's binary-BVH traversal "
-                       "(target_objects = 4), which visits more nodes than the 4-wide "
-                       "near-first traversal here, and most reads hit L2 / Infinity Cache: "
-                       "frac > 1 is not a bound; measured_frac is the HBM-side figure")
+        out["note"] = ("frac > 1: the algorithmic bytes (the reference's binary-BVH AoS "
+                       "traversal, target_objects = 4) exceed what HBM could deliver in the "
+                       "step, because the 4-wide tree visits fewer nodes and its working set "
+                       "stays in L2 / Infinity Cache; HBM is not the limit (measured_frac), "
+                       "the kernel is bound by issue and latency (binding)")
     return out
-
-
-def _note_pipelines(roof, n):
-    """Streaming renders run n pipelines whose launches overlap: the average
-    launch duration is measured under that contention."""
-    roof["concurrent_pipelines"] = n
-    if n > 1:
-        roof["pipelines_note"] = (f"{n} pipelines run at once and their launches overlap; "
-                                  "avg_launch_ms covers a launch's share of the rays while "
-                                  "the others run, so achieved is per launch, not per GPU")
 
 
 def _bytes_per_ray(counts, kind):
@@ -136,24 +154,49 @@ def bench_datagen(a, cfg):
         tuned = autotune(r, gens[: min(n, 128)], spp, settings)
     for _ in range(max(a.warmup - 1, 0)):
         render_batch(r, gens, spp, settings)
-    # images and G-buffers stay in HBM (one device buffer each, scenes in
-    # batch order) and the scenes stay resident between batches, so the timed
-    # steps hold no PCIe transfer; the host-output rate is reported beside it
-    dev = render_batch_device(r, gens, spp, settings)
+    # Timed steps: every step renders a batch of freshly generated scene
+    # objects (the same seeds, regenerated outside the timed region), so the
+    # step holds what datagen with new scenes costs: flattening, packing,
+    # upload, the GPU tree builds of all scenes, the render, and the images
+    # and G-buffers left in HBM (one device buffer each, batch order).
+    from path_amd.datagen import last_bvh_build, set_bvh_builder
     sync = _sync_fn()
+    set_bvh_builder(r, True)
+    dev = render_batch_device(r, [generate_scene(seed) for seed in range(n)], spp, settings)
     sync()
-    t0 = time.perf_counter()
-    rays = 0
-    kernel_ms = 0.0
-    for _ in range(a.steps):
-        dev = render_batch_device(r, gens, spp, settings, out=dev)
+    fresh_s, rays, kernel_ms, gen_step_s = [], 0, 0.0, []
+    fresh_mismatch = 0
+    for step in range(a.steps):
+        t0 = time.perf_counter()
+        batch = [generate_scene(seed) for seed in range(n)]
+        gen_step_s.append(time.perf_counter() - t0)
+        sync()
+        t0 = time.perf_counter()
+        dev = render_batch_device(r, batch, spp, settings, out=dev)
+        sync()
+        fresh_s.append(time.perf_counter() - t0)
         st = r.last_stats()
         rays += st.rays
         kernel_ms += st.kernel_ms
-    sync()
-    elapsed = time.perf_counter() - t0
+        del batch
+    tree_ms, tree_levels = last_bvh_build(r)
+    elapsed = sum(fresh_s)
     value = n * a.steps / elapsed
     total_px = sum(g.resolution ** 2 for g in gens)
+    fresh_rgb = dev[0].numpy((total_px, 3)).copy()
+    fresh_gb = dev[1].numpy((total_px, 12)).copy()
+
+    # the same scenes kept resident between batches (no packing, upload or
+    # tree builds): a re-render rate, reported beside the headline
+    set_bvh_builder(r, False)
+    render_batch(r, gens, spp, settings)
+    dev = render_batch_device(r, gens, spp, settings, out=dev)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dev = render_batch_device(r, gens, spp, settings, out=dev)
+    sync()
+    resident_value = n * a.steps / (time.perf_counter() - t0)
     t1 = time.perf_counter()
     for _ in range(a.steps):
         images, gbufs, _ = render_batch(r, gens, spp, settings)
@@ -163,35 +206,26 @@ def bench_datagen(a, cfg):
     dev_mismatch = 0
     for i, g in enumerate(gens):
         o, px = int(dev[2][i]), g.resolution ** 2
-        dev_mismatch += not (np.array_equal(dev_rgb[o:o + px].view(np.uint32),
-                                            images[i].reshape(-1, 3).view(np.uint32)) and
-                             np.array_equal(dev_gb[o:o + px].view(np.uint32),
-                                            gbufs[i].reshape(-1, 12).view(np.uint32)))
+        want_rgb = images[i].reshape(-1, 3).view(np.uint32)
+        want_gb = gbufs[i].reshape(-1, 12).view(np.uint32)
+        dev_mismatch += not (np.array_equal(dev_rgb[o:o + px].view(np.uint32), want_rgb) and
+                             np.array_equal(dev_gb[o:o + px].view(np.uint32), want_gb))
+        # fresh batches with GPU-built trees against host trees and host output
+        fresh_mismatch += not (np.array_equal(fresh_rgb[o:o + px].view(np.uint32), want_rgb) and
+                               np.array_equal(fresh_gb[o:o + px].view(np.uint32), want_gb))
     assert dev_mismatch == 0, f"{dev_mismatch} device-output images differ from host output"
+    assert fresh_mismatch == 0, f"{fresh_mismatch} fresh-batch images differ"
 
-    # new scenes every batch: the trees must be built each time, on host
-    # threads (SAH, the default) or on the GPU (LBVH); generation excluded
-    from path_amd.datagen import last_bvh_build, set_bvh_builder
-    fresh = {}
-    for gpu_tree in (False, True, False, True):
+    # new scenes with host-built SAH trees (16 host threads) instead
+    host_tree_s = []
+    for _ in range(2):
         batch = [generate_scene(seed) for seed in range(n)]
-        set_bvh_builder(r, gpu_tree)
         sync()
         t0 = time.perf_counter()
-        imgs_f, _, _ = render_batch(r, batch, spp, settings)
+        render_batch_device(r, batch, spp, settings, out=dev)
         sync()
-        secs = time.perf_counter() - t0
-        key = "gpu_lbvh" if gpu_tree else "host_sah"
-        fresh.setdefault(key, []).append(secs)
-        if gpu_tree:
-            tree_ms, tree_levels = last_bvh_build(r)
-            fresh["gpu_tree_ms"] = tree_ms
-            fresh["gpu_tree_levels"] = tree_levels
-            fresh["gpu_tree_images_differing"] = sum(
-                not np.array_equal(x.view(np.uint32), y.view(np.uint32))
-                for x, y in zip(imgs_f, images))
+        host_tree_s.append(time.perf_counter() - t0)
         del batch
-    set_bvh_builder(r, False)
 
     # per-scene launches: the same scenes, one render call each
     single = settings.copy()
@@ -240,31 +274,30 @@ def bench_datagen(a, cfg):
                    "triangles_min": min(g.scene.counts()[0] for g in gens),
                    "triangles_max": max(g.scene.counts()[0] for g in gens),
                    "integrator": "Streaming", "sampler": "Sobel", "gbuffer": True},
+        "step": "a batch of newly created scene objects every step (generation outside the "
+                "timed region, reported below): flattening, packing, upload, GPU tree builds "
+                "of all scenes, render; images and G-buffers left in HBM",
         "msamples_per_s": round(total_px * spp * a.steps / elapsed / 1e6, 2),
         "mrays_per_s": round(rays / elapsed / 1e6, 2),
         "gpu_ms_per_batch": round(kernel_ms / a.steps, 2),
+        "gpu_tree_build_ms": round(tree_ms, 2),
+        "gpu_tree_levels": tree_levels,
+        "images_differing_from_host_trees": fresh_mismatch,
         "per_scene_launches": {"value": round(per_scene_value, 2), "unit": "scenes/s",
                                "seconds": round(per_scene_s, 3),
-                               "images_differing_from_batched": mismatched},
+                               "images_differing_from_batched": mismatched,
+                               "note": "one render call per scene; each call uploads its scene, "
+                                       "whose host SAH tree was built once before timing"},
         "speedup_vs_per_scene": round(value / per_scene_value, 2),
-        "output": "images and G-buffers left in HBM; the batch's scenes stay resident between "
-                  "batches (per-scene launches upload each scene, as one Renderer::render per "
-                  "scene does)",
-        "host_output_scenes_per_s": round(host_value, 2),
+        "resident_batch_scenes_per_s": round(resident_value, 2),
+        "resident_batch_note": "the same scene objects again: packing, upload and tree builds "
+                               "skipped (a re-render, not datagen)",
+        "host_output_resident_scenes_per_s": round(host_value, 2),
+        "new_scenes_host_sah_trees_scenes_per_s": round(n / min(host_tree_s), 2),
         "autotuned": tuned,
-        "host_seconds": {"generate": round(gen_s, 3),
-                         "first_batch_incl_bvh_builds": round(first_s, 3)},
-        "new_scenes_every_batch": {
-            "host_sah_trees_scenes_per_s": round(n / min(fresh["host_sah"]), 2),
-            "gpu_lbvh_trees_scenes_per_s": round(n / min(fresh["gpu_lbvh"]), 2),
-            "gpu_tree_build_ms": round(fresh["gpu_tree_ms"], 2),
-            "gpu_tree_levels": fresh["gpu_tree_levels"],
-            "images_differing_from_host_trees": fresh["gpu_tree_images_differing"],
-            "gpu_lbvh_speedup_vs_per_scene": round(n / min(fresh["gpu_lbvh"]) / per_scene_value,
-                                                   2),
-            "note": "a fresh set of 1024 generated scenes per batch (generation excluded): "
-                    "render_batch time including packing, upload, tree builds and host "
-                    "output"},
+        "host_seconds": {"generate_per_batch": round(min(gen_step_s), 3),
+                         "first_batch_incl_host_bvh_builds": round(first_s, 3),
+                         "generate_first": round(gen_s, 3)},
     }
 
     counts = None
@@ -285,6 +318,7 @@ def bench_datagen(a, cfg):
             "kind": "port",
             "sample": f"first {done} scenes of the batch, full resolution and spp, FP32 CPU "
                       f"checker, {secs:.1f} s"}
+        out["cpu_baseline"].update(host_cpu())
     roof = None
     if counts is not None and pst.ext_launches:
         kernels = {
@@ -293,11 +327,10 @@ def bench_datagen(a, cfg):
                              _bytes_per_ray(counts, 1)),
         }
         name, vals = max(kernels.items(), key=lambda kv: kv[1][0])
-        roof = _roofline(name, *vals, config="c3")
-        _note_pipelines(roof, pst.pipelines)
-        if roof["frac"] > 1.0:
-            roof["note"] = ("per-scene BVHs (<= 20k triangles) stay cache resident, so the "
-                            "algorithmic node/triangle bytes exceed the HBM traffic")
+        step_bytes = (pst.ext_rays * kernels["trace_ext"][3] +
+                      pst.shadow_rays * kernels["trace_shadow"][3])
+        roof = _roofline(name, *vals, config="c3", step_bytes=step_bytes,
+                         step_ms=elapsed / a.steps * 1e3, pipelines=pst.pipelines)
     out["roofline"] = roof
     print(json.dumps(out))
     return 0
@@ -439,6 +472,7 @@ def bench_adjoint(a, cfg):
             "cores": O.max_threads(), "kind": "port",
             "sample": f"whole workload ({size}x{size} x {spp} spp), FP64 forward-mode adjoint "
                       f"of the CPU checker (no forward render), {secs:.2f} s"}
+        out["cpu_baseline"].update(host_cpu())
         _, info = O.render(scene.flat(), ftw, settings, spp, size, size, tab, 32, 0)
         counts = info["counts"]
     roof = None
@@ -447,7 +481,8 @@ def bench_adjoint(a, cfg):
         bpr = (RAY_BYTES * n_rays + NODE_BYTES * float(counts[0][1] + counts[1][1]) +
                TRI_BYTES * float(counts[0][2] + counts[1][2])) / max(n_rays, 1.0)
         # every path is replayed `replays` times (once when lengths are bounded)
-        roof = _roofline("adjoint", adj_ms / a.steps, 1, replays * n_rays, bpr, config="c4")
+        roof = _roofline("adjoint", adj_ms / a.steps, 1, replays * n_rays, bpr, config="c4",
+                         step_bytes=replays * n_rays * bpr, step_ms=elapsed / a.steps * 1e3)
         roof["replays_per_path"] = replays
         roof["note"] = ("36-triangle scene: node and triangle reads are cache hits; the kernel "
                         "is bound by traversal/shading latency, not HBM")
@@ -622,9 +657,28 @@ def make_settings(cfg):
     return s
 
 
+def host_cpu():
+    """CPU model string, usable cores and OMP_NUM_THREADS of this host."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(scene, ftw, settings, cfg, target_seconds=12.0):
     """FP32 CPU checker (same estimator and samples) on a band of rows,
-    all host threads. Returns (baseline dict, per-kind traversal counts)."""
+    all host threads. Returns (baseline dict, per-kind traversal counts,
+    (band image, first pixel, end pixel))."""
     import numpy as np
     from oracle import oracle as O
     size, spp = cfg["size"], cfg["spp"]
@@ -640,7 +694,7 @@ def cpu_baseline(scene, ftw, settings, cfg, target_seconds=12.0):
     rows = int(min(size, max(2, target_seconds / per_row)))
     begin = max(0, (mid - rows // 2)) * size
     end = min(size * size, begin + rows * size)
-    _, info = O.render(flat, ftw, settings, spp, size, size, tab, 32, 0, begin, end)
+    img, info = O.render(flat, ftw, settings, spp, size, size, tab, 32, 0, begin, end)
     n_px = end - begin
     secs = info["seconds"]
     base = {
@@ -651,7 +705,36 @@ def cpu_baseline(scene, ftw, settings, cfg, target_seconds=12.0):
         "sample": f"{n_px} pixels (rows {begin // size}-{end // size - 1}) x {spp} spp of the "
                   f"same render, FP32 CPU checker, {secs:.1f} s",
     }
-    return base, info["counts"]
+    base.update(host_cpu())
+    return base, info["counts"], (img, begin, end)
+
+
+def band_parity(gpu_img, band, scene, ftw, settings, cfg, fp64_rows=16):
+    """The GPU image against the CPU checker on the baseline's band: FP32
+    pixels compared bit for bit, and the relative RMSE against the FP64
+    checker (same samples) on a sub-band of fp64_rows rows."""
+    import numpy as np
+    from oracle import oracle as O
+    ref, begin, end = band
+    size, spp = cfg["size"], cfg["spp"]
+    g = gpu_img.reshape(-1, 3)[begin:end]
+    c = ref.reshape(-1, 3)[begin:end]
+    equal = np.all(g.view(np.uint32) == c.view(np.uint32), axis=1)
+    mid = (begin + end) // 2 // size
+    b64 = max(begin, (mid - fp64_rows // 2) * size)
+    e64 = min(end, b64 + fp64_rows * size)
+    tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp)
+    ref64, _ = O.render(scene.flat(), ftw, settings, spp, size, size, tab, 64, 0, b64, e64)
+    a64 = gpu_img.reshape(-1, 3)[b64:e64].astype(np.float64)
+    r64 = ref64.reshape(-1, 3)[b64:e64]
+    # RMSE of per-pixel differences relative to the mean of the FP64 image
+    rmse = float(np.sqrt(np.mean((a64 - r64) ** 2)) / max(float(r64.mean()), 1e-12))
+    return {"pixels_compared": int(end - begin), "pixels_bit_equal": int(equal.sum()),
+            "rows": f"{begin // size}-{end // size - 1}",
+            "rel_rmse_vs_fp64": float(f"{rmse:.3e}"),
+            "fp64_rows": f"{b64 // size}-{e64 // size - 1}",
+            "checker": "FP32 CPU checker on the same Sobol samples (bit for bit), FP64 checker "
+                       "(relative RMSE)"}
 
 
 def main(argv=None):
@@ -781,29 +864,33 @@ def main(argv=None):
 
     counts = None
     if not a.no_cpu_baseline:
-        base, counts = cpu_baseline(scene, ftw, settings, cfg)
+        base, counts, band = cpu_baseline(scene, ftw, settings, cfg)
         out["cpu_baseline"] = base
+        out["parity"] = band_parity(host_img, band, scene, ftw, settings, cfg)
+        assert out["parity"]["pixels_bit_equal"] == out["parity"]["pixels_compared"], \
+            f"GPU image differs from the CPU checker: {out['parity']}"
 
     # roofline of the dominant kernel
     roof = None
     if counts is not None:
+        step_ms = elapsed / a.steps * 1e3
         if cfg["approach"] == "Streaming" and st.ext_launches:
+            bpr_e, bpr_s = _bytes_per_ray(counts, 0), _bytes_per_ray(counts, 1)
             kernels = {
-                "trace_ext": (st.ext_ms, st.ext_launches, st.ext_rays,
-                              _bytes_per_ray(counts, 0)),
-                "trace_shadow": (st.shadow_ms, st.shadow_launches, st.shadow_rays,
-                                 _bytes_per_ray(counts, 1)),
+                "trace_ext": (st.ext_ms, st.ext_launches, st.ext_rays, bpr_e),
+                "trace_shadow": (st.shadow_ms, st.shadow_launches, st.shadow_rays, bpr_s),
             }
+            step_bytes = st.ext_rays * bpr_e + st.shadow_rays * bpr_s
         else:
             # mega-kernel: every ray of the render
             tot_rays = float(counts[0][0] + counts[1][0])
             bpr = (RAY_BYTES * tot_rays + NODE_BYTES * float(counts[0][1] + counts[1][1]) +
                    TRI_BYTES * float(counts[0][2] + counts[1][2])) / max(tot_rays, 1.0)
             kernels = {"megakernel": (st.mega_ms, st.mega_launches, st.rays, bpr)}
+            step_bytes = st.rays * bpr
         name, vals = max(kernels.items(), key=lambda kv: kv[1][0])
-        roof = _roofline(name, *vals, config=a.config)
-        if cfg["approach"] == "Streaming":
-            _note_pipelines(roof, st.pipelines)
+        roof = _roofline(name, *vals, config=a.config, step_bytes=step_bytes, step_ms=step_ms,
+                         pipelines=st.pipelines if cfg["approach"] == "Streaming" else 1)
     out["roofline"] = roof
     print(json.dumps(out))
     return 0
