@@ -149,9 +149,11 @@ def bench_datagen(a, cfg):
     t0 = time.perf_counter()
     render_batch(r, gens, spp, settings)
     first_s = time.perf_counter() - t0
+    _progress(f"datagen: first batch {first_s:.1f} s")
     tuned = None
     if not a.no_autotune:
         tuned = autotune(r, gens[: min(n, 128)], spp, settings)
+        _progress(f"datagen: autotuned {tuned}")
     for _ in range(max(a.warmup - 1, 0)):
         render_batch(r, gens, spp, settings)
     # Timed steps: every step renders a batch of freshly generated scene
@@ -181,6 +183,7 @@ def bench_datagen(a, cfg):
         del batch
     tree_ms, tree_levels = last_bvh_build(r)
     elapsed = sum(fresh_s)
+    _progress(f"datagen: {a.steps} new-scene batches in {elapsed:.2f} s")
     value = n * a.steps / elapsed
     total_px = sum(g.resolution ** 2 for g in gens)
     fresh_rgb = dev[0].numpy((total_px, 3)).copy()
@@ -227,6 +230,7 @@ def bench_datagen(a, cfg):
         host_tree_s.append(time.perf_counter() - t0)
         del batch
 
+    _progress("datagen: resident and host-tree batches done")
     # per-scene launches: the same scenes, one render call each
     single = settings.copy()
     set_launch_params(r, 6, 1, False)
@@ -626,6 +630,11 @@ def bench_mlp_forward(a):
     out["value"] = round((1 << 20) / (out["forward_ms_per_1M_queries"] * 1e-3) / 1e6, 2)
     print(json.dumps(out))
     return 0
+
+
+def _progress(msg):
+    """Progress of the long legs on stderr (the JSON line stays alone on stdout)."""
+    print(msg, file=sys.stderr, flush=True)
 
 
 def _sync_fn():

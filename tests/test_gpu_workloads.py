@@ -55,7 +55,7 @@ def test_l6_intersect_matches_loopall(renderer, l6_scene):
     rt, ridx, rback = O.loopall(flat["tris"], rays)
     assert np.array_equal(idx, ridx)
     hit = ridx != 0xFFFFFFFF
-    assert hit.sum() > 15000
+    assert hit.sum() > 10000
     assert np.array_equal(t[hit].view(np.uint32), rt[hit].view(np.uint32))
     assert np.array_equal(back[hit], rback[hit])
 
