@@ -202,11 +202,14 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
     Ray<float> last = {{o.x, o.y, o.z}, {d.x, d.y, d.z}};
     R rng = make_rng<R>(p.rng, location, sample, counter);
     Ray<float> nl = last, nn = last;
-    VertexInfo<float> vi{};
+    // The vertex record is always filled (a few register moves): a pointer
+    // that is null for most lanes made the compiler keep it in scratch and
+    // clear it there on every launch.
+    VertexInfo<float> vi;
+    vi.valid = 0u;
     const bool want_gbuffer = p.gbuffer && st.iters == 0 && sample == 0;
     finished = rendering_equation_iteration(st, rng, last, light_hit, next_hit,
-                                            p.settings, sc.view, &ns, &nl, &nn,
-                                            want_gbuffer ? &vi : nullptr);
+                                            p.settings, sc.view, &ns, &nl, &nn, &vi);
     if (want_gbuffer && vi.valid) {
       const Material<float> &mat = sc.view.materials[vi.material];
       V3<float> alb = material_albedo(mat);
