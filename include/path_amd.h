@@ -224,6 +224,23 @@ int pt_render(pt_renderer *r, const pt_sample_spec *spec, const pt_output *out,
               const pt_settings *settings, int show_progress, int show_times,
               double *seconds);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): a MegaKernel SquareImage render into `device_rgb`
+ * (x_dim * y_dim * 3 floats of device memory on the renderer's GPU) that
+ * allocates nothing and uploads nothing. The scene and, for the Sobel rng,
+ * the Sobol table for (max_sample_size, samples_per) must already be
+ * resident: an earlier pt_render of the same scene with the same samples_per
+ * and settings leaves them there; otherwise PT_ERR_INVALID. Scratch (the ray
+ * counter and the per-sample-block partial sums) is a 16 MiB device array per
+ * GPU: renders whose pixels x sample blocks exceed 1,398,080 (only when a
+ * pixel's samples span several sample blocks) are rejected with
+ * PT_ERR_INVALID. All calls on one GPU share that array; calls that overlap
+ * on one GPU race. Same image bits as pt_render. */
+int pt_render_noalloc(pt_renderer *r, const pt_sample_spec *spec, float *device_rgb,
+                      const pt_scene *scene, uint32_t samples_per,
+                      const pt_settings *settings, double *seconds);
+
 /* ------------------------------------------------------------------------ */
 /* Lower-level entry points of the same path (tests, benchmarks, tools).     */
 /* ------------------------------------------------------------------------ */

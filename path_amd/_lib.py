@@ -127,6 +127,8 @@ SIGNATURES = {
     "pt_settings_to_yaml": (C.c_int, [C.POINTER(PtSettings), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pt_renderer_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "pt_renderer_destroy": (None, [_P]),
+    "pt_render_noalloc": (C.c_int, [_P, C.POINTER(PtSampleSpec), _P, _P, C.c_uint32,
+                                    C.POINTER(PtSettings), C.POINTER(C.c_double)]),
     "pt_render": (C.c_int, [_P, C.POINTER(PtSampleSpec), C.POINTER(PtOutput), _P, C.c_uint32,
                             C.POINTER(PtSettings), C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pt_intersect": (C.c_int, [_P, _P, _F, C.c_uint32, _F, _U, _U]),

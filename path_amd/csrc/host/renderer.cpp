@@ -165,7 +165,7 @@ bool pt_renderer::init(int dev, std::string *err) {
     *err = std::string("HIP init failed: ") + hipGetErrorString(e);
     return false;
   }
-  return true;
+  return noalloc_init(dev, err) == PT_OK;
 }
 
 int pt_renderer::ensure_scene(const pt_scene &scene) {

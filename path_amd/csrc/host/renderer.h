@@ -16,6 +16,9 @@
 namespace pt {
 namespace host {
 int set_error(int code, const std::string &msg);
+// Device address of the no-allocation entry point's scratch array on
+// `device` (current), once per device (gpu/noalloc.hip).
+int noalloc_init(int device, std::string *err);
 inline int gpu_stack_size() { return gpu::kTraceStackSize; }
 } // namespace host
 } // namespace pt
