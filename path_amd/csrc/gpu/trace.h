@@ -218,6 +218,34 @@ struct TraceState {
     }
   }
 
+  // One step of a wave whose lanes vote on the kind of step: the wave runs
+  // inner-node visits when vote * (lanes at an inner node) >= 4 * (lanes at a
+  // leaf), else leaf tests; lanes at the other kind wait. Each wave iteration
+  // then executes one of the two code paths instead of both when its lanes
+  // are mixed. `has`: this lane holds an unfinished ray (node != kNoRef).
+  template <class Stack>
+  __device__ __forceinline__ bool step_voted(const float4 *__restrict__ nodes,
+                                             const float4 *__restrict__ tris, Stack &stack,
+                                             bool has, uint32_t vote) {
+    const bool at_inner = has && node >= 0;
+    const bool at_leaf = has && node < 0;
+    const uint32_t ni = uint32_t(__popcll(__ballot(at_inner)));
+    const uint32_t nl = uint32_t(__popcll(__ballot(at_leaf)));
+    if (vote * ni >= 4u * nl) {
+      if (!at_inner)
+        return false;
+      visit_inner(nodes, stack);
+      return node == kNoRef;
+    }
+    if (!at_leaf)
+      return false;
+    intersect_leaf(tris, node);
+    if (best_idx != kNoHit && best_t < occlude_below)
+      return true;
+    node = stack.empty() ? kNoRef : stack.pop();
+    return node == kNoRef;
+  }
+
   // One step. Returns true when the ray is finished.
   //  kHoldLeaves = false: visit one inner node or one leaf.
   //  kHoldLeaves = true: visit inner nodes until every active lane of the
@@ -260,7 +288,9 @@ struct TraceState {
 
 // Whole traversal of one ray, holding leaves until the wave has one each
 // (mega-kernel, 82k-triangle scene: 188 vs 175 Msamples/s; 36-triangle
-// Cornell box: 688 vs 587).
+// Cornell box: 688 vs 587). Voting on the step kind per wave iteration
+// (step_voted, the wavefront traversal's schedule) measured slower here: Cornell
+// box 1,919-1,931 vs 2,174 Msamples/s, 82k-triangle scene 265-270 vs 278.
 template <class Stack>
 __device__ __forceinline__ Hit<float>
 trace_closest(const float4 *__restrict__ nodes, const float4 *__restrict__ tris,

@@ -393,7 +393,15 @@ constexpr uint32_t kRefillMinIdle = 16;
 // mbcnt, no atomics) while the other lanes continue, so a long ray delays
 // only itself. Lanes step one node or leaf at a time: holding leaves until
 // the whole wave has one (as trace_closest does) measured 438 vs 456
-// Msamples/s here, since it delays the refill.
+// Msamples/s here, since it delays the refill. Each wave iteration runs one
+// kind of step, chosen by a vote of the lanes (TraceState::step_voted): with
+// both kinds run whenever the lanes were mixed, 29% of the issued lanes did
+// work (SQ_THREAD_CYCLES_VALU / 64 SQ_INSTS_VALU); voting raised that to 38%
+// and the 82k-triangle workload from 680 to 755-760 Msamples/s. Vote weights
+// (inner if w * inner lanes >= 4 * leaf lanes) 2 / 3 / 4 / 6 / 8 / 16: 746 /
+// 760 / 754 / 749 / 728 / 682; one triangle per leaf step instead of the
+// whole leaf: 748.
+constexpr uint32_t kTraceVote = 3;
 template <bool kShadow, bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStackInts * 4];
@@ -476,7 +484,7 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
         break;
       continue;
     }
-    if (has && ts.template step<false>(nodes, tris, stack)) {
+    if (ts.step_voted(nodes, tris, stack, has, kTraceVote)) {
       hits[slot] = pack_hit(ts.hit());
       has = false;
     }
