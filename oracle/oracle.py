@@ -60,6 +60,10 @@ def lib():
         l.oracle_to_bgra.restype = None
         l.oracle_hash_rng.argtypes = [C.c_uint32, C.c_uint32, _F]
         l.oracle_hash_rng.restype = None
+        l.indep_render.argtypes = [_F, _F, _U, C.c_uint32, C.POINTER(L.PtMaterial), C.c_uint32,
+                                   _F, C.POINTER(L.PtSettings), C.c_uint32, C.c_uint32,
+                                   C.c_uint32, _F, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                   _D]
         _lib = l
     return _lib
 
@@ -209,6 +213,27 @@ def to_bgra(rgb: np.ndarray) -> np.ndarray:
 def hash_rng(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.float32)
     lib().oracle_hash_rng(seed, n, _f(out))
+    return out
+
+
+def indep_render(flat: dict, film_to_world, settings, samples_per: int, x_dim: int, y_dim: int,
+                 sobol: np.ndarray, pixel_begin: int = 0, pixel_end: int = 0,
+                 threads: int = 0, mistake: int = 0) -> np.ndarray:
+    """Per-pixel mean radiance (y, x, 3) in FP64 from the independent
+    restatement of the reference estimator (oracle/indep.cpp, which shares no
+    code with the product). Sobol rng only. `mistake` plants a known error
+    in the restatement (tests of the comparison's sensitivity)."""
+    raw = settings.raw if hasattr(settings, "raw") else settings
+    ftw = np.ascontiguousarray(film_to_world, np.float32)
+    sob = np.ascontiguousarray(sobol, np.float32)
+    out = np.zeros((y_dim, x_dim, 3), np.float64)
+    rc = lib().indep_render(_f(flat["tris"]), _f(flat["normals"]), _u(flat["tri_mat"]),
+                            flat["tris"].shape[0], flat["materials"], flat["n_materials"],
+                            _f(ftw), C.byref(raw), samples_per, x_dim, y_dim, _f(sob),
+                            pixel_begin, pixel_end, threads, mistake,
+                            out.ctypes.data_as(_D))
+    if rc != 0:
+        raise ValueError(f"independent render failed ({rc})")
     return out
 
 

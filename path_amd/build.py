@@ -108,7 +108,8 @@ def build_oracle() -> Path:
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run([cxx, *HOST_FLAGS, "-march=x86-64-v3", "-fopenmp", "-shared",
           "-I/opt/rocm/include",
-          str(ORACLE_DIR / "oracle.cpp"), str(CSRC / "host" / "bvh.cpp"),
+          str(ORACLE_DIR / "oracle.cpp"), str(ORACLE_DIR / "indep.cpp"),
+          str(CSRC / "host" / "bvh.cpp"),
           "-o", str(tmp), "-L/opt/rocm/lib", "-lrocrand", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB

@@ -83,6 +83,10 @@ def test_l6_images_bit_exact_at_benchmark_depth(renderer, l6_scene, approach):
     assert info["counts"][0][0] > 3 * size * size * spp
     ref64, _ = O.render(scene.flat(), ftw, s, spp, size, size, tab, precision=64)
     assert rel_rmse(img, ref64) < 2e-2
+    # the independent FP64 restatement of the reference (no shared code)
+    ind = O.indep_render(scene.flat(), ftw, s, spp, size, size, tab)
+    assert rel_rmse(img, ind) < 2e-2
+    assert rel_rmse(ref64, ind) < 1e-9
 
 
 def test_l6_streaming_equals_megakernel_full_size(renderer, l6_scene):
@@ -129,6 +133,10 @@ def test_batched_datagen_scenes_bit_exact(renderer):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), g.seed
         assert np.array_equal(gb.view(np.uint32), info["gbuffer"].view(np.uint32)), g.seed
         assert img.mean() > 0 and (gb[..., 3:6] != 0).any()
+        # independent FP64 restatement on up to 32 rows of the image
+        e = min(r, 32) * r
+        ind = O.indep_render(g.scene.flat(), g.film_to_world, st, spp, r, r, tab, 0, e)
+        assert rel_rmse(img.reshape(-1, 3)[:e], ind.reshape(-1, 3)[:e]) < 2e-2, g.seed
 
 
 def test_reference_benchmark_yaml_bit_exact(renderer):
