@@ -336,8 +336,7 @@ def bench_datagen(a, cfg):
         roof = _roofline(name, *vals, config="c3", step_bytes=step_bytes,
                          step_ms=elapsed / a.steps * 1e3, pipelines=pst.pipelines)
     out["roofline"] = roof
-    print(json.dumps(out))
-    return 0
+    return out
 
 
 def bench_adjoint(a, cfg):
@@ -491,8 +490,7 @@ def bench_adjoint(a, cfg):
         roof["note"] = ("36-triangle scene: node and triangle reads are cache hits; the kernel "
                         "is bound by traversal/shading latency, not HBM")
     out["roofline"] = roof
-    print(json.dumps(out))
-    return 0
+    return out
 
 
 def bench_mlp(a, cfg):
@@ -632,6 +630,34 @@ def bench_mlp_forward(a):
     return 0
 
 
+def bench_mlp_train(a):
+    """Training alone on a synthetic device-resident set (2^20 rows of random
+    features and smooth targets): Adam steps of 65,536 rows, the C5 network's
+    training kernels, for profiling."""
+    import numpy as np
+    from path_amd.mlp import RadianceMLP
+    rng = np.random.default_rng(0)
+    rows = 1 << 20
+    f = rng.uniform(-1, 1, (rows, 9)).astype(np.float32)
+    t = np.stack([np.sin(f[:, 0] * 2) + 1.5, np.cos(f[:, 1]) + 1.0, f[:, 2] ** 2 + 0.5],
+                 1).astype(np.float32)
+    m = RadianceMLP(0, seed=1)
+    m.set_dataset(f, t)
+    batch = 65536
+    m.train(max(a.warmup, 1), batch, 1e-3, seed=1, record_losses=False)
+    losses, secs = m.train(a.steps, batch, 1e-3, seed=2)
+    tflops = 3 * 2 * MLP_MACS * batch * a.steps / secs / 1e12
+    out = {"metric": "radiance-MLP training (1 GPU)", "value": round(tflops, 2),
+           "unit": "TFLOP/s (3 x forward MACs x 2, valid dims)", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
+           "higher_is_better": True, "dtype": "bf16 (fp32 accumulate, fp32 Adam)",
+           "data": "synthetic",
+           "config": {"workload": "radiance_mlp_6x256_train_65536_rows_per_step"},
+           "loss_first_last": [float(losses[0]), float(losses[-1])]}
+    print(json.dumps(out))
+    return 0
+
+
 def _progress(msg):
     """Progress of the long legs on stderr (the JSON line stays alone on stdout)."""
     print(msg, file=sys.stderr, flush=True)
@@ -767,6 +793,8 @@ def main(argv=None):
                     help="wavefront traversal grid (blocks of 256 per CU)")
     ap.add_argument("--iters-per-sync", type=int, default=0,
                     help="wavefront iterations enqueued per host check (hipGraph replay)")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="default run: skip the short C1 / C4 / C3 runs in other_workloads")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c3: keep the default launch parameters")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -777,9 +805,14 @@ def main(argv=None):
     ap.add_argument("--mlp-forward", action="store_true",
                     help="only time the radiance MLP's fused inference forward (2^20 queries, "
                          "--steps launches)")
+    ap.add_argument("--mlp-train", action="store_true",
+                    help="only time radiance-MLP training (Adam steps of 65,536 rows on a "
+                         "synthetic resident set, --steps steps)")
     a = ap.parse_args(argv)
     if a.mlp_forward:
         return bench_mlp_forward(a)
+    if a.mlp_train:
+        return bench_mlp_train(a)
     if a.copy_calibration:
         import torch
         x = torch.ones(256 * 1024 * 1024 // 4, device="cuda")
@@ -794,14 +827,51 @@ def main(argv=None):
               "image sum-reduce, which is not built here; run with --gpus 1", file=sys.stderr)
         return 2
 
-    from path_amd.renderer import Renderer
     cfg = dict(CONFIGS[a.config])
     if a.config == "c3":
-        return bench_datagen(a, cfg)
+        print(json.dumps(bench_datagen(a, cfg)))
+        return 0
     if a.config == "c4":
-        return bench_adjoint(a, cfg)
+        print(json.dumps(bench_adjoint(a, cfg)))
+        return 0
     if a.config == "c5":
         return bench_mlp(a, cfg)
+    out = bench_render(a, cfg)
+    if a.config == "c2" and not a.no_other_workloads:
+        out["other_workloads"] = other_workloads(a)
+    print(json.dumps(out))
+    return 0
+
+
+def other_workloads(a):
+    """Short runs of the other workloads in the same process, so the default
+    run measures them too: the Cornell-box mega-kernel (C1), the adjoint
+    gradient step (C4) and batched datagen with new scenes every batch (C3),
+    each with its own parity checks; their CPU baselines and the datagen
+    autotuner are left to the --config runs."""
+    import copy
+    res = {}
+    for name, steps in (("c1", 20), ("c4", 5), ("c3", 3)):
+        b = copy.copy(a)
+        b.config, b.steps, b.warmup = name, steps, 2
+        b.no_cpu_baseline = True
+        b.no_autotune = True
+        cfg = dict(CONFIGS[name])
+        _progress(f"other workloads: {name}")
+        o = (bench_render(b, cfg) if name == "c1" else
+             bench_adjoint(b, cfg) if name == "c4" else bench_datagen(b, cfg))
+        keep = ["metric", "value", "unit", "steps", "warmup", "ms_per_step", "config",
+                "mrays_per_s", "msamples_per_s", "speedup_vs_per_scene",
+                "images_differing_from_host_trees", "per_scene_launches",
+                "max_rel_err_vs_central_differences_fp32_renders", "forward_ms", "adjoint_ms",
+                "resident_batch_scenes_per_s", "gpu_tree_build_ms"]
+        res[name] = {k: o[k] for k in keep if k in o}
+    return res
+
+
+def bench_render(a, cfg):
+    """One SquareImage render of the configuration per step (C1, C2)."""
+    from path_amd.renderer import Renderer
     if a.integrator:
         cfg["approach"] = a.integrator
     with tempfile.TemporaryDirectory() as d:
@@ -907,8 +977,9 @@ def main(argv=None):
         roof = _roofline(name, *vals, config=a.config, step_bytes=step_bytes, step_ms=step_ms,
                          pipelines=st.pipelines if cfg["approach"] == "Streaming" else 1)
     out["roofline"] = roof
-    print(json.dumps(out))
-    return 0
+    return out
+
+
 
 
 if __name__ == "__main__":

@@ -133,10 +133,16 @@ def test_batched_datagen_scenes_bit_exact(renderer):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), g.seed
         assert np.array_equal(gb.view(np.uint32), info["gbuffer"].view(np.uint32)), g.seed
         assert img.mean() > 0 and (gb[..., 3:6] != 0).any()
-        # independent FP64 restatement on up to 32 rows of the image
+        # Independent FP64 restatement on up to 32 rows of the image, by the
+        # relative mean absolute difference (measured <= 6.4e-4): these
+        # scenes' emitters are small bright quads, and a float ray through a
+        # quad's shared diagonal can miss both triangles (Moller-Trumbore in
+        # float is not watertight, as in the reference), so a single sample
+        # seeing radiance 26 in FP64 and black in FP32 dominates an RMSE
         e = min(r, 32) * r
         ind = O.indep_render(g.scene.flat(), g.film_to_world, st, spp, r, r, tab, 0, e)
-        assert rel_rmse(img.reshape(-1, 3)[:e], ind.reshape(-1, 3)[:e]) < 2e-2, g.seed
+        a, b = img.reshape(-1, 3)[:e].astype(np.float64), ind.reshape(-1, 3)[:e]
+        assert np.abs(a - b).sum() / max(b.sum(), 1e-12) < 5e-3, g.seed
 
 
 def test_reference_benchmark_yaml_bit_exact(renderer):
