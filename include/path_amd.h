@@ -311,6 +311,29 @@ int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                     uint32_t samples_per, const pt_settings *settings,
                     double *seconds);
 
+/* Ray sample specs of many scenes in one launch sequence: the batched form
+ * of pt_render with an InitialRays or InitialIdxAndDir sample spec and a
+ * FloatRGB (n_steps = 0) or OutputPerStep (n_steps images) output, which the
+ * reference's training-data generator calls once per scene
+ * (src/generate_data/full_scene/generate_data.cpp:280-294). Each item's values
+ * are bit-identical to pt_render of that item alone with the MegaKernel
+ * integrator (same work division, sample sequence and reduction order). All
+ * items are InitialRays (initial_idx NULL) or all are InitialIdxAndDir; items
+ * with n_rays = 0 are skipped. */
+typedef struct pt_ray_batch_item {
+  const pt_scene *scene;
+  const float *rays;           /* n_rays * 6 floats (origin, direction), host */
+  const uint32_t *initial_idx; /* NULL, or n_rays flattened triangle indices: the
+                                  first intersection is that triangle at
+                                  distance 1 (initial_ray_sample.h:58-75) */
+  uint32_t n_rays;
+  float *out;                  /* host, max(n_steps, 1) * n_rays * 3 floats,
+                                  step-major (the per-step images in order) */
+} pt_ray_batch_item;
+int pt_render_rays_batch(pt_renderer *r, const pt_ray_batch_item *items, uint32_t n,
+                         uint32_t samples_per, uint32_t n_steps,
+                         const pt_settings *settings, double *seconds);
+
 /* Launch parameters of the wavefront integrator. Results never depend on
  * them. The path pool (max_num_samples_per_launch paths) runs as
  * independent pipelines (pt_set_pipelines) of an equal share of the pool

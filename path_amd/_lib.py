@@ -86,6 +86,12 @@ class PtSampleSpec(C.Structure):
                 ("initial_idx", C.POINTER(C.c_uint32)), ("n_rays", C.c_uint32)]
 
 
+class PtRayBatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", C.POINTER(C.c_float)),
+                ("initial_idx", C.POINTER(C.c_uint32)), ("n_rays", C.c_uint32),
+                ("out", C.POINTER(C.c_float))]
+
+
 class PtOutput(C.Structure):
     _fields_ = [("type", C.c_uint32), ("data", C.c_void_p),
                 ("per_step", C.POINTER(C.POINTER(C.c_float))), ("n_steps", C.c_uint32)]
@@ -127,6 +133,9 @@ SIGNATURES = {
     "pt_settings_to_yaml": (C.c_int, [C.POINTER(PtSettings), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pt_renderer_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "pt_renderer_destroy": (None, [_P]),
+    "pt_render_rays_batch": (C.c_int, [_P, C.POINTER(PtRayBatchItem), C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.POINTER(PtSettings),
+                                       C.POINTER(C.c_double)]),
     "pt_render_noalloc": (C.c_int, [_P, C.POINTER(PtSampleSpec), _P, _P, C.c_uint32,
                                     C.POINTER(PtSettings), C.POINTER(C.c_double)]),
     "pt_render": (C.c_int, [_P, C.POINTER(PtSampleSpec), C.POINTER(PtOutput), _P, C.c_uint32,

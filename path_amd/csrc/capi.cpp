@@ -445,6 +445,16 @@ int pt_render_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
   PT_GUARD_END
 }
 
+int pt_render_rays_batch(pt_renderer *r, const pt_ray_batch_item *items, uint32_t n,
+                         uint32_t samples_per, uint32_t n_steps,
+                         const pt_settings *settings, double *seconds) {
+  PT_GUARD_BEGIN
+  if (!r || !settings || (n && !items))
+    return fail(PT_ERR_INVALID, "null argument");
+  return render_rays_batch(*r, items, n, samples_per, n_steps, *settings, seconds);
+  PT_GUARD_END
+}
+
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
                          uint32_t iters_per_sync, int use_graphs) {
   if (!r || trace_blocks_per_cu == 0 || trace_blocks_per_cu > 256 ||

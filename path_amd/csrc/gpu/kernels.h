@@ -47,6 +47,36 @@ struct MegaParams {
   unsigned long long *ray_counter;
 };
 
+// One item of a batched ray-sample-spec render (InitialRays /
+// InitialIdxAndDir): its scene's offsets into the batch arrays, its work
+// division over its rays, and where its rays and outputs start.
+struct MegaBatchItem {
+  uint32_t node_off, tri_off, shade_off, mat_off, light_off, n_lights;
+  uint32_t ray_off;
+  uint32_t pad;
+  size_t out_off; // floats: n_steps (or 1) images of num_sample_blocks * rays * 3
+  WorkDivision wd;
+};
+
+struct MegaBatchParams {
+  const MegaBatchItem *items;
+  const uint32_t *block_begin; // n_items + 1 (first grid block of each item)
+  uint32_t n_items;
+  uint32_t n_steps;            // per-step images, or 0 for one FloatRGB image
+  const float4 *nodes;
+  const float4 *tris;
+  const ShadeTri<float> *shade;
+  const Material<float> *materials;
+  const TriGeom<float> *light_tris;
+  const float *light_cdf;
+  IntegratorSettings settings;
+  RngParams rng;
+  const float *rays;           // 6 floats per ray, all items
+  const uint32_t *initial_idx; // InitialIdxAndDir: triangle index per ray, or null
+  float *out;
+  unsigned long long *ray_counter;
+};
+
 // Path-replay adjoint of a SquareImage render: gradients of sum(dI * I) with
 // respect to every material's albedo color (material_albedo) and emission.
 constexpr uint32_t kMaxAdjointMaterials = 40; // LDS: 6 * 40 * 65 floats
@@ -99,6 +129,9 @@ hipError_t launch_megakernel(const MegaParams &p, uint32_t start_block,
                              uint32_t end_block, bool split, hipStream_t s);
 // Sum of the num_sample_blocks partials per pixel (in sample-block order),
 // divided by samples_per.
+hipError_t launch_megakernel_batch(const MegaBatchParams &bp, uint32_t block_size,
+                                   uint32_t total_blocks, uint32_t max_blocks_per_launch,
+                                   hipStream_t s);
 hipError_t launch_reduce_sample_blocks(const float *in, uint32_t nsb,
                                        uint32_t n_pixels, uint32_t samples_per,
                                        float *out, hipStream_t s);

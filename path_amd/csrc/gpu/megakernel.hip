@@ -243,13 +243,13 @@ __device__ __forceinline__ void count_rays(const MegaParams &p, uint32_t traced)
 
 // Blocks of up to 1024 threads (16 waves); dynamic LDS holds the traversal
 // stacks (kLdsStackInts per wave), deeper entries go to a private array.
+// One block of the mega-kernel grid of `p` (block index within its work
+// division); `mk_stack` is the block's dynamic LDS (kLdsStackInts per wave).
 template <bool kPerStep>
-__global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
-                                                     uint32_t start_block) {
-  extern __shared__ int mk_stack[];
+__device__ __forceinline__ void megakernel_block(const MegaParams &p, uint32_t block,
+                                                 int *mk_stack) {
   int spill[kOverflowDepth];
   LdsStack stack{lds_stack_column(mk_stack), spill, 1};
-  uint32_t block = blockIdx.x + start_block;
   ThreadInfo info = p.wd.get_thread_info(block, threadIdx.x);
   V3<float> total = {0.f, 0.f, 0.f};
   V3<float> steps[kPerStep ? kMaxOutputSteps : 1];
@@ -272,6 +272,73 @@ __global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
     reduce_store(p, info, block, total, p.out_rgb);
   }
   count_rays(p, traced);
+}
+
+template <bool kPerStep>
+__global__ __launch_bounds__(1024) void k_megakernel(MegaParams p,
+                                                     uint32_t start_block) {
+  extern __shared__ int mk_stack[];
+  megakernel_block<kPerStep>(p, blockIdx.x + start_block, mk_stack);
+}
+
+// Many ray-sample-spec renders in one grid: the blocks of item i are
+// [block_begin[i], block_begin[i + 1]); each block runs the item's own work
+// division over its scene (offsets into the batch's concatenated arrays), so
+// every item's values equal its render alone.
+template <bool kPerStep>
+__global__ __launch_bounds__(1024) void k_megakernel_batch(MegaBatchParams bp,
+                                                           uint32_t start_block) {
+  extern __shared__ int mk_stack[];
+  const uint32_t block = blockIdx.x + start_block;
+  uint32_t lo = 0, hi = bp.n_items; // block_begin[lo] <= block < block_begin[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bp.block_begin[mid] <= block)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const MegaBatchItem &it = bp.items[lo];
+  MegaParams p{};
+  p.scene.nodes = bp.nodes + 4 * size_t(it.node_off);
+  p.scene.tris = bp.tris + 3 * size_t(it.tri_off);
+  p.scene.view.tris = bp.shade + it.shade_off;
+  p.scene.view.materials = bp.materials + it.mat_off;
+  p.scene.view.light_tris = bp.light_tris + it.light_off;
+  p.scene.view.light_cdf = bp.light_cdf + it.light_off;
+  p.scene.view.n_lights = it.n_lights;
+  p.settings = bp.settings;
+  p.wd = it.wd;
+  p.rng = bp.rng;
+  p.initial_rays = bp.rays + 6 * size_t(it.ray_off);
+  p.initial_idx = bp.initial_idx ? bp.initial_idx + it.ray_off : nullptr;
+  p.out_rgb = bp.out + it.out_off;
+  p.n_steps = bp.n_steps;
+  p.divide_by_spp = it.wd.num_sample_blocks == 1 ? 1u : 0u;
+  p.ray_counter = bp.ray_counter;
+  megakernel_block<kPerStep>(p, block - bp.block_begin[lo], mk_stack);
+}
+
+hipError_t launch_megakernel_batch(const MegaBatchParams &bp, uint32_t block_size,
+                                   uint32_t total_blocks, uint32_t max_blocks_per_launch,
+                                   hipStream_t s) {
+  if (block_size == 0 || block_size > 1024 || block_size % 64 != 0)
+    return hipErrorInvalidConfiguration;
+  const size_t lds = size_t(block_size / 64) * kLdsStackInts * sizeof(int);
+  const uint32_t chunk = max_blocks_per_launch ? max_blocks_per_launch : total_blocks;
+  for (uint32_t start = 0; start < total_blocks; start += chunk) {
+    const uint32_t n = total_blocks - start < chunk ? total_blocks - start : chunk;
+    if (bp.n_steps)
+      hipLaunchKernelGGL(k_megakernel_batch<true>, dim3(n), dim3(block_size), lds, s, bp,
+                         start);
+    else
+      hipLaunchKernelGGL(k_megakernel_batch<false>, dim3(n), dim3(block_size), lds, s, bp,
+                         start);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
 }
 
 // Sample-parallel mega-kernel. The work division gives a thread a pixel and a

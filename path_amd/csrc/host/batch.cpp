@@ -254,48 +254,16 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   return PT_OK;
 }
 
-} // namespace
-
-namespace pt {
-namespace host {
-
-int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
-                 uint32_t spp, const pt_settings &settings, double *seconds) {
-  if (n == 0)
-    return PT_OK;
-  if (spp == 0)
-    return set_error(PT_ERR_INVALID, "samples_per must be positive");
-  auto t0 = std::chrono::steady_clock::now();
-  HIP_TRY(hipSetDevice(r.device));
-  for (uint32_t i = 0; i < n; ++i)
-    if (!items[i].scene || items[i].x_dim == 0 || items[i].y_dim == 0 || !items[i].out_rgb)
-      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
-                                           ": null scene/output or empty image");
-  // outputs: all host memory, or all device memory of this GPU
-  bool device_out = false;
-  for (uint32_t i = 0; i < n; ++i) {
-    for (const void *ptr : {static_cast<const void *>(items[i].out_rgb),
-                            static_cast<const void *>(items[i].out_gbuffer)}) {
-      if (!ptr)
-        continue;
-      hipPointerAttribute_t attr{};
-      bool dev = hipPointerGetAttributes(&attr, ptr) == hipSuccess &&
-                 attr.type == hipMemoryTypeDevice;
-      if (!dev)
-        (void)hipGetLastError();
-      if (dev && attr.device != r.device)
-        return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
-                                             ": output on another GPU");
-      if (i == 0 && ptr == items[0].out_rgb)
-        device_out = dev;
-      else if (dev != device_out)
-        return set_error(PT_ERR_INVALID, "batch outputs must be all host or all device memory");
-    }
-  }
-  const bool gpu_bvh = r.gpu_bvh;
-  // The scenes of the last batch stay on the GPU: the same scenes (by
-  // generation), sizes, spp and tree builder skip the host packing, the upload
-  // and the tree builds.
+// Make the batch's scenes resident in the renderer's concatenated arrays
+// (b_nodes, b_tris, b_shade, b_mats, b_light_*, b_desc, b_sample_begin).
+// The scenes of the last batch stay on the GPU: the same scenes (by
+// generation), sizes, spp and tree builder skip the host packing, the upload
+// and the tree builds; new cameras only re-upload the descriptors. When a GPU
+// tree build falls back, r.gpu_bvh is cleared and nothing is uploaded: the
+// caller repeats the call with host trees (and restores r.gpu_bvh).
+int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
+                        bool gpu_bvh, std::vector<gpu::SceneDesc> *desc_out,
+                        uint64_t *n_pixels_out, uint64_t *total_out) {
   std::vector<uint64_t> key;
   key.reserve(3 * size_t(n) + 2);
   key.push_back(spp);
@@ -306,9 +274,9 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
     key.push_back(uint64_t(items[i].x_dim) | (uint64_t(items[i].y_dim) << 32));
   }
   const bool resident = key == r.batch_key;
-  std::vector<gpu::SceneDesc> desc;
+  std::vector<gpu::SceneDesc> &desc = *desc_out;
   Layout lay;
-  uint64_t n_pixels = 0, total = 0;
+  uint64_t &n_pixels = *n_pixels_out, &total = *total_out;
   int rc = PT_OK;
   hipStream_t s = r.stream;
   if (resident) {
@@ -361,11 +329,10 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
       return rc;
     HIP_TRY(hipStreamSynchronize(s));
     if (fallback) {
-      // a tree too deep for the traversal stack: build this batch on the host
+      // a tree too deep for the traversal stack: the caller builds this
+      // batch on the host (r.gpu_bvh is cleared for that call)
       r.gpu_bvh = false;
-      rc = render_batch(r, items, n, spp, settings, seconds);
-      r.gpu_bvh = true;
-      return rc;
+      return PT_OK;
     }
     r.last_gpu_bvh_ms = std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now() - tb).count();
@@ -378,6 +345,60 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   r.batch_samples = total;
   }
 
+  return PT_OK;
+}
+
+} // namespace
+
+namespace pt {
+namespace host {
+
+int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
+                 uint32_t spp, const pt_settings &settings, double *seconds) {
+  if (n == 0)
+    return PT_OK;
+  if (spp == 0)
+    return set_error(PT_ERR_INVALID, "samples_per must be positive");
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipSetDevice(r.device));
+  for (uint32_t i = 0; i < n; ++i)
+    if (!items[i].scene || items[i].x_dim == 0 || items[i].y_dim == 0 || !items[i].out_rgb)
+      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                           ": null scene/output or empty image");
+  // outputs: all host memory, or all device memory of this GPU
+  bool device_out = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    for (const void *ptr : {static_cast<const void *>(items[i].out_rgb),
+                            static_cast<const void *>(items[i].out_gbuffer)}) {
+      if (!ptr)
+        continue;
+      hipPointerAttribute_t attr{};
+      bool dev = hipPointerGetAttributes(&attr, ptr) == hipSuccess &&
+                 attr.type == hipMemoryTypeDevice;
+      if (!dev)
+        (void)hipGetLastError();
+      if (dev && attr.device != r.device)
+        return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                             ": output on another GPU");
+      if (i == 0 && ptr == items[0].out_rgb)
+        device_out = dev;
+      else if (dev != device_out)
+        return set_error(PT_ERR_INVALID, "batch outputs must be all host or all device memory");
+    }
+  }
+  const bool gpu_bvh = r.gpu_bvh;
+  hipStream_t s = r.stream;
+  std::vector<gpu::SceneDesc> desc;
+  uint64_t n_pixels = 0, total = 0;
+  int rc = ensure_batch_scenes(r, items, n, spp, gpu_bvh, &desc, &n_pixels, &total);
+  if (rc != PT_OK)
+    return rc;
+  if (gpu_bvh && !r.gpu_bvh) {
+    // a tree too deep for the traversal stack: this batch was built on the host
+    rc = render_batch(r, items, n, spp, settings, seconds);
+    r.gpu_bvh = true;
+    return rc;
+  }
   gpu::RngParams rng{};
   rng.type = settings.rng;
   rng.samples_per = spp;
@@ -497,6 +518,201 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   r.stats.kernel_ms = ms;
   r.stats.samples = total;
   r.stats.rays = r.stats.ext_rays + r.stats.shadow_rays;
+  if (seconds)
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return PT_OK;
+}
+
+int render_rays_batch(pt_renderer &r, const pt_ray_batch_item *items, uint32_t n,
+                      uint32_t spp, uint32_t n_steps, const pt_settings &settings,
+                      double *seconds) {
+  if (spp == 0)
+    return set_error(PT_ERR_INVALID, "samples_per must be positive");
+  if (n_steps > gpu::kMaxOutputSteps)
+    return set_error(PT_ERR_INVALID, "at most " + std::to_string(gpu::kMaxOutputSteps) +
+                                         " per-step outputs");
+  if (settings.rng == PT_RNG_SOBEL &&
+      (settings.max_sample_size == 0 || settings.max_sample_size > 20000))
+    return set_error(PT_ERR_INVALID, "max_sample_size must be in [1, 20000]");
+  const auto t0 = std::chrono::steady_clock::now();
+  // non-empty items, all InitialRays or all InitialIdxAndDir
+  std::vector<uint32_t> act;
+  int with_idx = -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const pt_ray_batch_item &it = items[i];
+    if (!it.scene)
+      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) + ": null scene");
+    if (it.n_rays == 0)
+      continue;
+    if (!it.rays || !it.out)
+      return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                           ": null rays or output");
+    const int has = it.initial_idx ? 1 : 0;
+    if (with_idx >= 0 && has != with_idx)
+      return set_error(PT_ERR_INVALID,
+                       "InitialRays and InitialIdxAndDir items cannot share one batch");
+    with_idx = has;
+    if (has) {
+      const uint32_t nt = uint32_t(it.scene->flat().tris.size());
+      for (uint32_t k = 0; k < it.n_rays; ++k)
+        if (it.initial_idx[k] >= nt)
+          return set_error(PT_ERR_INVALID, "batch item " + std::to_string(i) +
+                                               ": triangle index out of range");
+    }
+    act.push_back(i);
+  }
+  r.stats = pt_render_stats{};
+  if (act.empty()) {
+    if (seconds)
+      *seconds = 0.0;
+    return PT_OK;
+  }
+  const uint32_t m = uint32_t(act.size());
+  HIP_TRY(hipSetDevice(r.device));
+  hipStream_t s = r.stream;
+  // the items' scenes in the batch arrays (host SAH trees), as pt_render_batch
+  std::vector<pt_batch_item> bitems(m);
+  for (uint32_t j = 0; j < m; ++j) {
+    bitems[j] = pt_batch_item{};
+    bitems[j].scene = items[act[j]].scene;
+    bitems[j].x_dim = items[act[j]].n_rays;
+    bitems[j].y_dim = 1;
+  }
+  std::vector<gpu::SceneDesc> desc;
+  uint64_t n_pixels = 0, total = 0;
+  int rc = ensure_batch_scenes(r, bitems.data(), m, spp, false, &desc, &n_pixels, &total);
+  if (rc != PT_OK)
+    return rc;
+  // per-item work divisions, grid blocks, ray and output offsets
+  const uint32_t n_img = n_steps ? n_steps : 1;
+  std::vector<gpu::MegaBatchItem> mb(m);
+  std::vector<uint32_t> block_begin(m + 1, 0);
+  std::vector<size_t> final_off(m + 1, 0); // reduced outputs (items with several blocks)
+  uint64_t ray_total = 0;
+  size_t out_total = 0;
+  uint32_t block_size = 0;
+  for (uint32_t j = 0; j < m; ++j) {
+    const pt_ray_batch_item &it = items[act[j]];
+    WorkDivision wd;
+    if (!WorkDivision::make(work_division_settings(settings.render_work_division), spp,
+                            it.n_rays, 1, &wd) ||
+        wd.block_size > 1024)
+      return set_error(PT_ERR_INVALID, "invalid render work division settings");
+    block_size = wd.block_size;
+    gpu::MegaBatchItem &b = mb[j];
+    b = gpu::MegaBatchItem{};
+    b.node_off = desc[j].node_off;
+    b.tri_off = desc[j].tri_off;
+    b.shade_off = desc[j].shade_off;
+    b.mat_off = desc[j].mat_off;
+    b.light_off = desc[j].light_off;
+    b.n_lights = desc[j].n_lights;
+    b.ray_off = uint32_t(ray_total);
+    b.out_off = out_total;
+    b.wd = wd;
+    const uint64_t blocks = uint64_t(block_begin[j]) + wd.total_num_blocks();
+    if (blocks > 0xffffffffull || ray_total + it.n_rays > 0xffffffffull)
+      return set_error(PT_ERR_INVALID, "ray batch too large");
+    block_begin[j + 1] = uint32_t(blocks);
+    ray_total += it.n_rays;
+    out_total += size_t(n_img) * wd.num_sample_blocks * it.n_rays * 3;
+    final_off[j + 1] =
+        final_off[j] + (wd.num_sample_blocks == 1 ? 0 : size_t(n_img) * it.n_rays * 3);
+  }
+  // rays and triangle indices through pinned staging
+  const size_t ray_bytes = size_t(ray_total) * 6 * sizeof(float);
+  const size_t idx_bytes = with_idx ? size_t(ray_total) * sizeof(uint32_t) : 0;
+  HIP_TRY(r.b_stage_in.ensure(ray_bytes + idx_bytes));
+  char *stage = static_cast<char *>(r.b_stage_in.ptr);
+  parallel_for(m, [&](size_t j) {
+    const pt_ray_batch_item &it = items[act[j]];
+    std::memcpy(stage + size_t(mb[j].ray_off) * 6 * sizeof(float), it.rays,
+                size_t(it.n_rays) * 6 * sizeof(float));
+    if (with_idx)
+      std::memcpy(stage + ray_bytes + size_t(mb[j].ray_off) * sizeof(uint32_t), it.initial_idx,
+                  size_t(it.n_rays) * sizeof(uint32_t));
+  });
+  HIP_TRY(r.scratch_rays.upload(reinterpret_cast<const float *>(stage), size_t(ray_total) * 6, s));
+  if (with_idx)
+    HIP_TRY(r.scratch_idx.upload(reinterpret_cast<const uint32_t *>(stage + ray_bytes),
+                                 size_t(ray_total), s));
+  HIP_TRY(r.mb_items.upload(mb.data(), m, s));
+  HIP_TRY(r.mb_blocks.upload(block_begin.data(), m + 1, s));
+  HIP_TRY(r.rgb.ensure(out_total));
+  if (final_off[m])
+    HIP_TRY(r.rgb2.ensure(final_off[m]));
+  HIP_TRY(r.counter.ensure(1));
+  HIP_TRY(hipMemsetAsync(r.counter.ptr, 0, sizeof(unsigned long long), s));
+  gpu::RngParams rng{};
+  rng.type = settings.rng;
+  rng.samples_per = spp;
+  if (settings.rng == PT_RNG_SOBEL) {
+    rc = r.ensure_sobol(settings.max_sample_size, spp);
+    if (rc != PT_OK)
+      return rc;
+    rng.vals = r.sobol_vals.ptr;
+    rng.dim_bound = settings.max_sample_size;
+    rng.initial_dim_bound = settings.max_sample_size < 64 ? settings.max_sample_size : 64;
+  }
+  HIP_TRY(hipStreamSynchronize(s)); // the staging buffer is reused below
+  gpu::MegaBatchParams bp{};
+  bp.items = r.mb_items.ptr;
+  bp.block_begin = r.mb_blocks.ptr;
+  bp.n_items = m;
+  bp.n_steps = n_steps;
+  bp.nodes = reinterpret_cast<const float4 *>(r.b_nodes.ptr);
+  bp.tris = reinterpret_cast<const float4 *>(r.b_tris.ptr);
+  bp.shade = r.b_shade.ptr;
+  bp.materials = r.b_mats.ptr;
+  bp.light_tris = r.b_light_tris.ptr;
+  bp.light_cdf = r.b_light_cdf.ptr;
+  bp.settings = integrator_settings(settings);
+  bp.rng = rng;
+  bp.rays = r.scratch_rays.ptr;
+  bp.initial_idx = with_idx ? r.scratch_idx.ptr : nullptr;
+  bp.out = r.rgb.ptr;
+  bp.ray_counter = r.counter.ptr;
+  HIP_TRY(hipEventRecord(r.ev_start, s));
+  HIP_TRY(gpu::launch_megakernel_batch(bp, block_size, block_begin[m],
+                                       settings.max_blocks_per_launch_gpu, s));
+  // items whose rays span several sample blocks: the second reduction pass
+  for (uint32_t j = 0; j < m; ++j) {
+    const WorkDivision &wd = mb[j].wd;
+    if (wd.num_sample_blocks == 1)
+      continue;
+    const uint32_t nr = items[act[j]].n_rays;
+    for (uint32_t k = 0; k < n_img; ++k)
+      HIP_TRY(gpu::launch_reduce_sample_blocks(
+          r.rgb.ptr + mb[j].out_off + size_t(k) * wd.num_sample_blocks * nr * 3,
+          wd.num_sample_blocks, nr, spp, r.rgb2.ptr + final_off[j] + size_t(k) * nr * 3, s));
+  }
+  HIP_TRY(hipEventRecord(r.ev_stop, s));
+  // results: one copy per buffer into pinned staging, then a parallel scatter
+  const size_t out_bytes = out_total * sizeof(float), fin_bytes = final_off[m] * sizeof(float);
+  HIP_TRY(r.b_stage_out.ensure(out_bytes + fin_bytes + sizeof(unsigned long long)));
+  char *ost = static_cast<char *>(r.b_stage_out.ptr);
+  HIP_TRY(hipMemcpyAsync(ost, r.rgb.ptr, out_bytes, hipMemcpyDeviceToHost, s));
+  if (fin_bytes)
+    HIP_TRY(hipMemcpyAsync(ost + out_bytes, r.rgb2.ptr, fin_bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(ost + out_bytes + fin_bytes, r.counter.ptr, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  parallel_for(m, [&](size_t j) {
+    const pt_ray_batch_item &it = items[act[j]];
+    const size_t bytes = size_t(n_img) * it.n_rays * 3 * sizeof(float);
+    const char *src = mb[j].wd.num_sample_blocks == 1
+                          ? ost + mb[j].out_off * sizeof(float)
+                          : ost + out_bytes + final_off[j] * sizeof(float);
+    std::memcpy(it.out, src, bytes);
+  });
+  unsigned long long rays_traced = 0;
+  std::memcpy(&rays_traced, ost + out_bytes + fin_bytes, sizeof(rays_traced));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
+  r.stats.kernel_ms = ms;
+  r.stats.rays = rays_traced;
+  r.stats.samples = ray_total * spp;
+  r.stats.mega_launches = 1;
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return PT_OK;

@@ -208,6 +208,10 @@ struct pt_renderer {
   std::vector<pt::gpu::SceneDesc> batch_desc;
   uint64_t batch_pixels = 0, batch_samples = 0;
 
+  // batched ray-sample-spec renders: per-item descriptors and grid prefix
+  pt::host::DeviceBuf<pt::gpu::MegaBatchItem> mb_items;
+  pt::host::DeviceBuf<uint32_t> mb_blocks;
+
   // adjoint buffers
   pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;
 
@@ -269,6 +273,9 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
                   uint32_t iters_per_sync);
 int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                  uint32_t spp, const pt_settings &settings, double *seconds);
+int render_rays_batch(pt_renderer &r, const pt_ray_batch_item *items, uint32_t n,
+                      uint32_t spp, uint32_t n_steps, const pt_settings &settings,
+                      double *seconds);
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[5]);
 int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_world[12],

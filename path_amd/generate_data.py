@@ -5,10 +5,13 @@ renderer: `generate_data`, `generate_data_for_image`, `deinit_renderers`,
 with the reference's field names, tensor shapes and feature layouts
 (src/generate_data/full_scene/generate_data.cpp:36-323).
 
-Per scene the radiance values come from one GPU render with an
-InitialIdxAndDir sample spec (each sample starts on a given triangle point
-with a given direction) and OutputPerStep output (one value per path step),
-as in the reference.
+The radiance values come from InitialIdxAndDir sample specs (each sample
+starts on a given triangle point with a given direction) with OutputPerStep
+output (one value per path step), as in the reference, but all scenes of a
+call are rendered in one batched launch (pt_render_rays_batch) where the
+reference calls Renderer::render once per scene
+(src/generate_data/full_scene/generate_data.cpp:280-294); each scene's values
+equal its own render.
 
 Differences (each deliberate): scenes come from path_amd's seeded
 box/sphere generator (`datagen.generate_scene`, seeds base_seed, base_seed+1,
@@ -189,14 +192,6 @@ def uniform_sphere(rng, n):
     return np.stack([st * np.cos(phi), st * np.sin(phi), ct], 1)
 
 
-def _render_values(r, scene, rays, idx, n_samples, n_steps, settings):
-    if len(rays) == 0:
-        return np.zeros((n_steps, 0, 3), np.float32)
-    img, _ = r.render(scene, None, 0, 0, n_samples, settings, output="OutputPerStep",
-                      rays=rays, initial_idx=idx, n_steps=n_steps)
-    return img[:, 0]  # [n_steps, n, 3]
-
-
 def _settings():
     s = Settings()
     s.kernel_approach = "MegaKernel"
@@ -204,11 +199,13 @@ def _settings():
     return s
 
 
-def _generate(max_tris, forced_n_scenes, per, n_samples, n_steps, base_seed, image):
+def sample_specs(max_tris, forced_n_scenes, per, base_seed, image):
+    """The scenes and sample specifications of one generate_data(_for_image)
+    call: a list of (GeneratedScene, flat scene, vertices, coords, triangle
+    indexes, rays (n, 6), image indexes or None) per scene."""
     rng = np.random.default_rng(base_seed)
     r = _renderer()
-    settings = _settings()
-    scenes, items = [], []
+    scenes = []
     tri_count = 0
     seed = base_seed
     while (len(scenes) < forced_n_scenes) if forced_n_scenes is not None else \
@@ -221,21 +218,10 @@ def _generate(max_tris, forced_n_scenes, per, n_samples, n_steps, base_seed, ima
         verts = np.stack([tris[:, 0], tris[:, 0] + tris[:, 1], tris[:, 0] + tris[:, 2]], 1)
         tri_count += len(verts)
         scenes.append((g, flat, verts))
-    max_t = max(len(v) for _, _, v in scenes)
-    S = len(scenes)
-    tf = np.zeros((S, max_t, constants.n_tri_values), np.float32)
-    bf = np.zeros((S, max_t, constants.n_bsdf_values), np.float32)
-    ev = np.zeros((S, max_t, 3), np.float32)
-    mask = np.ones((S, max_t), bool)
-    per_scene = []
-    for si, (g, flat, verts) in enumerate(scenes):
+    out = []
+    for g, flat, verts in scenes:
         n = len(verts)
-        feats, norm = triangle_features(verts)
-        tf[si, :n] = feats
-        mats = [flat["materials"][i] for i in range(flat["n_materials"])]
-        bf[si, :n] = bsdf_features(mats, flat["tri_mat"])
-        ev[si, :n] = np.array([list(mats[m].emission) for m in flat["tri_mat"]])
-        mask[si, :n] = False
+        _, norm = triangle_features(verts)
         if image:
             dim = per
             ftw = camera_from_look((0, 0, -1), (0, 1, 0), (0, 0, 10), 45.0, 1.0)
@@ -286,24 +272,47 @@ def _generate(max_tris, forced_n_scenes, per, n_samples, n_steps, base_seed, ima
             image_idx = None
         coords = coord_features(s, tt, norm, idx, dirs)
         rays = np.concatenate([origins, dirs], 1).astype(np.float32)
-        values = _render_values(r, g.scene, rays, idx.astype(np.uint32), n_samples, n_steps,
-                                settings)
-        per_scene.append((coords, idx, values, image_idx))
-    N = max(len(c) for c, _, _, _ in per_scene)
+        out.append((g, flat, verts, coords, idx, rays, image_idx))
+    return out
+
+
+def _generate(max_tris, forced_n_scenes, per, n_samples, n_steps, base_seed, image):
+    r = _renderer()
+    specs = sample_specs(max_tris, forced_n_scenes, per, base_seed, image)
+    # every scene's samples in one batched launch
+    values = r.render_rays_batch([(g.scene, rays, idx.astype(np.uint32))
+                                  for g, _, _, _, idx, rays, _ in specs],
+                                 n_samples, n_steps, _settings())
+    tri_count = sum(len(v) for _, _, v, _, _, _, _ in specs)
+    max_t = max(len(v) for _, _, v, _, _, _, _ in specs)
+    S = len(specs)
+    tf = np.zeros((S, max_t, constants.n_tri_values), np.float32)
+    bf = np.zeros((S, max_t, constants.n_bsdf_values), np.float32)
+    ev = np.zeros((S, max_t, 3), np.float32)
+    mask = np.ones((S, max_t), bool)
+    for si, (g, flat, verts, _, _, _, _) in enumerate(specs):
+        n = len(verts)
+        feats, _ = triangle_features(verts)
+        tf[si, :n] = feats
+        mats = [flat["materials"][i] for i in range(flat["n_materials"])]
+        bf[si, :n] = bsdf_features(mats, flat["tri_mat"])
+        ev[si, :n] = np.array([list(mats[m].emission) for m in flat["tri_mat"]])
+        mask[si, :n] = False
+    N = max(len(c) for _, _, _, c, _, _, _ in specs)
     bc = np.zeros((S, N, constants.n_coords_feature_values), np.float32)
     ti = np.zeros((S, N), np.int64)
     vals = np.zeros((S, n_steps, N, 3), np.float32)
     ii = np.zeros((S, N, 2), np.int64)
-    for si, (coords, idx, values, image_idx) in enumerate(per_scene):
+    for si, ((_, _, _, coords, idx, _, image_idx), v) in enumerate(zip(specs, values)):
         k = len(coords)
         bc[si, :k] = coords
         ti[si, :k] = idx
-        vals[si, :, :k] = values
+        vals[si, :, :k] = v
         if image_idx is not None:
             ii[si, :k] = image_idx
     inputs = NetworkInputs(torch.from_numpy(tf), torch.from_numpy(mask), torch.from_numpy(bf),
                            torch.from_numpy(ev), torch.from_numpy(bc), torch.from_numpy(ti),
-                           int(tri_count), [len(c) for c, _, _, _ in per_scene])
+                           int(tri_count), [len(c) for _, _, _, c, _, _, _ in specs])
     std = StandardData(inputs, torch.from_numpy(vals))
     return ImageData(std, torch.from_numpy(ii)) if image else std
 

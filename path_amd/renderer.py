@@ -339,6 +339,37 @@ class Renderer:
                                           L.fptr(ga), L.fptr(ge), C.byref(secs)))
         return ga, ge, secs.value
 
+    def render_rays_batch(self, items, samples_per: int, n_steps: int = 0,
+                          settings: Settings | None = None):
+        """Ray sample specs of many scenes in one batched launch
+        (pt_render_rays_batch). items: (scene, rays (n, 6), initial_idx (n,)
+        or None); all with or all without initial_idx. Returns per item an
+        array (n_steps, n, 3) (OutputPerStep), or (n, 3) when n_steps = 0
+        (FloatRGB); each equals render() of that item alone with the
+        MegaKernel integrator."""
+        settings = settings or Settings()
+        arr = (L.PtRayBatchItem * max(len(items), 1))()
+        keep, outs = [], []
+        for i, (scene, rays, idx) in enumerate(items):
+            rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+            n = rays.shape[0]
+            out = np.zeros((max(n_steps, 1), n, 3), np.float32)
+            keep.append(rays)
+            arr[i].scene = scene.handle
+            arr[i].rays = L.fptr(rays)
+            arr[i].n_rays = n
+            arr[i].out = L.fptr(out)
+            if idx is not None:
+                idx = np.ascontiguousarray(idx, np.uint32).ravel()
+                assert idx.size == n
+                keep.append(idx)
+                arr[i].initial_idx = L.uptr(idx)
+            outs.append(out if n_steps else out[0])
+        secs = C.c_double()
+        L.check(L.lib().pt_render_rays_batch(self.handle, arr, len(items), samples_per, n_steps,
+                                             C.byref(settings.raw), C.byref(secs)))
+        return outs
+
     def reduce_samples(self, wds: L.PtWorkDivisionSettings, values: np.ndarray,
                        num_locations: int, samples_per: int) -> np.ndarray:
         """The renderer's per-pixel reduction on the GPU applied to

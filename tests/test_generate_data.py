@@ -68,3 +68,32 @@ def test_generate_data_shapes_and_values(renderer):
     assert img.standard.values.shape[1] == 3
     moved = d.to(__import__("torch").zeros(1, dtype=__import__("torch").float64))
     assert moved.values.dtype == __import__("torch").float64
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("image", [False, True])
+def test_generate_data_values_equal_cpu_checker(renderer, image):
+    """Every scene's InitialIdxAndDir + OutputPerStep values, rendered in one
+    batched launch, equal the CPU checker's per-step values for the same
+    sample specification bit for bit."""
+    from oracle import oracle as O
+    from path_amd.generate_data import (_settings, generate_data, generate_data_for_image,
+                                        sample_specs)
+    n_samples, n_steps = 16, 4
+    if image:
+        d = generate_data_for_image(0, 3, 24, n_samples, n_steps, 5).standard
+        specs = sample_specs(0, 3, 24, 5, True)
+    else:
+        d = generate_data(0, 3, 3, n_samples, n_steps, 11)
+        specs = sample_specs(0, 3, 3, 11, False)
+    st = _settings()
+    tab = O.sobol_table(O.joe_kuo_directions(st.max_sample_size), st.max_sample_size, n_samples)
+    vals = d.values.numpy()
+    for si, (g, flat, _, coords, idx, rays, _) in enumerate(specs):
+        k = len(rays)
+        ref, _ = O.render(flat, np.zeros(12, np.float32), st, n_samples, k, 1, tab, 32, 0,
+                          initial_rays=rays, initial_idx=idx.astype(np.uint32), n_steps=n_steps)
+        got = np.ascontiguousarray(vals[si, :, :k])
+        assert np.array_equal(got.view(np.uint32), ref[:, 0].view(np.uint32)), si
+        assert np.array_equal(d.inputs.triangle_idxs_for_coords.numpy()[si, :k], idx)
+    assert vals.sum() > 0

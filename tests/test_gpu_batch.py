@@ -168,3 +168,58 @@ def test_resident_batch_with_moved_cameras(renderer, gen_scenes):
     back, _, _ = render_batch(renderer, scenes, SPP, st)
     for a, b in zip(first, back):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _ray_items(gen_scenes, n_each, with_idx, seed=4):
+    """Rays starting 1 unit in front of random triangles of each scene, aimed at
+    them (an InitialIdxAndDir spec names that triangle as the first hit)."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for k, g in enumerate(gen_scenes):
+        tris = g.scene.flat()["tris"].reshape(-1, 3, 3)
+        n = 0 if k == 1 else n_each + 7 * k  # one empty item, uneven sizes
+        pick = rng.integers(0, len(tris), n).astype(np.uint32)
+        c = tris[pick, 0] + (tris[pick, 1] + tris[pick, 2]) / 3.0
+        nrm = np.cross(tris[pick, 1], tris[pick, 2])
+        nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+        rays = np.concatenate([c + nrm, -nrm], 1).astype(np.float32)
+        items.append((g.scene, rays, pick if with_idx else None))
+    return items
+
+
+@pytest.mark.parametrize("with_idx,n_steps,spp", [(True, 5, 16), (False, 0, 16),
+                                                  (True, 3, 4096)])
+def test_ray_batch_equals_single_renders(renderer, gen_scenes, with_idx, n_steps, spp):
+    """pt_render_rays_batch (the reference datagen's per-scene InitialIdxAndDir
+    + OutputPerStep renders, batched) equals pt_render of each item alone,
+    bit for bit; 4096 spp over few rays spans several sample blocks per ray
+    (the second reduction pass)."""
+    from path_amd.renderer import Settings
+    s = Settings()
+    s.kernel_approach = "MegaKernel"
+    s.back_cull_emission = 0
+    items = _ray_items(gen_scenes[:4], 9 if spp > 1000 else 120, with_idx)
+    outs = renderer.render_rays_batch(items, spp, n_steps, s)
+    assert renderer.last_stats().samples == sum(len(r) for _, r, _ in items) * spp
+    for (scene, rays, idx), got in zip(items, outs):
+        if len(rays) == 0:
+            assert got.size == 0
+            continue
+        if n_steps:
+            ref, _ = renderer.render(scene, None, 0, 0, spp, s, output="OutputPerStep",
+                                     rays=rays, initial_idx=idx, n_steps=n_steps)
+            ref = ref[:, 0]
+        else:
+            ref, _ = renderer.render(scene, None, 0, 0, spp, s, rays=rays, initial_idx=idx)
+            ref = ref[0]
+        assert got.shape == ref.shape
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        assert got.sum() > 0
+
+
+def test_ray_batch_rejects_mixed_specs(renderer, gen_scenes):
+    from path_amd._lib import PathError
+    items = _ray_items(gen_scenes[:3], 10, True)
+    items[2] = (items[2][0], items[2][1], None)
+    with pytest.raises(PathError):
+        renderer.render_rays_batch(items, 4, 2)
