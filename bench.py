@@ -800,6 +800,10 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (scale for the "
                          "profiler's FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--spatial-splits", action="store_true",
+                    help="build the scene's tree with spatial splits (use_spatial_splits)")
+    ap.add_argument("--overlap-threshold", type=float, default=0.0,
+                    help="with --spatial-splits: overlap_threshold (default 1e-5)")
     ap.add_argument("--pipelines", type=int, default=0,
                     help="wavefront pipelines per render (default 3)")
     ap.add_argument("--mlp-forward", action="store_true",
@@ -879,6 +883,10 @@ def bench_render(a, cfg):
     settings = make_settings(cfg)
     if a.max_paths:
         settings.max_num_samples_per_launch = a.max_paths
+    if getattr(a, "spatial_splits", False):
+        settings.use_spatial_splits = 1
+        if a.overlap_threshold:
+            settings.overlap_threshold = a.overlap_threshold
     n_tris = scene.counts()[0]
     r = Renderer(0)
     if a.pipelines:
@@ -943,7 +951,9 @@ def bench_render(a, cfg):
         "host_output_msamples_per_s": round(host_rate, 3),
         "config": {"workload": cfg["workload"], "width": size, "height": size, "spp": spp,
                    "max_depth": cfg["iters"] + 1, "triangles": n_tris,
-                   "integrator": cfg["approach"], "sampler": "Sobel"},
+                   "integrator": cfg["approach"], "sampler": "Sobel",
+                   "spatial_splits": bool(settings.use_spatial_splits),
+                   "tree_references": int(len(scene.bvh()["perm"]))},
         "mrays_per_s": round(rays / elapsed / 1e6, 2),
     }
 

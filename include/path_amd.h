@@ -274,12 +274,25 @@ int pt_reduce_samples(pt_renderer *r, const pt_work_division_settings *s,
                       float *out);
 /* Host SAH BVH of the flattened scene in the reference node layout
  * (node.h:22-126): per node lo[3], hi[3] (float) and is_leaf, a, b (uint32);
- * perm[n_tris]. This is the binary tree the GPU's 4-wide tree is collapsed
- * from (SAH traversal cost 0.5, the reference default). Pass NULL arrays to
- * query n_nodes only. */
-int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
-                 uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
+ * a leaf covers perm[a..b), perm[n_refs] holding triangle indices (n_refs =
+ * n_tris, or more when spatial splits reference a triangle from several
+ * leaves). This is the binary tree the GPU's 4-wide tree is collapsed from,
+ * built with the accel settings of the scene's last render (or
+ * pt_scene_set_accel). Pass NULL arrays to query n_nodes and n_refs only. */
+int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, uint32_t *n_refs, float *lo,
+                 float *hi, uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
                  uint32_t *max_depth);
+/* Tree build settings for the scene from the accel fields of settings
+ * (traversal_per_intersect_cost, use_spatial_splits, overlap_threshold;
+ * include/intersect/accel/sbvh/settings.h:9-19). Renders apply their own
+ * settings the same way; a change rebuilds the tree on next use. */
+int pt_scene_set_accel(pt_scene *scene, const pt_settings *settings);
+/* Bounds of the part of triangle tri (3 vertices) inside the box [lo, hi]
+ * (empty, out_lo > out_hi, when they do not meet): the clipping a spatial
+ * split applies to a triangle reference (ClippedTriangle::new_bounds,
+ * include/intersect/accel/detail/clipped_triangle_impl.h:13-124). */
+int pt_clip_triangle_bounds(const float tri[9], const float lo[3], const float hi[3],
+                            float out_lo[3], float out_hi[3]);
 
 /* ------------------------------------------------------------------------ */
 /* Batched rendering for training-data generation. No reference counterpart */

@@ -235,14 +235,16 @@ int pt_camera_from_look(const float look[3], const float up[3],
   return PT_OK;
 }
 
-int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
-                 uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
+int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, uint32_t *n_refs, float *lo,
+                 float *hi, uint32_t *is_leaf, uint32_t *a, uint32_t *b, uint32_t *perm,
                  uint32_t *max_depth) {
   PT_GUARD_BEGIN
   if (!scene || !n_nodes)
     return fail(PT_ERR_INVALID, "null argument");
   const Bvh &bvh = scene->bvh();
   *n_nodes = uint32_t(bvh.nodes.size());
+  if (n_refs)
+    *n_refs = uint32_t(bvh.perm.size());
   if (max_depth)
     *max_depth = bvh.max_depth;
   for (size_t i = 0; i < bvh.nodes.size(); ++i) {
@@ -262,6 +264,25 @@ int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, float *lo, float *hi,
     std::memcpy(perm, bvh.perm.data(), bvh.perm.size() * sizeof(uint32_t));
   return PT_OK;
   PT_GUARD_END
+}
+
+int pt_scene_set_accel(pt_scene *scene, const pt_settings *settings) {
+  PT_GUARD_BEGIN
+  if (!scene || !settings)
+    return fail(PT_ERR_INVALID, "null argument");
+  scene->use_tree_settings(tree_settings(*settings));
+  return PT_OK;
+  PT_GUARD_END
+}
+
+int pt_clip_triangle_bounds(const float tri[9], const float lo[3], const float hi[3],
+                            float out_lo[3], float out_hi[3]) {
+  if (!tri || !lo || !hi || !out_lo || !out_hi)
+    return fail(PT_ERR_INVALID, "null argument");
+  const V3<float> v[3] = {{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]},
+                          {tri[6], tri[7], tri[8]}};
+  clip_triangle_bounds(v, lo, hi, out_lo, out_hi);
+  return PT_OK;
 }
 
 // ------------------------------ settings --------------------------------

@@ -53,7 +53,7 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
                            "with fixed samples");
   auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipSetDevice(r.device));
-  int rc = r.ensure_scene(scene);
+  int rc = r.ensure_scene(scene, &settings);
   if (rc != PT_OK)
     return rc;
   gpu::AdjointParams p{};

@@ -262,8 +262,12 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
 // tree build falls back, r.gpu_bvh is cleared and nothing is uploaded: the
 // caller repeats the call with host trees (and restores r.gpu_bvh).
 int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp,
-                        bool gpu_bvh, std::vector<gpu::SceneDesc> *desc_out,
-                        uint64_t *n_pixels_out, uint64_t *total_out) {
+                        bool gpu_bvh, const pt_settings &settings,
+                        std::vector<gpu::SceneDesc> *desc_out, uint64_t *n_pixels_out,
+                        uint64_t *total_out) {
+  if (!gpu_bvh)
+    for (uint32_t i = 0; i < n; ++i)
+      items[i].scene->use_tree_settings(tree_settings(settings));
   std::vector<uint64_t> key;
   key.reserve(3 * size_t(n) + 2);
   key.push_back(spp);
@@ -390,7 +394,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   hipStream_t s = r.stream;
   std::vector<gpu::SceneDesc> desc;
   uint64_t n_pixels = 0, total = 0;
-  int rc = ensure_batch_scenes(r, items, n, spp, gpu_bvh, &desc, &n_pixels, &total);
+  int rc = ensure_batch_scenes(r, items, n, spp, gpu_bvh, settings, &desc, &n_pixels, &total);
   if (rc != PT_OK)
     return rc;
   if (gpu_bvh && !r.gpu_bvh) {
@@ -580,7 +584,8 @@ int render_rays_batch(pt_renderer &r, const pt_ray_batch_item *items, uint32_t n
   }
   std::vector<gpu::SceneDesc> desc;
   uint64_t n_pixels = 0, total = 0;
-  int rc = ensure_batch_scenes(r, bitems.data(), m, spp, false, &desc, &n_pixels, &total);
+  int rc = ensure_batch_scenes(r, bitems.data(), m, spp, false, settings, &desc, &n_pixels,
+                               &total);
   if (rc != PT_OK)
     return rc;
   // per-item work divisions, grid blocks, ray and output offsets

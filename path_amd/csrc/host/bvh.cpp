@@ -36,87 +36,300 @@ struct Box {
   float centroid(int axis) const { return (lo[axis] + hi[axis]) / 2.f; }
 };
 
+// One reference to a triangle: its id and the bounds of the part of it this
+// reference stands for (the whole triangle until a spatial split cuts it).
+struct Ref {
+  Box box;
+  uint32_t tri;
+};
+
+Box intersect(const Box &a, const Box &b) {
+  Box o;
+  for (int i = 0; i < 3; ++i) {
+    o.lo[i] = std::max(a.lo[i], b.lo[i]);
+    o.hi[i] = std::min(a.hi[i], b.hi[i]);
+  }
+  return o;
+}
+
+bool is_empty(const Box &b) {
+  return b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2];
+}
+
+// Spatial-split search resolution: planes per axis of a node with n
+// references (2n, as the reference's generator.cu:443, capped so a node's
+// search stays linear in n).
+constexpr uint32_t kMaxSpatialBins = 256;
+// No spatial splits below this depth of the binary tree: the GPU's traversal
+// stack bounds the collapsed tree's depth.
+constexpr uint32_t kMaxSpatialDepth = 40;
+
 struct Builder {
-  const std::vector<Box> *bounds;
+  const std::vector<TriGeom<float>> *tris;
   BvhSettings settings;
   std::vector<BvhNode> nodes;
-  uint32_t max_depth = 0, max_leaf = 0;
+  std::vector<uint32_t> perm;
+  uint32_t max_depth = 0, max_leaf = 0, n_spatial = 0;
+  float root_sa = 0.f;
+  size_t n_refs = 0, ref_budget = 0;
 
   struct Split {
     float cost;
     uint32_t point;
     int axis;
+    float overlap_sa; // surface area of left box ∩ right box
   };
 
-  Split best_split(std::vector<uint32_t> &idx, int axis,
-                   std::vector<uint32_t> &sorted_out) {
-    sorted_out = idx;
-    const auto &bd = *bounds;
-    std::stable_sort(sorted_out.begin(), sorted_out.end(),
-                     [&](uint32_t l, uint32_t r) {
-                       return bd[l].centroid(axis) < bd[r].centroid(axis);
-                     });
+  Split best_object_split(const std::vector<Ref> &refs, int axis,
+                          std::vector<uint32_t> &sorted_out) {
+    sorted_out.resize(refs.size());
+    for (size_t i = 0; i < refs.size(); ++i)
+      sorted_out[i] = uint32_t(i);
+    std::stable_sort(sorted_out.begin(), sorted_out.end(), [&](uint32_t l, uint32_t r) {
+      return refs[l].box.centroid(axis) < refs[r].box.centroid(axis);
+    });
     size_t n = sorted_out.size();
-    std::vector<float> back_sa(n);
+    std::vector<Box> back(n);
     Box run = Box::empty();
     for (size_t i = n; i-- > 0;) {
-      run.grow(bd[sorted_out[i]]);
-      back_sa[i] = run.surface_area();
+      run.grow(refs[sorted_out[i]].box);
+      back[i] = run;
     }
     float best = std::numeric_limits<float>::max();
     uint32_t best_i = 0;
+    float best_overlap = 0.f;
     Box left = Box::empty();
     for (size_t i = 0; i < n; ++i) {
-      float cost = float(i) * left.surface_area() + float(n - i) * back_sa[i];
+      float cost = float(i) * left.surface_area() + float(n - i) * back[i].surface_area();
       if (cost < best) {
         best = cost;
         best_i = uint32_t(i);
+        best_overlap = intersect(left, back[i]).surface_area();
       }
-      left.grow(bd[sorted_out[i]]);
+      left.grow(refs[sorted_out[i]].box);
     }
-    return {best, best_i, axis};
+    return {best, best_i, axis, best_overlap};
   }
 
-  // Builds node `self` from the triangle ids in idx (reordered in place into
-  // leaf order); leaf ranges start at start_idx.
-  void create(uint32_t self, std::vector<uint32_t> &idx, uint32_t start_idx,
-              uint32_t depth) {
+  // Part of reference r between axis planes a <= b.
+  Box chop(const Ref &r, int axis, float a, float b) const {
+    Box slab = r.box;
+    slab.lo[axis] = std::max(slab.lo[axis], a);
+    slab.hi[axis] = std::min(slab.hi[axis], b);
+    const TriGeom<float> &t = (*tris)[r.tri];
+    const V3<float> v[3] = {t.v0, t.v0 + t.e1, t.v0 + t.e2};
+    Box o;
+    clip_triangle_bounds(v, slab.lo, slab.hi, o.lo, o.hi);
+    // Rounding can leave a sliver empty; its slab bound is conservative.
+    return is_empty(o) ? slab : o;
+  }
+
+  struct Spatial {
+    float cost = std::numeric_limits<float>::max();
+    int axis = 0;
+    float plane = 0.f;
+    std::vector<Ref> left, right;
+  };
+
+  // Binned spatial split on one axis (Stich, Friedrich, Dietrich, "Spatial
+  // Splits in Bounding Volume Hierarchies", HPG 2009): references are chopped
+  // into the bins they cross; a plane's cost counts a reference on each side
+  // it enters. The chosen plane's straddlers are then offered "unsplitting"
+  // (kept whole on one side when that is cheaper), and the cost returned is
+  // that of the final partition.
+  void spatial_split(const std::vector<Ref> &refs, const Box &overall, int axis,
+                     Spatial &best) const {
+    const float lo = overall.lo[axis], hi = overall.hi[axis];
+    if (!(hi > lo))
+      return;
+    const uint32_t nb = uint32_t(std::min<size_t>(kMaxSpatialBins, 2 * refs.size()));
+    std::vector<float> plane(nb + 1);
+    for (uint32_t k = 0; k < nb; ++k)
+      plane[k] = lo + (hi - lo) * (float(k) / float(nb));
+    plane[nb] = hi;
+    // lower bin: plane[b] <= x; upper bin: x <= plane[b + 1]
+    auto bin_lo = [&](float x) {
+      int b = int(std::floor((x - lo) / (hi - lo) * float(nb)));
+      b = std::min(std::max(b, 0), int(nb) - 1);
+      while (b > 0 && plane[b] > x)
+        --b;
+      while (b + 1 < int(nb) && plane[b + 1] <= x)
+        ++b;
+      return b;
+    };
+    auto bin_hi = [&](float x) {
+      int b = int(std::floor((x - lo) / (hi - lo) * float(nb)));
+      b = std::min(std::max(b, 0), int(nb) - 1);
+      while (b + 1 < int(nb) && plane[b + 1] < x)
+        ++b;
+      while (b > 0 && plane[b] >= x)
+        --b;
+      return b;
+    };
+    std::vector<Box> bins(nb, Box::empty());
+    std::vector<uint32_t> enter(nb, 0), leave(nb, 0);
+    for (const Ref &r : refs) {
+      const int b0 = bin_lo(r.box.lo[axis]);
+      const int b1 = std::max(b0, bin_hi(r.box.hi[axis]));
+      ++enter[b0];
+      ++leave[b1];
+      if (b0 == b1) {
+        bins[b0].grow(r.box);
+        continue;
+      }
+      for (int k = b0; k <= b1; ++k)
+        bins[k].grow(chop(r, axis, plane[k], plane[k + 1]));
+    }
+    std::vector<Box> back(nb);
+    Box run = Box::empty();
+    for (uint32_t k = nb; k-- > 0;) {
+      run.grow(bins[k]);
+      back[k] = run;
+    }
+    Box left = Box::empty();
+    uint32_t n_left = 0, n_right = uint32_t(refs.size());
+    float est = std::numeric_limits<float>::max();
+    uint32_t best_k = 0;
+    for (uint32_t k = 1; k < nb; ++k) {
+      left.grow(bins[k - 1]);
+      n_left += enter[k - 1];
+      n_right -= leave[k - 1];
+      const float cost = float(n_left) * left.surface_area() +
+                         float(n_right) * back[k].surface_area();
+      if (cost < est) {
+        est = cost;
+        best_k = k;
+      }
+    }
+    if (best_k == 0 || !(est < best.cost))
+      return;
+    // partition at plane[best_k]
+    const float p = plane[best_k];
+    Spatial s;
+    s.axis = axis;
+    s.plane = p;
+    Box bl = Box::empty(), br = Box::empty();
+    std::vector<uint32_t> straddle;
+    for (uint32_t i = 0; i < refs.size(); ++i) {
+      const Ref &r = refs[i];
+      if (r.box.lo[axis] >= p) {
+        s.right.push_back(r);
+        br.grow(r.box);
+      } else if (r.box.hi[axis] <= p) {
+        s.left.push_back(r);
+        bl.grow(r.box);
+      } else {
+        straddle.push_back(i);
+      }
+    }
+    std::vector<Box> cut_l(straddle.size()), cut_r(straddle.size());
+    for (size_t j = 0; j < straddle.size(); ++j) {
+      const Ref &r = refs[straddle[j]];
+      cut_l[j] = chop(r, axis, std::numeric_limits<float>::lowest(), p);
+      cut_r[j] = chop(r, axis, p, std::numeric_limits<float>::max());
+      bl.grow(cut_l[j]);
+      br.grow(cut_r[j]);
+    }
+    float nl = float(s.left.size() + straddle.size());
+    float nr = float(s.right.size() + straddle.size());
+    for (size_t j = 0; j < straddle.size(); ++j) {
+      const Ref &r = refs[straddle[j]];
+      Box ul = bl, ur = br;
+      ul.grow(r.box);
+      ur.grow(r.box);
+      const float c_split = nl * bl.surface_area() + nr * br.surface_area();
+      const float c_left = nl * ul.surface_area() + (nr - 1.f) * br.surface_area();
+      const float c_right = (nl - 1.f) * bl.surface_area() + nr * ur.surface_area();
+      if (c_left < c_split && c_left <= c_right) {
+        s.left.push_back(r);
+        bl = ul;
+        nr -= 1.f;
+      } else if (c_right < c_split) {
+        s.right.push_back(r);
+        br = ur;
+        nl -= 1.f;
+      } else {
+        s.left.push_back({cut_l[j], r.tri});
+        s.right.push_back({cut_r[j], r.tri});
+      }
+    }
+    if (s.left.empty() || s.right.empty())
+      return;
+    Box fl = Box::empty(), fr = Box::empty();
+    for (const Ref &r : s.left)
+      fl.grow(r.box);
+    for (const Ref &r : s.right)
+      fr.grow(r.box);
+    s.cost = float(s.left.size()) * fl.surface_area() + float(s.right.size()) * fr.surface_area();
+    if (s.cost < best.cost)
+      best = std::move(s);
+  }
+
+  // Builds node `self` from the references in refs; leaves append their
+  // triangles to perm in depth-first order, so every leaf covers a
+  // contiguous range.
+  void create(uint32_t self, std::vector<Ref> &refs, uint32_t depth) {
     max_depth = std::max(max_depth, depth);
-    const auto &bd = *bounds;
     Box overall = Box::empty();
-    for (uint32_t i : idx)
-      overall.grow(bd[i]);
-    Split best{std::numeric_limits<float>::max(), 0, 0};
+    for (const Ref &r : refs)
+      overall.grow(r.box);
+    if (depth == 0)
+      root_sa = overall.surface_area();
+    Split best{std::numeric_limits<float>::max(), 0, 0, 0.f};
     std::vector<uint32_t> best_order, tmp;
     for (int axis = 0; axis < 3; ++axis) {
-      Split s = best_split(idx, axis, tmp);
+      Split s = best_object_split(refs, axis, tmp);
       if (s.cost < best.cost || axis == 0) {
         best = s;
         best_order.swap(tmp);
       }
     }
-    float cost = best.cost / overall.surface_area() +
-                 settings.traversal_per_intersect_cost;
+    // reference generator.cu:112-126: spatial splits are tried when the best
+    // object split's children overlap by more than alpha x the root's area
+    Spatial sp;
+    if (settings.spatial_splits && depth < kMaxSpatialDepth && n_refs < ref_budget &&
+        best.overlap_sa / root_sa > settings.overlap_threshold) {
+      sp.cost = best.cost;
+      for (int axis = 0; axis < 3; ++axis)
+        spatial_split(refs, overall, axis, sp);
+    }
+    const bool use_spatial = !sp.left.empty();
+    const float split_cost = use_spatial ? sp.cost : best.cost;
+    float cost = split_cost / overall.surface_area() + settings.traversal_per_intersect_cost;
     BvhNode node;
     node.lo = {overall.lo[0], overall.lo[1], overall.lo[2]};
     node.hi = {overall.hi[0], overall.hi[1], overall.hi[2]};
-    if (!(cost < float(idx.size()))) {
+    if (!(cost < float(refs.size()))) {
       node.is_leaf = 1;
-      node.a = start_idx;
-      node.b = start_idx + uint32_t(idx.size());
-      max_leaf = std::max(max_leaf, uint32_t(idx.size()));
+      node.a = uint32_t(perm.size());
+      for (const Ref &r : refs)
+        perm.push_back(r.tri);
+      node.b = uint32_t(perm.size());
+      max_leaf = std::max(max_leaf, uint32_t(refs.size()));
       nodes[self] = node;
       return;
     }
     uint32_t left = uint32_t(nodes.size());
     nodes.resize(nodes.size() + 2);
     uint32_t right = left + 1;
-    std::vector<uint32_t> l(best_order.begin(), best_order.begin() + best.point);
-    std::vector<uint32_t> r(best_order.begin() + best.point, best_order.end());
-    create(left, l, start_idx, depth + 1);
-    create(right, r, start_idx + uint32_t(l.size()), depth + 1);
-    std::copy(l.begin(), l.end(), idx.begin());
-    std::copy(r.begin(), r.end(), idx.begin() + l.size());
+    std::vector<Ref> l, r;
+    if (use_spatial) {
+      ++n_spatial;
+      n_refs += sp.left.size() + sp.right.size() - refs.size();
+      l = std::move(sp.left);
+      r = std::move(sp.right);
+    } else {
+      l.reserve(best.point);
+      r.reserve(refs.size() - best.point);
+      for (uint32_t i = 0; i < best.point; ++i)
+        l.push_back(refs[best_order[i]]);
+      for (size_t i = best.point; i < refs.size(); ++i)
+        r.push_back(refs[best_order[i]]);
+    }
+    std::vector<Ref>().swap(refs);
+    create(left, l, depth + 1);
+    std::vector<Ref>().swap(l);
+    create(right, r, depth + 1);
     node.is_leaf = 0;
     node.a = left;
     node.b = right;
@@ -126,12 +339,65 @@ struct Builder {
 
 } // namespace
 
+void clip_triangle_bounds(const V3<float> v[3], const float lo[3], const float hi[3],
+                          float out_lo[3], float out_hi[3]) {
+  // At most 3 + 6 vertices after six planes.
+  double poly[2][12][3];
+  int n = 3, cur = 0;
+  for (int k = 0; k < 3; ++k) {
+    poly[0][k][0] = v[k].x;
+    poly[0][k][1] = v[k].y;
+    poly[0][k][2] = v[k].z;
+  }
+  for (int axis = 0; axis < 3 && n > 0; ++axis)
+    for (int side = 0; side < 2 && n > 0; ++side) {
+      const double p = side ? hi[axis] : lo[axis];
+      auto inside = [&](const double *q) { return side ? q[axis] <= p : q[axis] >= p; };
+      const double(*in)[3] = poly[cur];
+      double(*out)[3] = poly[cur ^ 1];
+      int m = 0;
+      for (int i = 0; i < n; ++i) {
+        const double *a = in[i], *b = in[(i + 1) % n];
+        const bool ia = inside(a), ib = inside(b);
+        if (ia) {
+          out[m][0] = a[0], out[m][1] = a[1], out[m][2] = a[2];
+          ++m;
+        }
+        if (ia != ib) {
+          const double t = (p - a[axis]) / (b[axis] - a[axis]);
+          for (int c = 0; c < 3; ++c)
+            out[m][c] = a[c] + t * (b[c] - a[c]);
+          out[m][axis] = p;
+          ++m;
+        }
+      }
+      n = m;
+      cur ^= 1;
+    }
+  for (int c = 0; c < 3; ++c) {
+    out_lo[c] = std::numeric_limits<float>::max();
+    out_hi[c] = std::numeric_limits<float>::lowest();
+  }
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) {
+      const float x = float(poly[cur][i][c]);
+      out_lo[c] = std::min(out_lo[c], x);
+      out_hi[c] = std::max(out_hi[c], x);
+    }
+  if (n == 0)
+    return;
+  for (int c = 0; c < 3; ++c) {
+    out_lo[c] = std::max(out_lo[c], lo[c]);
+    out_hi[c] = std::min(out_hi[c], hi[c]);
+  }
+}
+
 Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
                   const BvhSettings &settings) {
   Bvh out;
   if (tris.empty())
     return out;
-  std::vector<Box> bounds(tris.size());
+  std::vector<Ref> refs(tris.size());
   for (size_t i = 0; i < tris.size(); ++i) {
     const auto &t = tris[i];
     V3<float> v[3] = {t.v0, t.v0 + t.e1, t.v0 + t.e2};
@@ -143,20 +409,22 @@ Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
         b.hi[k] = std::max(b.hi[k], c[k]);
       }
     }
-    bounds[i] = b;
+    refs[i] = {b, uint32_t(i)};
   }
   Builder b;
-  b.bounds = &bounds;
+  b.tris = &tris;
   b.settings = settings;
   b.nodes.resize(1);
-  std::vector<uint32_t> idx(tris.size());
-  for (size_t i = 0; i < idx.size(); ++i)
-    idx[i] = uint32_t(i);
-  b.create(0, idx, 0, 0);
+  b.perm.reserve(tris.size());
+  b.n_refs = tris.size();
+  // duplicates stop once references reach twice the triangle count
+  b.ref_budget = 2 * tris.size();
+  b.create(0, refs, 0);
   out.nodes = std::move(b.nodes);
-  out.perm = std::move(idx);
+  out.perm = std::move(b.perm);
   out.max_depth = b.max_depth;
   out.max_leaf = b.max_leaf;
+  out.spatial_splits = b.n_spatial;
   return out;
 }
 

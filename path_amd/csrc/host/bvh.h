@@ -25,14 +25,39 @@ struct BvhNode {
 
 struct Bvh {
   std::vector<BvhNode> nodes;
-  std::vector<uint32_t> perm; // perm[i] = flattened index of the i-th leaf-ordered triangle
+  // perm[i] = flattened index of the i-th leaf-ordered triangle reference
+  // (with spatial splits a triangle can be referenced by several leaves)
+  std::vector<uint32_t> perm;
   uint32_t max_depth = 0;
   uint32_t max_leaf = 0;
+  uint32_t spatial_splits = 0; // nodes split by a spatial split
 };
 
 struct BvhSettings {
   float traversal_per_intersect_cost = 0.5f; // reference default (sbvh/settings.h)
+  // Spatial splits (SBVH; reference sbvh/settings.h:13-14, off by default):
+  // a node also tries splitting planes that cut triangles when the children
+  // of its best object split overlap by more than overlap_threshold times the
+  // root's surface area. A cut triangle is referenced by both children.
+  // Measured on MI355X, 82k-triangle scene, wavefront integrator: object
+  // splits 756-758 Msamples/s; spatial splits at overlap thresholds 1e-5 /
+  // 1e-4 / 1e-3 / 1e-2 (95,930 / 82,751 / 82,513 / 82,224 references):
+  // 743 / 736 / 735 / 734. So they stay off by default, as in the reference.
+  bool spatial_splits = false;
+  float overlap_threshold = 1e-5f;
+  bool operator==(const BvhSettings &o) const {
+    return traversal_per_intersect_cost == o.traversal_per_intersect_cost &&
+           spatial_splits == o.spatial_splits && overlap_threshold == o.overlap_threshold;
+  }
 };
+
+// Bounds of the part of triangle v[0..2] inside the box [lo, hi] (inclusive
+// planes; empty bounds, lo > hi, if they do not meet): the triangle polygon
+// clipped by the six planes, Sutherland-Hodgman. The spatial-split builder
+// chops a reference at a plane with it (reference ClippedTriangle::new_bounds,
+// include/intersect/accel/detail/clipped_triangle_impl.h:13-124).
+void clip_triangle_bounds(const V3<float> v[3], const float lo[3], const float hi[3],
+                          float out_lo[3], float out_hi[3]);
 
 // Traversal cost of the tree the GPU traverses. Measured on MI355X (82k-
 // triangle scene, quantised 4-wide nodes, screened triangle test): 0.25 ->
@@ -42,11 +67,13 @@ struct BvhSettings {
 // on the tree, so images are unchanged.
 constexpr float kGpuSahTraversalCost = 0.5f;
 
-// SAH object-split builder following src/intersect/accel/sbvh/detail/generator.cu:
-// 16-438 (spatial splits off, the reference default): per node and axis, sort
-// by bound centroid, sweep all split points with cost i*SA_L + (n-i)*SA_R, make
-// a leaf when cost/SA + traversal_cost >= n. Centroid ties keep input order
-// (the reference's std::sort leaves them unspecified).
+// SAH builder following src/intersect/accel/sbvh/detail/generator.cu:16-713:
+// per node and axis, sort the references by bound centroid, sweep all object
+// split points with cost i*SA_L + (n-i)*SA_R; with spatial splits, also 32
+// planes per axis over the node's extent with the references chopped into
+// the bins they cross; make a leaf when cost/SA + traversal_cost >= n.
+// Centroid ties keep input order (the reference's std::sort leaves them
+// unspecified).
 Bvh build_sah_bvh(const std::vector<TriGeom<float>> &tris,
                   const BvhSettings &settings = {});
 

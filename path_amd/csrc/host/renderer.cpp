@@ -37,7 +37,7 @@ const FlatScene &pt_scene::flat() const {
 
 const Bvh &pt_scene::bvh() const {
   if (bvh_gen_ != generation) {
-    bvh_ = build_sah_bvh(flat().tris, BvhSettings{kGpuSahTraversalCost});
+    bvh_ = build_sah_bvh(flat().tris, tree_settings_);
     bvh_gen_ = generation;
   }
   return bvh_;
@@ -168,7 +168,9 @@ bool pt_renderer::init(int dev, std::string *err) {
   return noalloc_init(dev, err) == PT_OK;
 }
 
-int pt_renderer::ensure_scene(const pt_scene &scene) {
+int pt_renderer::ensure_scene(const pt_scene &scene, const pt_settings *settings) {
+  if (settings)
+    scene.use_tree_settings(tree_settings(*settings));
   if (cached_scene == &scene && cached_gen == scene.generation) {
     if (cached_mat_gen == scene.mat_generation)
       return PT_OK;
@@ -309,7 +311,7 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
     return set_error(PT_ERR_INVALID, "unknown kernel approach");
 
   HIP_TRY(hipSetDevice(device));
-  int rc = ensure_scene(scene);
+  int rc = ensure_scene(scene, &settings);
   if (rc != PT_OK)
     return rc;
   gpu::RngParams rng{};

@@ -20,6 +20,15 @@ int set_error(int code, const std::string &msg);
 // `device` (current), once per device (gpu/noalloc.hip).
 int noalloc_init(int device, std::string *err);
 inline int gpu_stack_size() { return gpu::kTraceStackSize; }
+// Tree build settings from the accel fields of render settings
+// (sbvh/settings.h, bvh/settings.h).
+inline BvhSettings tree_settings(const pt_settings &s) {
+  BvhSettings b;
+  b.traversal_per_intersect_cost = s.traversal_per_intersect_cost;
+  b.spatial_splits = s.use_spatial_splits != 0;
+  b.overlap_threshold = s.overlap_threshold;
+  return b;
+}
 } // namespace host
 } // namespace pt
 
@@ -48,8 +57,21 @@ uint64_t next_generation();
 // built on different threads at once.
 struct pt_scene {
   pt::host::Scene scene;
-  uint64_t generation = pt::host::next_generation();
+  // Generation of the scene's derived data (flattened arrays, tree): bumped
+  // by geometry edits and by a change of tree build settings.
+  mutable uint64_t generation = pt::host::next_generation();
   uint64_t mat_generation = 0;
+
+  // Build settings of the SAH tree; a render with other accel settings
+  // rebuilds the tree (as the reference builds its accel per render,
+  // include/render/detail/renderer_impl.h:40-73).
+  void use_tree_settings(const pt::host::BvhSettings &s) const {
+    if (!(s == tree_settings_)) {
+      tree_settings_ = s;
+      generation = pt::host::next_generation();
+    }
+  }
+  const pt::host::BvhSettings &tree_settings() const { return tree_settings_; }
 
   const pt::host::FlatScene &flat() const;
   const pt::host::Bvh &bvh() const;
@@ -62,6 +84,7 @@ private:
   mutable uint64_t flat_gen_ = ~0ull, flat_mat_gen_ = ~0ull, bvh_gen_ = ~0ull,
                    shade_gen_ = ~0ull, packed_gen_ = ~0ull, packed_mat_gen_ = ~0ull,
                    packed_mats_of_gen_ = ~0ull;
+  mutable pt::host::BvhSettings tree_settings_{pt::host::kGpuSahTraversalCost};
   mutable pt::host::FlatScene flat_;
   mutable pt::host::Bvh bvh_;
   mutable pt::host::PackedScene packed_;
@@ -246,7 +269,9 @@ struct pt_renderer {
 
   ~pt_renderer();
   bool init(int device, std::string *err);
-  int ensure_scene(const pt_scene &scene);
+  // Makes the scene resident; with settings, its tree is first (re)built
+  // with their accel fields.
+  int ensure_scene(const pt_scene &scene, const pt_settings *settings = nullptr);
   int ensure_sobol(uint32_t dim_bound, uint32_t count);
   int render(const pt_sample_spec &spec, const pt_output &out,
              const pt_scene &scene, uint32_t samples_per,

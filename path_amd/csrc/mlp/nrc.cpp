@@ -32,7 +32,7 @@ extern "C" int pt_render_mlp(pt_renderer *r, pt_mlp *m, const pt_scene *scene,
     return set_error(PT_ERR_INVALID, "too many samples");
   auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipSetDevice(r->device));
-  int rc = r->ensure_scene(*scene);
+  int rc = r->ensure_scene(*scene, settings);
   if (rc != PT_OK)
     return rc;
   gpu::NrcParams p{};

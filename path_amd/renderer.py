@@ -131,22 +131,27 @@ class Scene:
 
     def bvh(self) -> dict:
         n = C.c_uint32()
-        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), None, None, None, None, None,
-                                     None, None))
+        nr = C.c_uint32()
+        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), C.byref(nr), None, None, None,
+                                     None, None, None, None))
         nn = n.value
-        nt = self.counts()[0]
         lo = np.zeros((nn, 3), np.float32)
         hi = np.zeros((nn, 3), np.float32)
         leaf = np.zeros(nn, np.uint32)
         a = np.zeros(nn, np.uint32)
         b = np.zeros(nn, np.uint32)
-        perm = np.zeros(nt, np.uint32)
+        perm = np.zeros(nr.value, np.uint32)
         depth = C.c_uint32()
-        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), L.fptr(lo), L.fptr(hi),
-                                     L.uptr(leaf), L.uptr(a), L.uptr(b), L.uptr(perm),
-                                     C.byref(depth)))
+        L.check(L.lib().pt_scene_bvh(self.handle, C.byref(n), C.byref(nr), L.fptr(lo),
+                                     L.fptr(hi), L.uptr(leaf), L.uptr(a), L.uptr(b),
+                                     L.uptr(perm), C.byref(depth)))
         return {"lo": lo, "hi": hi, "is_leaf": leaf, "a": a, "b": b, "perm": perm,
                 "max_depth": depth.value}
+
+    def set_accel(self, settings: "Settings") -> None:
+        """Build the tree with the accel fields of settings (spatial splits,
+        overlap threshold, traversal cost); renders do the same."""
+        L.check(L.lib().pt_scene_set_accel(self.handle, C.byref(settings.raw)))
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -459,3 +464,17 @@ class RendererFromFiles:
             self.renderer = Renderer(self.device)
         return self.renderer.render(self.scene, self.film_to_world, x_dim, y_dim, samples_per,
                                     self.settings, output, show_progress, show_times)
+
+
+def clip_triangle_bounds(tri, lo, hi):
+    """Bounds (lo, hi) of the part of triangle tri (3 x 3) inside the box
+    [lo, hi]; lo > hi when they do not meet. The clipping a spatial split
+    applies to a triangle (include/intersect/accel/detail/clipped_triangle_impl.h)."""
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    blo = np.ascontiguousarray(lo, np.float32).reshape(3)
+    bhi = np.ascontiguousarray(hi, np.float32).reshape(3)
+    olo = np.zeros(3, np.float32)
+    ohi = np.zeros(3, np.float32)
+    L.check(L.lib().pt_clip_triangle_bounds(L.fptr(t), L.fptr(blo), L.fptr(bhi), L.fptr(olo),
+                                            L.fptr(ohi)))
+    return olo, ohi

@@ -35,3 +35,14 @@ def cornell_sphere(tmp_path_factory):
     d = tmp_path_factory.mktemp("cornell_sphere")
     scene, ftw = Scene.load(scenes.cornell_icosphere(d, level=4, seed=1), 1.0)
     return scene, ftw
+
+
+@pytest.fixture(scope="session")
+def benchmark_scene(tmp_path_factory):
+    """The throughput benchmark's scene: Cornell box + icosphere level 6
+    (81,956 triangles)."""
+    from path_amd import scenes
+    from path_amd.renderer import Scene
+    d = tmp_path_factory.mktemp("benchmark_scene")
+    scene, ftw = Scene.load(scenes.cornell_icosphere(d, level=6, seed=0), 1.0)
+    return scene, ftw

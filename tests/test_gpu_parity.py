@@ -139,6 +139,40 @@ def test_streaming_image_equals_megakernel(renderer, request, which, spp, size):
         set_pipelines(renderer, 3)
 
 
+@pytest.mark.parametrize("which", ["cornell_sphere", "benchmark_scene"])
+def test_spatial_split_tree_same_hits_and_images(renderer, request, which):
+    """With spatial splits a triangle can sit in several leaves, each under a
+    box of only its clipped part. Closest hits equal the loop-all oracle's
+    (index, t and side bit for bit) and images equal the CPU checker's (whose
+    tree has no spatial splits) for both integrators."""
+    from oracle import oracle as O
+    from path_amd.renderer import Settings
+    scene, ftw = request.getfixturevalue(which)
+    flat = scene.flat()
+    s = _settings()
+    s.use_spatial_splits = 1
+    try:
+        scene.set_accel(s)
+        assert len(scene.bvh()["perm"]) > flat["tris"].shape[0]  # triangles were split
+        rays = random_rays(20000, -1.2, 2.2, seed=11)
+        t, idx, back = renderer.intersect(scene, rays)
+        rt, ridx, rback = O.loopall(flat["tris"], rays)
+        assert np.array_equal(idx, ridx)
+        hit = ridx != 0xFFFFFFFF
+        assert hit.mean() > 0.3
+        assert np.array_equal(t[hit].view(np.uint32), rt[hit].view(np.uint32))
+        assert np.array_equal(back[hit], rback[hit])
+        spp, size = 4, 40
+        tab = O.sobol_table(O.joe_kuo_directions(256), 256, spp)
+        ref32, _ = O.render(flat, ftw, s, spp, size, size, tab, precision=32)
+        for approach in ("MegaKernel", "Streaming"):
+            s.kernel_approach = approach
+            img, _ = renderer.render(scene, ftw, size, size, spp, s)
+            assert np.array_equal(img.view(np.uint32), ref32.view(np.uint32)), approach
+    finally:
+        scene.set_accel(Settings())
+
+
 def test_streaming_default_settings_multiplier_func(renderer, cornell):
     """Reference default settings (MultiplierFunc Russian roulette): wavefront
     == mega-kernel == CPU checker."""
