@@ -261,6 +261,13 @@ int pt_sobol_direction_numbers(uint32_t dim_bound, uint32_t *out);
  * num_x_blocks, num_y_blocks, x_dim, y_dim. */
 int pt_work_division(const pt_work_division_settings *s, uint32_t samples_per,
                      uint32_t x_dim, uint32_t y_dim, uint32_t out[11]);
+/* Stable radix sort of n (key, value) pairs by key bits [0, end_bit) on the
+ * GPU (host arrays in and out). No reference counterpart (the reference
+ * builds its trees on the host): it is the sort of the device tree builder
+ * (pt_set_bvh_builder), which orders triangles by (scene, Morton code) keys,
+ * exposed so that it can be checked on its own. */
+int pt_sort_pairs(pt_renderer *r, const uint64_t *keys, const uint32_t *vals, uint32_t n,
+                  uint32_t end_bit, uint64_t *keys_out, uint32_t *vals_out);
 /* The renderer's per-pixel sample reduction on given values: each
  * work-division thread sums values[sample + location * samples_per] over its
  * samples in order, then a location's threads are combined by the same lane

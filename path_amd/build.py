@@ -54,6 +54,7 @@ SOURCES = [
     "mlp/mlp_kernels.hip",
     "gpu/nrc.hip",
     "gpu/lbvh.hip",
+    "gpu/radix_sort.hip",
     "gpu/noalloc.hip",
 ]
 

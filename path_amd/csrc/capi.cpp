@@ -4,6 +4,7 @@
 #include "core/camera.h"
 #include "core/rng.h"
 #include "gpu/kernels.h"
+#include "gpu/radix_sort.h"
 #include "host/bvh.h"
 #include "host/renderer.h"
 #include "host/scene_gen.h"
@@ -398,6 +399,45 @@ int pt_sobol_table(pt_renderer *r, uint32_t dim_bound, uint32_t count,
   if (!r || !out)
     return fail(PT_ERR_INVALID, "null argument");
   return r->sobol_table_to_host(dim_bound, count, out);
+  PT_GUARD_END
+}
+
+int pt_sort_pairs(pt_renderer *r, const uint64_t *keys, const uint32_t *vals, uint32_t n,
+                  uint32_t end_bit, uint64_t *keys_out, uint32_t *vals_out) {
+  PT_GUARD_BEGIN
+  if (!r || (n && (!keys || !vals || !keys_out || !vals_out)))
+    return fail(PT_ERR_INVALID, "null argument");
+  if (end_bit < 1 || end_bit > 64)
+    return fail(PT_ERR_INVALID, "end_bit must be in [1, 64]");
+  if (n == 0)
+    return PT_OK;
+  hipStream_t s = r->stream;
+  DeviceBuf<unsigned long long> k0, k1;
+  DeviceBuf<uint32_t> v0, v1;
+  DeviceBuf<char> temp;
+  const size_t tb = gpu::radix_sort_temp_bytes(n);
+  hipError_t e = hipSetDevice(r->device);
+  if (e == hipSuccess)
+    e = k0.upload(reinterpret_cast<const unsigned long long *>(keys), n, s);
+  if (e == hipSuccess)
+    e = v0.upload(vals, n, s);
+  if (e == hipSuccess)
+    e = k1.ensure(n);
+  if (e == hipSuccess)
+    e = v1.ensure(n);
+  if (e == hipSuccess)
+    e = temp.ensure(tb);
+  if (e == hipSuccess)
+    e = gpu::radix_sort_pairs(k0.ptr, v0.ptr, k1.ptr, v1.ptr, n, end_bit, temp.ptr, tb, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(keys_out, k1.ptr, size_t(n) * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(vals_out, v1.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return fail(PT_ERR_HIP, std::string("sort: ") + hipGetErrorString(e));
+  return PT_OK;
   PT_GUARD_END
 }
 

@@ -1,7 +1,6 @@
 // GPU LBVH build kernels (see lbvh.h for the algorithm and numbering).
 #include "lbvh.h"
-
-#include <hipcub/hipcub.hpp>
+#include "radix_sort.h"
 
 #include <vector>
 
@@ -120,11 +119,9 @@ static uint32_t key_end_bit(uint32_t n_scenes) {
 }
 
 hipError_t lbvh_sort_temp_bytes(uint32_t n, uint32_t n_scenes, size_t *bytes) {
-  *bytes = 0;
-  return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (unsigned long long *)nullptr,
-                                            (unsigned long long *)nullptr,
-                                            (uint32_t *)nullptr, (uint32_t *)nullptr, int(n),
-                                            0, int(key_end_bit(n_scenes)));
+  (void)n_scenes;
+  *bytes = radix_sort_temp_bytes(n);
+  return hipSuccess;
 }
 
 hipError_t build_lbvh(const LbvhBuffers &b, const LbvhRoot *roots, uint32_t n_roots,
@@ -140,10 +137,10 @@ hipError_t build_lbvh(const LbvhBuffers &b, const LbvhRoot *roots, uint32_t n_ro
   hipLaunchKernelGGL(k_lbvh_keys, dim3((a.n_tris + 255) / 256), dim3(256), 0, s, a);
   if ((e = hipGetLastError()) != hipSuccess)
     return e;
-  size_t temp = b.sort_temp_bytes;
-  if ((e = hipcub::DeviceRadixSort::SortPairs(b.sort_temp, temp, a.keys, a.keys_sorted, a.vals,
-                                              a.vals_sorted, int(a.n_tris), 0,
-                                              int(key_end_bit(a.n_scenes)), s)) != hipSuccess)
+  // (scene, Morton code) keys; equal keys keep triangle order
+  if ((e = radix_sort_pairs(a.keys, a.vals, a.keys_sorted, a.vals_sorted, a.n_tris,
+                            key_end_bit(a.n_scenes), b.sort_temp, b.sort_temp_bytes, s)) !=
+      hipSuccess)
     return e;
   if (a.n_internal)
     hipLaunchKernelGGL(k_lbvh_karras, dim3((a.n_internal + 255) / 256), dim3(256), 0, s, a);

@@ -391,6 +391,18 @@ class Renderer:
         L.check(L.lib().pt_sobol_table(self.handle, dim_bound, count, L.fptr(out)))
         return out
 
+    def sort_pairs(self, keys, vals, end_bit: int = 64):
+        """Stable GPU radix sort of (uint64 key, uint32 value) pairs by key
+        bits [0, end_bit): the device tree builder's sort."""
+        k = np.ascontiguousarray(keys, np.uint64)
+        v = np.ascontiguousarray(vals, np.uint32)
+        ko = np.zeros_like(k)
+        vo = np.zeros_like(v)
+        P64 = C.POINTER(C.c_uint64)
+        L.check(L.lib().pt_sort_pairs(self.handle, k.ctypes.data_as(P64), L.uptr(v), len(k),
+                                      end_bit, ko.ctypes.data_as(P64), L.uptr(vo)))
+        return ko, vo
+
     def last_stats(self) -> RenderStats:
         s = L.PtRenderStats()
         L.check(L.lib().pt_last_render_stats(self.handle, C.byref(s)))
