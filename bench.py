@@ -494,11 +494,26 @@ def bench_adjoint(a, cfg):
 
 
 def bench_mlp(a, cfg):
-    """Radiance MLP: datagen -> training -> held-out time-to-RMSE."""
+    """Radiance MLP: datagen -> training -> held-out time-to-RMSE of each
+    MLP-assisted estimator against the plain tracer.
+
+    Estimators (all on the same held-out scene and 4096-spp FP64-summed
+    ground truth):
+      plain            the tracer itself, spp samples per pixel
+      vertex2          per sample: camera ray, light sample of the first
+                       vertex, one bounce; the model (trained on reflected
+                       radiance) stands in for the light reflected at the
+                       second vertex (one query per sample)
+      first_vertex     the tracer terminated after the first vertex (emission
+                       plus light sample, spp samples) plus one query per
+                       pixel of a model of the indirect light there
+      model_only       emission seen plus one query per pixel of the
+                       reflected-radiance model: no sampling at all
+    each beside the same estimator with a zero network."""
     import numpy as np
     from path_amd.datagen import datagen_settings, generate_scene
     from path_amd.mlp import RadianceMLP, param_count
-    from path_amd.radiance_cache import rel_rmse, training_set
+    from path_amd.radiance_cache import rel_rmse, training_sets
     from path_amd.renderer import Renderer
     r = Renderer(0)
     st = datagen_settings()
@@ -506,24 +521,33 @@ def bench_mlp(a, cfg):
     st.n_iters = cfg["iters"]
     rows = cfg["rows"] if not a.max_paths else a.max_paths
     t0 = time.perf_counter()
-    feats, targs, seed = [], [], 0
+    feats, refl, ind, seed = [], [], [], 0
     while sum(x.shape[0] for x in feats) < rows:
-        f, t = training_set(r, range(seed, seed + 1024), samples_per=cfg["spp"], settings=st)
+        f, tr, ti = training_sets(r, range(seed, seed + 1024), samples_per=cfg["spp"],
+                                  settings=st)
         feats.append(f)
-        targs.append(t)
+        refl.append(tr)
+        ind.append(ti)
         seed += 1024
         print(f"datagen: {seed} scenes, {sum(x.shape[0] for x in feats)} rows, "
               f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     f = np.concatenate(feats)[:rows]
-    t = np.concatenate(targs)[:rows]
-    del feats, targs
+    tr = np.concatenate(refl)[:rows]
+    ti = np.concatenate(ind)[:rows]
+    del feats, refl, ind
     datagen_s = time.perf_counter() - t0
-    m = RadianceMLP(0, seed=1)
-    m.set_dataset(f, t)
     batch = 65536
     steps = 2 * rows // batch
-    losses, train_s = m.train(steps, batch, 1e-3, seed=2)
-    print(f"trained {steps} steps in {train_s:.1f} s", file=sys.stderr, flush=True)
+
+    def train(targets, seed_w):
+        m = RadianceMLP(0, seed=seed_w)
+        m.set_dataset(f, targets)
+        losses, secs = m.train(steps, batch, 1e-3, seed=seed_w + 1)
+        print(f"trained {steps} steps in {secs:.1f} s", file=sys.stderr, flush=True)
+        return m, losses, secs
+    m, losses, train_s = train(tr, 1)
+    mi, losses_i, train_i_s = train(ti, 3)
+    del f, tr, ti
     train_tflops = 3 * 2 * MLP_MACS * batch * steps / train_s / 1e12
 
     # held-out scene at 512x512
@@ -533,6 +557,7 @@ def bench_mlp(a, cfg):
     stm.kernel_approach = "Streaming"
     r.render(g.scene, g.film_to_world, size, size, cfg["gt_spp"], stm)
     gt = r.last_render_fp64(size, size)  # per-sample radiance summed in FP64
+    indirect_share = None
 
     def curve(render, spps):
         out = []
@@ -547,19 +572,34 @@ def bench_mlp(a, cfg):
                         "gpu_ms": round(r.last_stats().kernel_ms, 3),
                         "rel_rmse": round(rel_rmse(img, gt), 5)})
         return out
-    plain = curve(lambda spp: r.render(g.scene, g.film_to_world, size, size, spp, stm)[0],
-                  [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024])
-    mlp = curve(lambda spp: m.render(r, g.scene, g.film_to_world, size, size, spp, stm)[0],
-                [1, 2, 4, 8, 16, 32, 64])
     zero = RadianceMLP(0, seed=1)
     zero.set_weights(np.zeros(param_count(), np.float32))
-    no_model = curve(lambda spp: zero.render(r, g.scene, g.film_to_world, size, size, spp,
-                                             stm)[0], [1, 4, 16, 64])
+    fv = lambda net: (lambda spp: net.render_first_vertex(r, g.scene, g.film_to_world, size,
+                                                          size, spp, stm)[0])
+    curves = {
+        "plain": curve(lambda spp: r.render(g.scene, g.film_to_world, size, size, spp, stm)[0],
+                       [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]),
+        "vertex2": curve(lambda spp: m.render(r, g.scene, g.film_to_world, size, size, spp,
+                                              stm)[0], [1, 2, 4, 8, 16, 32, 64]),
+        "vertex2_zero_model": curve(lambda spp: zero.render(r, g.scene, g.film_to_world, size,
+                                                            size, spp, stm)[0], [1, 4, 16, 64]),
+        "first_vertex": curve(fv(mi), [1, 2, 4, 8, 16, 32, 64, 128]),
+        "first_vertex_zero_model": curve(fv(zero), [4, 16, 64, 128]),
+        "model_only": curve(fv(m), [0]),
+    }
+    # share of indirect light in the ground truth: the first-vertex render
+    # at high spp against the full image
+    direct = fv(zero)(1024)
+    indirect_share = round(float(1.0 - direct.astype(np.float64).mean() / gt.mean()), 4)
 
     def time_to(cv, target):
         ok = [c["seconds"] for c in cv if c["rel_rmse"] <= target]
         return min(ok) if ok else None
-    tp, tm = time_to(plain, 0.01), time_to(mlp, 0.01)
+    tp = time_to(curves["plain"], 0.01)
+    per = {k: time_to(v, 0.01) for k, v in curves.items()
+           if k in ("vertex2", "first_vertex", "model_only")}
+    reached = [t for t in per.values() if t]
+    tm = min(reached) if reached else None
     # inference throughput of the network alone (forward of 2^20 queries)
     q = np.random.default_rng(0).uniform(-1, 1, (1 << 20, 9)).astype(np.float32)
     m.forward(q)
@@ -571,7 +611,7 @@ def bench_mlp(a, cfg):
     out = {
         "metric": "MLP-assisted vs plain time-to-1%-RMSE (held-out scene, 1 GPU)",
         "value": round(tp / tm, 3) if (tp and tm) else None,
-        "unit": "x (plain time / MLP-assisted time)",
+        "unit": "x (plain time / fastest MLP-assisted time)",
         "n_gpus": 1, "steps": steps, "warmup": 0,
         "ms_per_step": round(train_s / steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -581,11 +621,15 @@ def bench_mlp(a, cfg):
                    "epochs": 2, "heldout_seed": 10 ** 9 + 7, "width": size, "height": size,
                    "ground_truth_spp": cfg["gt_spp"], "max_depth": cfg["iters"] + 1},
         "time_to_1pct_plain_s": tp,
-        "time_to_1pct_mlp_s": tm,
-        "best_rel_rmse_mlp": min(c["rel_rmse"] for c in mlp),
-        "curve_plain": plain, "curve_mlp": mlp, "curve_no_model": no_model,
+        "time_to_1pct_s": per,
+        "best_rel_rmse": {k: min(c["rel_rmse"] for c in v) for k, v in curves.items()},
+        "indirect_share_of_ground_truth": indirect_share,
+        "curves": curves,
         "datagen_s": round(datagen_s, 2), "train_s": round(train_s, 2),
+        "train_indirect_s": round(train_i_s, 2),
         "train_loss_first_last": [float(losses[:20].mean()), float(losses[-20:].mean())],
+        "train_indirect_loss_first_last": [float(losses_i[:20].mean()),
+                                           float(losses_i[-20:].mean())],
         "train_tflops": round(train_tflops, 2),
         "inference_mqueries_per_s": round((1 << 20) / fwd_s / 1e6, 2),
     }

@@ -485,6 +485,19 @@ int pt_render_mlp(pt_renderer *r, pt_mlp *m, const pt_scene *scene,
                   uint32_t samples_per, const pt_settings *settings,
                   float *out_rgb, double *seconds);
 
+/* Radiance-cache render that queries the MLP once per pixel, at the first
+ * camera hit through the pixel centre. samples_per > 0: the image is the
+ * tracer's render terminated after the first vertex (emission seen plus its
+ * light sample, samples_per samples, the same samples as pt_render) plus the
+ * MLP's prediction there, for a network trained on the indirect part of the
+ * reflected radiance. samples_per = 0: emission seen plus the prediction,
+ * for a network trained on all reflected radiance (no tracing beyond the
+ * camera ray). Host out_rgb of x_dim*y_dim*3 floats. */
+int pt_render_mlp_first_vertex(pt_renderer *r, pt_mlp *m, const pt_scene *scene,
+                               const float film_to_world[12], uint32_t x_dim, uint32_t y_dim,
+                               uint32_t samples_per, const pt_settings *settings,
+                               float *out_rgb, double *seconds);
+
 /* Per-pixel means of the last Streaming SquareImage render re-summed in FP64
  * (sequentially over each pixel's per-sample radiance, in sample order):
  * host out_rgb of x_dim*y_dim*3 doubles. For ground-truth images. */

@@ -187,6 +187,9 @@ SIGNATURES = {
     "pt_last_render_fp64": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "pt_render_mlp": (C.c_int, [_P, _P, _P, _F, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(PtSettings), _F, C.POINTER(C.c_double)]),
+    "pt_render_mlp_first_vertex": (C.c_int, [_P, _P, _P, _F, C.c_uint32, C.c_uint32,
+                                             C.c_uint32, C.POINTER(PtSettings), _F,
+                                             C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -143,6 +143,9 @@ hipError_t launch_to_bgra(const float *rgb, uint32_t n, uint32_t *bgra,
                           hipStream_t s);
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s);
 hipError_t launch_nrc_paths(const NrcParams &p, hipStream_t s);
+// Per pixel (pixel-centre camera ray): features of the first hit, the
+// emission seen there in base and weight 1 (all zero for a miss).
+hipError_t launch_nrc_first_vertex(const NrcParams &p, hipStream_t s);
 // out_rgb[pixel] = sum_s (base + weight * max(pred[0..3], 0)) / spp with pred
 // rows of 16 floats.
 hipError_t launch_nrc_combine(const float *base, const float *weight, const float *pred,

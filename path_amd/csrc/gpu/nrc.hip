@@ -92,6 +92,59 @@ hipError_t launch_nrc_paths(const NrcParams &p, hipStream_t s) {
   return hipGetLastError();
 }
 
+// First camera hit of each pixel through the pixel centre: the MLP's
+// features there, the emission seen (base) and weight 1 when something is hit.
+__global__ __launch_bounds__(256) void k_nrc_first_vertex(NrcParams p, uint32_t n_pixels) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int stack_lds[kLdsStackInts * 4];
+  if (i >= n_pixels)
+    return;
+  const uint32_t px = i % p.x_dim, py = i / p.x_dim;
+  const Ray<float> cam = camera_ray(p.film_to_world, float(px) + 0.5f, float(py) + 0.5f,
+                                    p.x_dim, p.y_dim);
+  int spill[kOverflowDepth];
+  LdsStack stack{lds_stack_column(stack_lds), spill, 1};
+  const Hit<float> h = trace_closest(p.scene.nodes, p.scene.tris, cam, 3.4028235e38f, -1.0f,
+                                     stack);
+  V3<float> base = {0.f, 0.f, 0.f};
+  float w = 0.f;
+  float feat[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (h.idx != kNoHit) {
+    const SceneView<float> &sc = p.scene.view;
+    const ShadeTri<float> tri = sc.tris[h.idx];
+    const Material<float> mat = sc.materials[tri.material];
+    if (!(p.settings.back_cull_emission && h.back))
+      base = mat.emission;
+    const V3<float> q = hit_point(cam, h.t);
+    const TriGeom<float> g = {tri.v0, tri.e1, tri.e2};
+    const V3<float> vn[3] = {tri.n0, tri.n1, tri.n2};
+    const V3<float> nrm = shading_normal(g, vn, q);
+    const V3<float> alb = material_albedo(mat);
+    w = 1.f;
+    feat[0] = q.x, feat[1] = q.y, feat[2] = q.z;
+    feat[3] = nrm.x, feat[4] = nrm.y, feat[5] = nrm.z;
+    feat[6] = alb.x, feat[7] = alb.y, feat[8] = alb.z;
+  }
+  p.base[3 * size_t(i)] = base.x;
+  p.base[3 * size_t(i) + 1] = base.y;
+  p.base[3 * size_t(i) + 2] = base.z;
+  p.weight[3 * size_t(i)] = w;
+  p.weight[3 * size_t(i) + 1] = w;
+  p.weight[3 * size_t(i) + 2] = w;
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    p.features[9 * size_t(i) + k] = feat[k];
+}
+
+hipError_t launch_nrc_first_vertex(const NrcParams &p, hipStream_t s) {
+  const uint64_t n = uint64_t(p.x_dim) * p.y_dim;
+  if (n >= (1ull << 32))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_nrc_first_vertex, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, p,
+                     uint32_t(n));
+  return hipGetLastError();
+}
+
 __global__ void k_nrc_combine(const float *__restrict__ base, const float *__restrict__ weight,
                               const float *__restrict__ pred, uint32_t n_pixels, uint32_t spp,
                               float *__restrict__ out) {

@@ -243,7 +243,7 @@ struct pt_renderer {
   pt::host::DeviceBuf<double> sum64;
 
   // radiance-cache render buffers
-  pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features;
+  pt::host::DeviceBuf<float> nrc_base, nrc_weight, nrc_features, nrc_direct;
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
   // persistent traversal waves; lanes refill from the wave's queue range.

@@ -108,6 +108,22 @@ class RadianceMLP:
                                       L.fptr(img), C.byref(secs)))
         return img, secs.value
 
+    def render_first_vertex(self, renderer, scene, film_to_world, x_dim: int, y_dim: int,
+                            samples_per: int, settings=None):
+        """Radiance-cache render with one query per pixel at the first camera
+        hit: samples_per > 0 adds the tracer's first-vertex render (emission
+        plus light sample, a model of indirect light); 0 uses the model alone
+        (a model of all reflected light). Returns (image (y, x, 3), seconds)."""
+        from .renderer import Settings
+        settings = settings or Settings()
+        img = np.zeros((y_dim, x_dim, 3), np.float32)
+        ftw = np.ascontiguousarray(np.asarray(film_to_world).ravel(), np.float32)
+        secs = C.c_double()
+        L.check(L.lib().pt_render_mlp_first_vertex(
+            renderer.handle, self.handle, scene.handle, L.fptr(ftw), x_dim, y_dim, samples_per,
+            C.byref(settings.raw), L.fptr(img), C.byref(secs)))
+        return img, secs.value
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h and L._lib is not None:

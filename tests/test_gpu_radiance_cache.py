@@ -56,3 +56,90 @@ def test_trained_model_beats_no_model(renderer):
     e_m, e_0 = rel_rmse(img, gt), rel_rmse(base, gt)
     print("held-out relative RMSE: with model", e_m, "without", e_0)
     assert e_m < e_0
+
+
+def test_first_vertex_estimator_without_model_is_first_vertex_render(renderer, cornell):
+    """One query per pixel: with a zero network the image is the tracer's
+    own render terminated after the first vertex (samples_per > 0), or the
+    emission seen through each pixel centre (samples_per = 0)."""
+    from oracle import oracle as O
+    from path_amd.datagen import datagen_settings, generate_scene
+    from path_amd.mlp import RadianceMLP, param_count
+    from path_amd.radiance_cache import first_vertex_settings
+    g = generate_scene(321)
+    st = datagen_settings()
+    st.term_prob = "NIters"
+    st.n_iters = 5
+    zero = RadianceMLP(0, seed=1)
+    zero.set_weights(np.zeros(param_count(), np.float32))
+    for approach in ("MegaKernel", "Streaming"):
+        st.kernel_approach = approach
+        img, _ = zero.render_first_vertex(renderer, g.scene, g.film_to_world, 64, 48, 8, st)
+        ref, _ = renderer.render(g.scene, g.film_to_world, 64, 48, 8, first_vertex_settings(st))
+        assert np.array_equal(img, ref), approach
+    # model only: emission of the first hit through each pixel centre (the
+    # Cornell box's light faces the camera)
+    scene, ftw = cornell
+    x, y = 192, 160
+    img, _ = zero.render_first_vertex(renderer, scene, ftw, x, y, 0, st)
+    m = np.asarray(ftw, np.float64).reshape(3, 4)
+    px, py = np.meshgrid(np.arange(x) + 0.5, np.arange(y) + 0.5)
+    cam = np.stack([2 * px / x - 1, -2 * py / y + 1, -np.ones_like(px)], -1).reshape(-1, 3)
+    w = cam @ m[:, :3].T
+    d = w / np.linalg.norm(w, axis=1, keepdims=True)
+    o = np.broadcast_to(m[:, 3], d.shape)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    flat = scene.flat()
+    _, idx, back = O.loopall(flat["tris"], rays)
+    nm = scene.counts()[1]
+    emis = np.array([list(mt.emission) for mt in list(flat["materials"])[:nm]], np.float32)
+    hit = idx != 0xFFFFFFFF
+    exp = np.zeros((x * y, 3), np.float32)
+    exp[hit] = emis[flat["tri_mat"][idx[hit]]]
+    if st.back_cull_emission:
+        exp[hit & (back != 0)] = 0
+    got = img.reshape(-1, 3)
+    assert (exp.max(1) > 0).sum() > 20  # the light is in view
+    assert (np.abs(got - exp).max(1) > 0).mean() < 0.01  # pixel-centre rays: edges may differ
+
+
+def test_trained_indirect_model_in_first_vertex_estimator(renderer):
+    """The first-vertex estimator with a network trained on indirect light
+    (full render minus the first-vertex render of the same samples) beside
+    the same estimator without it. The network fits its targets (training
+    loss falls three-fold); in these scenes indirect light is about 1% of the
+    image, below the network's error, so the estimator's error with the
+    model stays within 10% of its error without it, on a training scene and
+    on a held-out one (the numbers are printed)."""
+    from path_amd.datagen import datagen_settings, generate_scene
+    from path_amd.mlp import RadianceMLP, param_count
+    from path_amd.radiance_cache import rel_rmse, training_sets
+    st = datagen_settings()
+    st.term_prob = "NIters"
+    st.n_iters = 5
+    f, refl, ind = training_sets(renderer, range(64), samples_per=64, settings=st)
+    # indirect light is part of the reflected light (rows where a pixel's
+    # samples see different surfaces, e.g. an emitter's edge, may differ:
+    # reflected subtracts the emission of sample 0's hit only)
+    assert (ind <= refl + 1e-3).all(1).mean() > 0.98
+    assert ind.mean() < refl.mean()
+    m = RadianceMLP(0, seed=4)
+    m.set_dataset(f, ind)
+    losses, _ = m.train(600, 16384, 1e-3, seed=5)
+    assert losses[-20:].mean() < losses[:10].mean() / 3
+    zero = RadianceMLP(0, seed=4)
+    zero.set_weights(np.zeros(param_count(), np.float32))
+    stm = st.copy()
+    stm.kernel_approach = "Streaming"
+    errs = {}
+    for name, seed in (("training", 5), ("held-out", 10 ** 6 + 7)):
+        g = generate_scene(seed)
+        gt, _ = renderer.render(g.scene, g.film_to_world, 128, 128, 1024, stm)
+        img, _ = m.render_first_vertex(renderer, g.scene, g.film_to_world, 128, 128, 64, stm)
+        base, _ = zero.render_first_vertex(renderer, g.scene, g.film_to_world, 128, 128, 64,
+                                           stm)
+        errs[name] = (rel_rmse(img, gt), rel_rmse(base, gt))
+        print(f"{name} scene relative RMSE, first-vertex estimator: with model "
+              f"{errs[name][0]:.5f}, without {errs[name][1]:.5f}")
+    for name, (e_m, e_0) in errs.items():
+        assert e_m < 1.1 * e_0, name
