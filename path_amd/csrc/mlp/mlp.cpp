@@ -71,6 +71,10 @@ bool pt_mlp::init(int dev, uint64_t seed, std::string *err) {
   if (e == hipSuccess)
     e = launch_to_bf16(w32.ptr, wbf.ptr, P, stream);
   if (e == hipSuccess)
+    e = wbf_t.ensure(P);
+  if (e == hipSuccess)
+    e = launch_transpose_weights(wbf.ptr, wbf_t.ptr, stream);
+  if (e == hipSuccess)
     e = adam_m.ensure(P);
   if (e == hipSuccess)
     e = adam_v.ensure(P);
@@ -103,6 +107,12 @@ int pt_mlp::ensure_rows(uint32_t rows_pad, bool training) {
     HIP_TRY(g0.ensure(size_t(rows_pad) * kWidth));
     HIP_TRY(g1.ensure(size_t(rows_pad) * kWidth));
     HIP_TRY(loss_part.ensure((rows_pad + 255) / 256));
+    HIP_TRY(x0_t.ensure(size_t(rows_pad) * kInPad));
+    for (uint32_t l = 0; l < kHidden; ++l)
+      HIP_TRY(act_t[l].ensure(size_t(rows_pad) * kWidth));
+    HIP_TRY(d_pred_t.ensure(size_t(rows_pad) * kOutPad));
+    HIP_TRY(g0_t.ensure(size_t(rows_pad) * kWidth));
+    HIP_TRY(g1_t.ensure(size_t(rows_pad) * kWidth));
   }
   return PT_OK;
 }
@@ -131,6 +141,8 @@ int pt_mlp::forward_device(const float *d_feat, uint32_t rows_pad, bool training
     return PT_OK;
   }
   HIP_TRY(launch_encode(d_feat, rows_pad, x0.ptr, stream));
+  if (training)
+    HIP_TRY(launch_transpose(x0.ptr, rows_pad, kInPad, kInPad, x0_t.ptr, rows_pad, stream));
   const bf16 *in = x0.ptr;
   for (uint32_t l = 0; l < kLayers; ++l) {
     const bf16 *w = wbf.ptr + layer_offset(l);
@@ -138,6 +150,10 @@ int pt_mlp::forward_device(const float *d_feat, uint32_t rows_pad, bool training
       bf16 *out = training ? act[l].ptr : act[l & 1].ptr;
       GemmArgs g = gemm_args(in, layer_in(l), w, layer_in(l), rows_pad, kWidth, layer_in(l),
                              out, kWidth);
+      if (training) {
+        g.CT = act_t[l].ptr;
+        g.ldct = rows_pad;
+      }
       HIP_TRY(launch_gemm(g, false, false, kEpiReluBf16, 1, stream));
       in = out;
     } else {
@@ -149,35 +165,48 @@ int pt_mlp::forward_device(const float *d_feat, uint32_t rows_pad, bool training
   return PT_OK;
 }
 
-int pt_mlp::backward(uint32_t n, uint32_t rows_pad, float *loss) {
+int pt_mlp::backward(uint32_t n, uint32_t rows_pad, float *loss, float *loss_dev) {
   uint32_t n_part = 0;
-  HIP_TRY(hipMemsetAsync(d_pred.ptr, 0, size_t(rows_pad) * kOutPad * sizeof(bf16), stream));
-  HIP_TRY(launch_loss(pred.ptr, targ.ptr, n, d_pred.ptr, loss_part.ptr, &n_part, stream));
-  // split-K over the batch for the weight gradients: about 256 workgroups
-  const uint32_t splits = std::max(1u, std::min(64u, rows_pad / 1024u));
+  HIP_TRY(launch_loss(pred.ptr, targ.ptr, n, rows_pad, d_pred.ptr, d_pred_t.ptr, rows_pad,
+                      loss_part.ptr, &n_part, stream));
+  if (loss_dev && n_part)
+    HIP_TRY(launch_loss_sum(loss_part.ptr, n_part, loss_dev, stream));
+  // split-K over the batch for the weight gradients (512 rows per split, at
+  // most 128 splits: 512 workgroups for a 256 x 256 gradient)
+  const uint32_t splits = std::max(1u, std::min(128u, rows_pad / 512u));
   const uint32_t kps = (rows_pad + splits - 1) / splits;
   const uint32_t kps32 = (kps + 63) / 64 * 64;
   HIP_TRY(split_part.ensure(size_t(splits) * kWidth * kWidth));
-  const bf16 *G = d_pred.ptr;
+  // G: gradient w.r.t. the layer's output [rows x out]; G_t the same
+  // transposed [out x rows]
+  const bf16 *G = d_pred.ptr, *G_t = d_pred_t.ptr;
   for (int l = int(kLayers) - 1; l >= 0; --l) {
     const uint32_t in = layer_in(l), out = layer_out(l);
-    const bf16 *a_in = l == 0 ? x0.ptr : act[l - 1].ptr;
-    // dW_l[out x in] = G^T[out x n] . A_in[n x in]
-    GemmArgs gw = gemm_args(G, out, a_in, in, out, in, rows_pad, split_part.ptr, in);
+    const bf16 *a_in_t = l == 0 ? x0_t.ptr : act_t[l - 1].ptr;
+    // dW_l[out x in] = sum over rows of G_t[out][row] a_in_t[in][row]
+    GemmArgs gw = gemm_args(G_t, rows_pad, a_in_t, rows_pad, out, in, rows_pad, split_part.ptr,
+                            in);
     gw.k_per_split = kps32;
     const uint32_t used = (rows_pad + kps32 - 1) / kps32;
-    HIP_TRY(launch_gemm(gw, true, true, kEpiSplitF32, used, stream));
+    HIP_TRY(launch_gemm(gw, false, false, kEpiSplitF32, used, stream));
     HIP_TRY(launch_sum_splits(split_part.ptr, used, uint64_t(out) * in,
                               grad.ptr + layer_offset(l), stream));
     if (l == 0)
       break;
-    // dA[n x in] = (G[n x out] . W_l[out x in]) masked by the layer input > 0
-    bf16 *gn = (G == g0.ptr) ? g1.ptr : g0.ptr;
-    GemmArgs gd = gemm_args(G, out, wbf.ptr + layer_offset(l), in, rows_pad, in, out, gn, in);
-    gd.mask = act[l - 1].ptr;
-    gd.ldm = kWidth;
-    HIP_TRY(launch_gemm(gd, false, true, kEpiMaskBf16, 1, stream));
+    // dA[rows x in] = (G[rows x out] . W_l[out x in]) masked by the layer
+    // input > 0 (read from its transposed copy, 4 rows per load); W_l^T
+    // ([in][out]) is the B operand; also stored transposed
+    const bool to0 = G != g0.ptr;
+    bf16 *gn = to0 ? g0.ptr : g1.ptr;
+    bf16 *gn_t = to0 ? g0_t.ptr : g1_t.ptr;
+    GemmArgs gd = gemm_args(G, out, wbf_t.ptr + layer_offset(l), out, rows_pad, in, out, gn, in);
+    gd.mask_t = act_t[l - 1].ptr;
+    gd.ldmt = rows_pad;
+    gd.CT = gn_t;
+    gd.ldct = rows_pad;
+    HIP_TRY(launch_gemm(gd, false, false, kEpiMaskBf16, 1, stream));
     G = gn;
+    G_t = gn_t;
   }
   if (loss) {
     std::vector<float> parts(n_part);
@@ -196,6 +225,7 @@ int pt_mlp::adam(float lr) {
   ++step;
   HIP_TRY(launch_adam(w32.ptr, grad.ptr, adam_m.ptr, adam_v.ptr, wbf.ptr, param_count(), lr,
                       0.9f, 0.99f, 1e-8f, step, stream));
+  HIP_TRY(launch_transpose_weights(wbf.ptr, wbf_t.ptr, stream));
   return PT_OK;
 }
 
@@ -248,6 +278,7 @@ int pt_mlp_set_weights(pt_mlp *m, const float *w) {
   HIP_TRY(hipSetDevice(m->device));
   HIP_TRY(m->w32.upload(w, param_count(), m->stream));
   HIP_TRY(launch_to_bf16(m->w32.ptr, m->wbf.ptr, param_count(), m->stream));
+  HIP_TRY(launch_transpose_weights(m->wbf.ptr, m->wbf_t.ptr, m->stream));
   HIP_TRY(hipMemsetAsync(m->adam_m.ptr, 0, param_count() * 4, m->stream));
   HIP_TRY(hipMemsetAsync(m->adam_v.ptr, 0, param_count() * 4, m->stream));
   m->step = 0;
@@ -407,6 +438,8 @@ int pt_mlp_train(pt_mlp *m, uint32_t steps, uint32_t batch, float lr, uint64_t s
   if (rc != PT_OK)
     return rc;
   HIP_TRY(m->idx.ensure(rp));
+  if (losses)
+    HIP_TRY(m->step_losses.ensure(std::max<uint32_t>(steps, 1)));
   HIP_TRY(hipMemsetAsync(m->feat.ptr, 0, size_t(rp) * kFeatures * 4, m->stream));
   auto t0 = std::chrono::steady_clock::now();
   for (uint32_t it = 0; it < steps; ++it) {
@@ -416,12 +449,17 @@ int pt_mlp_train(pt_mlp *m, uint32_t steps, uint32_t batch, float lr, uint64_t s
                           m->targ.ptr, m->stream));
     rc = m->forward_device(m->feat.ptr, rp, true);
     if (rc == PT_OK)
-      rc = m->backward(batch, rp, losses ? losses + it : nullptr);
+      rc = m->backward(batch, rp, nullptr, losses ? m->step_losses.ptr + it : nullptr);
     if (rc == PT_OK)
       rc = m->adam(lr);
     if (rc != PT_OK)
       return rc;
   }
+  // the per-step losses stay on the device until the end: no step waits
+  // for the host
+  if (losses && steps)
+    HIP_TRY(hipMemcpyAsync(losses, m->step_losses.ptr, size_t(steps) * sizeof(float),
+                           hipMemcpyDeviceToHost, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

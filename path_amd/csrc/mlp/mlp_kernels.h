@@ -40,7 +40,7 @@ inline uint64_t param_count() { return layer_offset(kLayers); }
 enum Epilogue : int {
   kEpiReluBf16 = 0, // C = bf16(max(acc, 0))
   kEpiF32 = 1,      // C = acc (fp32)
-  kEpiMaskBf16 = 2, // C = bf16(acc * (mask > 0))
+  kEpiMaskBf16 = 2, // C = bf16(acc * (layer input > 0)), the input read transposed
   kEpiSplitF32 = 3, // C[z] = acc (fp32 partial of K split z)
 };
 
@@ -48,6 +48,11 @@ enum Epilogue : int {
 // A(m, k) = a_trans ? A[k * lda + m] : A[m * lda + k]
 // B(k, n) = b_trans ? B[k * ldb + n] : B[n * ldb + k]
 // Requirements: M, N, K, lda, ldb multiples of 8; k_per_split a multiple of 32.
+// With the bf16 epilogues, CT (when set) also receives the result
+// transposed: CT[n * ldct + m] (M a multiple of 8, ldct of 4), the layout in
+// which the weight-gradient GEMM reads it with the batch as its K.
+// kEpiMaskBf16 reads its mask transposed: element (m, n) at
+// mask_t[n * ldmt + m] (ldmt a multiple of 4), 4 rows per load.
 struct GemmArgs {
   const bf16 *A;
   const bf16 *B;
@@ -55,9 +60,11 @@ struct GemmArgs {
   uint32_t M, N, K;
   void *C;
   uint32_t ldc;
-  const bf16 *mask;
-  uint32_t ldm;
   uint32_t k_per_split;
+  void *CT;
+  uint32_t ldct;
+  const bf16 *mask_t;
+  uint32_t ldmt;
 };
 
 hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilogue,
@@ -72,8 +79,19 @@ hipError_t launch_mlp_forward(const float *features, uint32_t rows_pad, const bf
 // Relative L2 loss on the first 3 outputs: l = (p - t)^2 / (p^2 + 0.01) with p
 // held constant in the denominator, mean over n * 3. Writes d(loss)/d(pred)
 // as bf16 [n x 16] (padding zero) and per-block loss partials (fp32).
-hipError_t launch_loss(const float *pred, const float *target, uint32_t n, bf16 *d_pred,
-                       float *loss_partials, uint32_t *n_partials, hipStream_t s);
+// Rows n .. rows_pad - 1 of d_pred are zeroed; d_pred_t (optional) receives
+// d_pred transposed: [16 x ld_t].
+hipError_t launch_loss(const float *pred, const float *target, uint32_t n, uint32_t rows_pad,
+                       bf16 *d_pred, bf16 *d_pred_t, uint32_t ld_t, float *loss_partials,
+                       uint32_t *n_partials, hipStream_t s);
+// *out = sum of the n loss partials in order (FP64), on the device.
+hipError_t launch_loss_sum(const float *partials, uint32_t n, float *out, hipStream_t s);
+// dst[c * ld_dst + r] = src[r * ld_src + c] for r < rows, c < cols (bf16).
+hipError_t launch_transpose(const bf16 *src, uint32_t rows, uint32_t cols, uint32_t ld_src,
+                            bf16 *dst, uint32_t ld_dst, hipStream_t s);
+// Per layer, the bf16 weights transposed ([in][out]): the B operand of the
+// data-gradient GEMM. One launch for all layers.
+hipError_t launch_transpose_weights(const bf16 *w, bf16 *w_t, hipStream_t s);
 // out[i] = sum over z < splits of partials[z * count + i] (fixed order)
 hipError_t launch_sum_splits(const float *partials, uint32_t splits, uint64_t count,
                              float *out, hipStream_t s);

@@ -72,12 +72,37 @@ __device__ __forceinline__ void store_tile(const uint4 (&v)[kChunks], uint16_t *
   }
 }
 
+// XCD-aware tile order. Workgroups go to the 8 XCDs round-robin by linear
+// id, and each XCD has its own L2. The tiles that share an operand tile (the
+// N tiles of one row block, or all tiles of one K split) form a group; the
+// members of a group get consecutive slots on one XCD, so the shared operand
+// is fetched into one L2 once. Identity when the groups do not fill the XCDs
+// evenly.
+struct TileId {
+  uint32_t x, y, z;
+};
+
+__device__ __forceinline__ TileId gemm_tile() {
+  const uint32_t gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const uint32_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const uint32_t group = gz > 1 ? gx * gy : gy;
+  const uint32_t n_groups = gx * gy * gz / group;
+  if (n_groups % 8u != 0)
+    return {blockIdx.x, blockIdx.y, blockIdx.z};
+  const uint32_t xcd = b % 8u, slot = b / 8u;
+  const uint32_t gi = (slot / group) * 8u + xcd, member = slot % group;
+  if (gz > 1)
+    return {member % gx, member / gx, gi};
+  return {gi, member, 0u};
+}
+
 template <bool kAT, bool kBT, int kEpi>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) uint16_t As[kBM * kLdsStride];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN * kLdsStride];
-  const uint32_t m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
-  const uint32_t k_begin = blockIdx.z * g.k_per_split;
+  const TileId tile = gemm_tile();
+  const uint32_t m0 = tile.x * kBM, n0 = tile.y * kBN;
+  const uint32_t k_begin = tile.z * g.k_per_split;
   const uint32_t k_end = min(g.K, k_begin + g.k_per_split);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t wm = wave >> 1, wn = wave & 1u;
@@ -124,29 +149,78 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
     __syncthreads();
   }
+  if constexpr (kEpi == kEpiReluBf16 || kEpi == kEpiMaskBf16) {
+    // bf16 results (ReLU, or the mask of the layer input), then the row-major
+    // store and, when asked, the transposed one (4 consecutive rows per store)
+    uint16_t e[2][2][16];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const uint32_t row = m0 + wm * 64u + i * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
-        const uint32_t col = n0 + wn * 64u + j * 32u + (lane & 31u);
-        if (row >= g.M || col >= g.N)
-          continue;
-        const float v = acc[i][j][r];
-        const size_t o = size_t(row) * g.ldc + col;
-        if constexpr (kEpi == kEpiReluBf16) {
-          reinterpret_cast<uint16_t *>(g.C)[o] = f2bf(v > 0.f ? v : 0.f);
-        } else if constexpr (kEpi == kEpiF32) {
-          reinterpret_cast<float *>(g.C)[o] = v;
-        } else if constexpr (kEpi == kEpiMaskBf16) {
-          const uint16_t mk = reinterpret_cast<const uint16_t *>(g.mask)[size_t(row) * g.ldm + col];
-          reinterpret_cast<uint16_t *>(g.C)[o] = f2bf(bf2f(mk) > 0.f ? v : 0.f);
-        } else {
-          reinterpret_cast<float *>(g.C)[size_t(blockIdx.z) * g.M * g.N + o] = v;
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t row0 = m0 + wm * 64u + i * 32u + q * 8u + (lane >> 5) * 4u;
+          const uint32_t col = n0 + wn * 64u + j * 32u + (lane & 31u);
+          const bool in = row0 < g.M && col < g.N;
+          uint2 mk = make_uint2(0u, 0u);
+          if constexpr (kEpi == kEpiMaskBf16) {
+            if (in)
+              mk = *reinterpret_cast<const uint2 *>(
+                  reinterpret_cast<const uint16_t *>(g.mask_t) + size_t(col) * g.ldmt + row0);
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float v = acc[i][j][4 * q + t];
+            float keep;
+            if constexpr (kEpi == kEpiMaskBf16) {
+              const uint32_t word = t < 2 ? mk.x : mk.y;
+              const uint16_t m = uint16_t(word >> (16 * (t & 1)));
+              keep = bf2f(m) > 0.f ? v : 0.f;
+            } else {
+              keep = v > 0.f ? v : 0.f;
+            }
+            e[i][j][4 * q + t] = f2bf(keep);
+            if (in)
+              reinterpret_cast<uint16_t *>(g.C)[size_t(row0 + t) * g.ldc + col] =
+                  e[i][j][4 * q + t];
+          }
         }
-      }
+    if (g.CT) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t row0 = m0 + wm * 64u + i * 32u + q * 8u + (lane >> 5) * 4u;
+            const uint32_t col = n0 + wn * 64u + j * 32u + (lane & 31u);
+            if (row0 >= g.M || col >= g.N)
+              continue;
+            const uint2 v = make_uint2(uint32_t(e[i][j][4 * q]) | (uint32_t(e[i][j][4 * q + 1]) << 16),
+                                       uint32_t(e[i][j][4 * q + 2]) | (uint32_t(e[i][j][4 * q + 3]) << 16));
+            *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(g.CT) + size_t(col) * g.ldct +
+                                       row0) = v;
+          }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t row = m0 + wm * 64u + i * 32u + (r / 4) * 8u + (lane >> 5) * 4u + (r % 4);
+          const uint32_t col = n0 + wn * 64u + j * 32u + (lane & 31u);
+          if (row >= g.M || col >= g.N)
+            continue;
+          const float v = acc[i][j][r];
+          const size_t o = size_t(row) * g.ldc + col;
+          if constexpr (kEpi == kEpiF32)
+            reinterpret_cast<float *>(g.C)[o] = v;
+          else
+            reinterpret_cast<float *>(g.C)[size_t(tile.z) * g.M * g.N + o] = v;
+        }
+  }
 }
 
 template <bool kAT, bool kBT>
@@ -170,7 +244,9 @@ static void launch_gemm_t(const GemmArgs &g, int epi, dim3 grid, hipStream_t s) 
 hipError_t launch_gemm(const GemmArgs &g, bool a_trans, bool b_trans, int epilogue,
                        uint32_t splits, hipStream_t s) {
   if (g.M % 8 || g.N % 8 || g.K % 8 || g.lda % 8 || g.ldb % 8 || g.k_per_split % 32 ||
-      splits == 0 || uint64_t(splits) * g.k_per_split < g.K)
+      splits == 0 || uint64_t(splits) * g.k_per_split < g.K || (g.CT && g.ldct % 4))
+    return hipErrorInvalidValue;
+  if (epilogue == kEpiMaskBf16 && (!g.mask_t || g.ldmt % 4))
     return hipErrorInvalidValue;
   dim3 grid((g.M + kBM - 1) / kBM, (g.N + kBN - 1) / kBN, splits);
   if (!a_trans && !b_trans)
@@ -387,11 +463,20 @@ hipError_t launch_mlp_forward(const float *features, uint32_t rows_pad, const bf
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
                                               const float *__restrict__ target, uint32_t n,
-                                              uint16_t *__restrict__ d_pred,
+                                              uint32_t rows_pad, uint16_t *__restrict__ d_pred,
+                                              uint16_t *__restrict__ d_pred_t, uint32_t ld_t,
                                               float *__restrict__ partials) {
   __shared__ float red[256];
   const uint32_t row = blockIdx.x * 256u + threadIdx.x;
   float l = 0.f;
+  if (row >= n && row < rows_pad) {
+#pragma unroll
+    for (uint32_t c = 0; c < kOutPad; ++c) {
+      d_pred[size_t(row) * kOutPad + c] = 0;
+      if (d_pred_t)
+        d_pred_t[size_t(c) * ld_t + row] = 0;
+    }
+  }
   if (row < n) {
     const float inv = 1.f / (3.f * float(n));
 #pragma unroll
@@ -405,6 +490,8 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
         d = f2bf(2.f * e / den * inv);
       }
       d_pred[size_t(row) * kOutPad + c] = d;
+      if (d_pred_t)
+        d_pred_t[size_t(c) * ld_t + row] = d;
     }
     l *= inv;
   }
@@ -419,31 +506,146 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
     partials[blockIdx.x] = red[0];
 }
 
-hipError_t launch_loss(const float *pred, const float *target, uint32_t n, bf16 *d_pred,
-                       float *loss_partials, uint32_t *n_partials, hipStream_t s) {
-  const uint32_t blocks = (n + 255) / 256;
-  *n_partials = blocks;
-  if (n == 0)
+hipError_t launch_loss(const float *pred, const float *target, uint32_t n, uint32_t rows_pad,
+                       bf16 *d_pred, bf16 *d_pred_t, uint32_t ld_t, float *loss_partials,
+                       uint32_t *n_partials, hipStream_t s) {
+  // partials cover the n valid rows (blocks past them add nothing)
+  *n_partials = (n + 255) / 256;
+  const uint32_t blocks = (rows_pad + 255) / 256;
+  if (rows_pad < n)
+    return hipErrorInvalidValue;
+  if (blocks == 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, pred, target, n,
-                     reinterpret_cast<uint16_t *>(d_pred), loss_partials);
+  hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, pred, target, n, rows_pad,
+                     reinterpret_cast<uint16_t *>(d_pred),
+                     reinterpret_cast<uint16_t *>(d_pred_t), ld_t, loss_partials);
   return hipGetLastError();
 }
 
-__global__ void k_sum_splits(const float *__restrict__ p, uint32_t splits, uint64_t count,
-                             float *__restrict__ out) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count)
+// One block: thread t sums partials t, t + 256, ... then a fixed-order tree.
+__global__ __launch_bounds__(256) void k_loss_sum(const float *__restrict__ p, uint32_t n,
+                                                  float *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256u)
+    s += p[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t off = 128; off >= 1; off >>= 1) {
+    if (threadIdx.x < off)
+      red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *out = float(red[0]);
+}
+
+hipError_t launch_loss_sum(const float *partials, uint32_t n, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, partials, n, out);
+  return hipGetLastError();
+}
+
+// 64 x 64 tiles through LDS (row stride 65 elements).
+__global__ __launch_bounds__(256) void k_transpose(const uint16_t *__restrict__ src,
+                                                   uint32_t rows, uint32_t cols, uint32_t ld_src,
+                                                   uint16_t *__restrict__ dst, uint32_t ld_dst) {
+  __shared__ uint16_t tile[64 * 65];
+  const uint32_t r0 = blockIdx.y * 64u, c0 = blockIdx.x * 64u;
+  for (uint32_t k = threadIdx.x; k < 64u * 64u; k += 256u) {
+    const uint32_t r = k / 64u, c = k % 64u;
+    if (r0 + r < rows && c0 + c < cols)
+      tile[r * 65u + c] = src[size_t(r0 + r) * ld_src + c0 + c];
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 64u * 64u; k += 256u) {
+    const uint32_t c = k / 64u, r = k % 64u;
+    if (r0 + r < rows && c0 + c < cols)
+      dst[size_t(c0 + c) * ld_dst + r0 + r] = tile[r * 65u + c];
+  }
+}
+
+hipError_t launch_transpose(const bf16 *src, uint32_t rows, uint32_t cols, uint32_t ld_src,
+                            bf16 *dst, uint32_t ld_dst, hipStream_t s) {
+  if (rows == 0 || cols == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_transpose, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0, s,
+                     reinterpret_cast<const uint16_t *>(src), rows, cols, ld_src,
+                     reinterpret_cast<uint16_t *>(dst), ld_dst);
+  return hipGetLastError();
+}
+
+// blockIdx.z = layer: the [out][in] matrix at layer l's offset becomes
+// [in][out] (every layer but the last is kWidth x kWidth).
+__global__ __launch_bounds__(256) void k_transpose_weights(const uint16_t *__restrict__ w,
+                                                           uint16_t *__restrict__ w_t) {
+  __shared__ uint16_t tile[64 * 65];
+  const uint32_t l = blockIdx.z;
+  const uint32_t in = kWidth, out = l + 1 == kLayers ? kOutPad : kWidth;
+  const size_t off = size_t(l) * kWidth * kWidth;
+  const uint32_t r0 = blockIdx.y * 64u, c0 = blockIdx.x * 64u;
+  if (r0 >= out)
     return;
-  float s = 0.f;
-  for (uint32_t z = 0; z < splits; ++z)
-    s += p[z * count + i];
-  out[i] = s;
+  for (uint32_t k = threadIdx.x; k < 64u * 64u; k += 256u) {
+    const uint32_t r = k / 64u, c = k % 64u;
+    if (r0 + r < out)
+      tile[r * 65u + c] = w[off + size_t(r0 + r) * in + c0 + c];
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 64u * 64u; k += 256u) {
+    const uint32_t c = k / 64u, r = k % 64u;
+    if (r0 + r < out)
+      w_t[off + size_t(c0 + c) * out + r0 + r] = tile[r * 65u + c];
+  }
+}
+
+hipError_t launch_transpose_weights(const bf16 *w, bf16 *w_t, hipStream_t s) {
+  static_assert(kInPad == kWidth, "layer offsets assume kWidth-wide inputs");
+  hipLaunchKernelGGL(k_transpose_weights, dim3(kWidth / 64, kWidth / 64, kLayers), dim3(256), 0,
+                     s, reinterpret_cast<const uint16_t *>(w), reinterpret_cast<uint16_t *>(w_t));
+  return hipGetLastError();
+}
+
+// Block of 256 threads per 256 outputs: wave w sums splits w, w + 4, ...
+// for its lane's 4 outputs (float4), then the 4 waves' sums are added in
+// wave order: a fixed order for every output.
+__global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ p,
+                                                    uint32_t splits, uint64_t count,
+                                                    float *__restrict__ out) {
+  __shared__ float4 part[4][64];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint64_t i4 = uint64_t(blockIdx.x) * 64u + lane; // float4 index
+  const uint64_t n4 = count / 4;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i4 < n4) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(p);
+#pragma unroll 4
+    for (uint32_t z = w; z < splits; z += 4u) {
+      const float4 v = p4[uint64_t(z) * n4 + i4];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+  }
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && i4 < n4) {
+    float4 s = part[0][lane];
+    for (uint32_t k = 1; k < 4; ++k) {
+      s.x += part[k][lane].x;
+      s.y += part[k][lane].y;
+      s.z += part[k][lane].z;
+      s.w += part[k][lane].w;
+    }
+    reinterpret_cast<float4 *>(out)[i4] = s;
+  }
 }
 
 hipError_t launch_sum_splits(const float *partials, uint32_t splits, uint64_t count,
                              float *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_splits, dim3(uint32_t((count + 255) / 256)), dim3(256), 0, s,
+  if (count % 4)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sum_splits, dim3(uint32_t((count / 4 + 63) / 64)), dim3(256), 0, s,
                      partials, splits, count, out);
   return hipGetLastError();
 }
