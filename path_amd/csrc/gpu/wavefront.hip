@@ -388,8 +388,11 @@ hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
 constexpr uint32_t kRefillMinIdle = 16;
 
 
-// Each wave owns one contiguous range of the queue and keeps its 64 lanes
-// busy: a lane whose ray finished takes the next entry of the range (ballot +
+// Each wave owns one contiguous range of the queue (sharing the second half
+// of the queue dynamically instead, in chunks of 64 from 16 pools, measured
+// 716 vs 762 Msamples/s on the 82k-triangle scene, and all of it 675: the
+// contiguous ranges keep neighbouring pixels' rays in one wave) and keeps its
+// 64 lanes busy: a lane whose ray finished takes the next entry of the range (ballot +
 // mbcnt, no atomics) while the other lanes continue, so a long ray delays
 // only itself. Lanes step one node or leaf at a time: holding leaves until
 // the whole wave has one (as trace_closest does) measured 438 vs 456
