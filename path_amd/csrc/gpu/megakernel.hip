@@ -196,16 +196,14 @@ __device__ __forceinline__ void reduce_store(const MegaParams &p, const ThreadIn
     }
     __syncthreads();
     if (writer) {
-      uint32_t ng = sbs / 64u, w0 = w;
-      float v[16][3];
-      for (uint32_t g = 0; g < ng; ++g)
-        for (int c = 0; c < 3; ++c)
-          v[g][c] = partial[w0 + g][c];
+      // the writer's own groups w .. w + ng - 1, summed in place (a private
+      // copy indexed at run time would live in scratch)
+      const uint32_t ng = sbs / 64u;
       for (uint32_t off = ng / 2; off >= 1; off /= 2)
         for (uint32_t i = 0; i < off; ++i)
           for (int c = 0; c < 3; ++c)
-            v[i][c] = v[i][c] + v[i + off][c];
-      total = {v[0][0], v[0][1], v[0][2]};
+            partial[w + i][c] = partial[w + i][c] + partial[w + i + off][c];
+      total = {partial[w][0], partial[w][1], partial[w][2]};
     }
   }
   if (writer) {
