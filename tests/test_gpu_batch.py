@@ -73,6 +73,34 @@ def test_gpu_tree_build_bit_exact(renderer, gen_scenes):
         assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
 
 
+def test_batch_with_spatial_split_trees(renderer, gen_scenes):
+    """Host trees built with spatial splits (some triangles referenced by
+    several leaves) give the batch the same images and G-buffers, bit for bit,
+    as the object-split trees; the scenes' trees are rebuilt when the setting
+    changes and again when it changes back."""
+    from path_amd.datagen import datagen_settings, render_batch, set_launch_params
+    from path_amd.renderer import Settings
+    set_launch_params(renderer, 8, 1, False)
+    st = datagen_settings()
+    base, base_g, _ = render_batch(renderer, gen_scenes, SPP, st)
+    refs0 = sum(len(g.scene.bvh()["perm"]) for g in gen_scenes)
+    sp = st.copy()
+    sp.use_spatial_splits = 1
+    try:
+        imgs, gbufs, _ = render_batch(renderer, gen_scenes, SPP, sp)
+        refs1 = sum(len(g.scene.bvh()["perm"]) for g in gen_scenes)
+    finally:
+        for g in gen_scenes:
+            g.scene.set_accel(Settings())
+    assert refs1 > refs0  # some triangles were split
+    for a, b, ga, gb in zip(base, imgs, base_g, gbufs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+    again, _, _ = render_batch(renderer, gen_scenes, SPP, st, gbuffer=False)
+    for a, b in zip(base, again):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_gbuffer_first_hit(renderer, gen_scenes):
     """The G-buffer position is the closest hit along the camera ray through
     it, its normal is the hit triangle's (unit, facing the camera side), and
