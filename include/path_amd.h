@@ -292,8 +292,19 @@ int pt_scene_bvh(const pt_scene *scene, uint32_t *n_nodes, uint32_t *n_refs, flo
 /* Tree build settings for the scene from the accel fields of settings
  * (traversal_per_intersect_cost, use_spatial_splits, overlap_threshold;
  * include/intersect/accel/sbvh/settings.h:9-19). Renders apply their own
- * settings the same way; a change rebuilds the tree on next use. */
+ * settings the same way; a change rebuilds the tree on next use. A scene
+ * must not be rendered from two threads at once with different accel
+ * settings (the tree is rebuilt in place). */
 int pt_scene_set_accel(pt_scene *scene, const pt_settings *settings);
+/* One node of the GPU tree from n (1-4) child boxes (lo, hi: n*3 floats) and
+ * child refs (>= 0 inner node, < 0 ~first triangle of a leaf): the 64-byte
+ * record the traversal reads (node), with the boxes it decodes for the four
+ * slots (decoded_lo / decoded_hi: 12 floats each, optional). Replaces the
+ * reference's packed node value (include/intersect/accel/detail/bvh/node.h:
+ * 22-126) in this build's 4-wide layout; PT_ERR_INVALID if a decoded box
+ * would not contain its child's box. */
+int pt_pack_node(const float *lo, const float *hi, uint32_t n, const int32_t *child,
+                 void *node, float *decoded_lo, float *decoded_hi);
 /* Bounds of the part of triangle tri (3 vertices) inside the box [lo, hi]
  * (empty, out_lo > out_hi, when they do not meet): the clipping a spatial
  * split applies to a triangle reference (ClippedTriangle::new_bounds,

@@ -921,6 +921,11 @@ struct ArrayRng {
   uint32_t n, used;
   double next() { return used < n ? v[used++] : 0.0; }
 };
+struct ArrayRngF {
+  const float *v;
+  uint32_t n, used;
+  float next() { return used < n ? v[used++] : 0.0f; }
+};
 Material<double> to_material(const pt_material &m) {
   return make_material<double>(
       m.type, V3<double>{m.diffuse[0], m.diffuse[1], m.diffuse[2]},
@@ -939,6 +944,31 @@ void oracle_bsdf_sample(const pt_material *mat, const double *in, const double *
   ArrayRng r{u, n_u, 0};
   DirSample<double> d = sample_direction(s, to_material(*mat), V3<double>{in[0], in[1], in[2]},
                                          V3<double>{n[0], n[1], n[2]}, r);
+  out[0] = d.dir.x, out[1] = d.dir.y, out[2] = d.dir.z;
+  out[3] = d.multiplier.x, out[4] = d.multiplier.y, out[5] = d.multiplier.z;
+  out[6] = d.is_discrete ? 1.0 : 0.0;
+  out[7] = double(r.used);
+}
+
+// The same in float (the GPU's precision): inputs and draws rounded to float.
+void oracle_bsdf_sample_f32(const pt_material *mat, const double *in, const double *n,
+                            const double *u, uint32_t n_u, int dir_sampler, double *out) {
+  IntegratorSettings s{};
+  s.dir_sampler = uint32_t(dir_sampler);
+  float uf[16];
+  const uint32_t m = n_u < 16 ? n_u : 16;
+  for (uint32_t i = 0; i < m; ++i)
+    uf[i] = float(u[i]);
+  ArrayRngF r{uf, m, 0};
+  const pt_material &pm = *mat;
+  const Material<float> mf = make_material<float>(
+      pm.type, V3<float>{pm.diffuse[0], pm.diffuse[1], pm.diffuse[2]},
+      V3<float>{pm.specular[0], pm.specular[1], pm.specular[2]}, pm.shininess, pm.ior,
+      pm.diffuse_weight, pm.glossy_weight,
+      V3<float>{pm.emission[0], pm.emission[1], pm.emission[2]});
+  DirSample<float> d = sample_direction(
+      s, mf, V3<float>{float(in[0]), float(in[1]), float(in[2])},
+      V3<float>{float(n[0]), float(n[1]), float(n[2])}, r);
   out[0] = d.dir.x, out[1] = d.dir.y, out[2] = d.dir.z;
   out[3] = d.multiplier.x, out[4] = d.multiplier.y, out[5] = d.multiplier.z;
   out[6] = d.is_discrete ? 1.0 : 0.0;

@@ -45,6 +45,8 @@ def lib():
         l.oracle_bsdf_sample.argtypes = [C.POINTER(L.PtMaterial), _D, _D, _D, C.c_uint32,
                                          C.c_int, _D]
         l.oracle_bsdf_sample.restype = None
+        l.oracle_bsdf_sample_f32.argtypes = l.oracle_bsdf_sample.argtypes
+        l.oracle_bsdf_sample_f32.restype = None
         l.oracle_bsdf_eval.argtypes = [C.POINTER(L.PtMaterial), _D, _D, _D, _D]
         l.oracle_bsdf_eval.restype = None
         l.oracle_term_prob.argtypes = [C.POINTER(L.PtSettings), C.c_uint32, _D]
@@ -264,12 +266,14 @@ def _d(a):
     return np.ascontiguousarray(a, np.float64).ctypes.data_as(_D)
 
 
-def bsdf_sample(mat, incoming, normal, u, dir_sampler=1):
-    """-> (dir, multiplier, is_discrete, n_values_consumed)."""
+def bsdf_sample(mat, incoming, normal, u, dir_sampler=1, precision=64):
+    """-> (dir, multiplier, is_discrete, n_values_consumed); precision 32
+    evaluates the shared code in float."""
     out = np.zeros(8)
     u = np.ascontiguousarray(u, np.float64)
-    lib().oracle_bsdf_sample(C.byref(mat), _d(incoming), _d(normal), u.ctypes.data_as(_D),
-                             u.size, dir_sampler, out.ctypes.data_as(_D))
+    fn = lib().oracle_bsdf_sample if precision == 64 else lib().oracle_bsdf_sample_f32
+    fn(C.byref(mat), _d(incoming), _d(normal), u.ctypes.data_as(_D), u.size, dir_sampler,
+       out.ctypes.data_as(_D))
     return out[:3], out[3:6], bool(out[6]), int(out[7])
 
 

@@ -151,6 +151,7 @@ SIGNATURES = {
     "pt_scene_bvh": (C.c_int, [_P, _U, _U, _F, _F, _U, _U, _U, _U, _U]),
     "pt_scene_set_accel": (C.c_int, [_P, C.POINTER(PtSettings)]),
     "pt_clip_triangle_bounds": (C.c_int, [_F, _F, _F, _F, _F]),
+    "pt_pack_node": (C.c_int, [_F, _F, C.c_uint32, C.POINTER(C.c_int32), C.c_void_p, _F, _F]),
     "pt_last_render_stats": (C.c_int, [_P, C.POINTER(PtRenderStats)]),
     "pt_set_profiling": (C.c_int, [_P, C.c_int]),
     "pt_scene_generate": (C.c_int, [C.c_uint64, C.POINTER(_P), _F, _U]),

@@ -14,6 +14,7 @@
 #include <rocrand/rocrand.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -274,6 +275,31 @@ int pt_scene_set_accel(pt_scene *scene, const pt_settings *settings) {
   scene->use_tree_settings(tree_settings(*settings));
   return PT_OK;
   PT_GUARD_END
+}
+
+int pt_pack_node(const float *lo, const float *hi, uint32_t n, const int32_t *child,
+                 void *node, float *decoded_lo, float *decoded_hi) {
+  if (!lo || !hi || !child || !node || n == 0 || n > 4)
+    return fail(PT_ERR_INVALID, "null argument or not 1-4 children");
+  float clo[4][3], chi[4][3];
+  for (uint32_t k = 0; k < n; ++k)
+    for (int a = 0; a < 3; ++a) {
+      clo[k][a] = lo[3 * k + a];
+      chi[k][a] = hi[3 * k + a];
+    }
+  GpuNode o;
+  const bool ok = quantise_gpu_node(clo, chi, n, child, &o);
+  std::memcpy(node, &o, sizeof(o));
+  // decode exactly as the traversal does: plane = fma(code, step, origin)
+  const uint8_t *ql[3] = {o.lo_x, o.lo_y, o.lo_z}, *qh[3] = {o.hi_x, o.hi_y, o.hi_z};
+  for (uint32_t k = 0; k < 4; ++k)
+    for (int a = 0; a < 3; ++a) {
+      if (decoded_lo)
+        decoded_lo[3 * k + a] = std::fma(float(ql[a][k]), o.step[a], o.origin[a]);
+      if (decoded_hi)
+        decoded_hi[3 * k + a] = std::fma(float(qh[a][k]), o.step[a], o.origin[a]);
+    }
+  return ok ? PT_OK : fail(PT_ERR_INVALID, "quantised node is not conservative");
 }
 
 int pt_clip_triangle_bounds(const float tri[9], const float lo[3], const float hi[3],
