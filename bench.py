@@ -118,6 +118,17 @@ def _roofline(name, ms, launches, n_rays, bpr, config, step_bytes=None, step_ms=
     return out
 
 
+def _trace_kernel(st, counts):
+    """The wavefront traversal kernel (one launch traces an iteration's
+    extension and shadow rays): (name, (ms, launches, rays, algorithmic bytes
+    per ray), algorithmic bytes of the step)."""
+    bpr_e, bpr_s = _bytes_per_ray(counts, 0), _bytes_per_ray(counts, 1)
+    n_rays = st.ext_rays + st.shadow_rays
+    step_bytes = st.ext_rays * bpr_e + st.shadow_rays * bpr_s
+    return "trace", (st.trace_ms, st.busy_trace_launches, n_rays,
+                     step_bytes / max(n_rays, 1)), step_bytes
+
+
 def _bytes_per_ray(counts, kind):
     n_rays, n_nodes, n_tris_t = (float(x) for x in counts[kind])
     if n_rays == 0:
@@ -324,15 +335,8 @@ def bench_datagen(a, cfg):
                       f"checker, {secs:.1f} s"}
         out["cpu_baseline"].update(host_cpu())
     roof = None
-    if counts is not None and pst.ext_launches:
-        kernels = {
-            "trace_ext": (pst.ext_ms, pst.ext_launches, pst.ext_rays, _bytes_per_ray(counts, 0)),
-            "trace_shadow": (pst.shadow_ms, pst.shadow_launches, pst.shadow_rays,
-                             _bytes_per_ray(counts, 1)),
-        }
-        name, vals = max(kernels.items(), key=lambda kv: kv[1][0])
-        step_bytes = (pst.ext_rays * kernels["trace_ext"][3] +
-                      pst.shadow_rays * kernels["trace_shadow"][3])
+    if counts is not None and pst.busy_trace_launches:
+        name, vals, step_bytes = _trace_kernel(pst, counts)
         roof = _roofline(name, *vals, config="c3", step_bytes=step_bytes,
                          step_ms=elapsed / a.steps * 1e3, pipelines=pst.pipelines)
     out["roofline"] = roof
@@ -1013,13 +1017,9 @@ def bench_render(a, cfg):
     roof = None
     if counts is not None:
         step_ms = elapsed / a.steps * 1e3
-        if cfg["approach"] == "Streaming" and st.ext_launches:
-            bpr_e, bpr_s = _bytes_per_ray(counts, 0), _bytes_per_ray(counts, 1)
-            kernels = {
-                "trace_ext": (st.ext_ms, st.ext_launches, st.ext_rays, bpr_e),
-                "trace_shadow": (st.shadow_ms, st.shadow_launches, st.shadow_rays, bpr_s),
-            }
-            step_bytes = st.ext_rays * bpr_e + st.shadow_rays * bpr_s
+        if cfg["approach"] == "Streaming" and st.busy_trace_launches:
+            name, vals, step_bytes = _trace_kernel(st, counts)
+            kernels = {name: vals}
         else:
             # mega-kernel: every ray of the render
             tot_rays = float(counts[0][0] + counts[1][0])

@@ -520,14 +520,15 @@ int pt_last_render_fp64(pt_renderer *r, double *out_rgb);
  * mega-kernel launch with its own events. */
 typedef struct pt_render_stats {
   double kernel_ms;
-  double trace_ms;         /* ext_ms + shadow_ms */
+  double trace_ms;         /* Streaming: traversal launches (profiled); each
+                              traces one iteration's extension and shadow
+                              rays together */
   uint64_t rays;           /* camera + extension + shadow rays traced */
   uint64_t samples;
-  uint64_t trace_launches;
-  double ext_ms;           /* extension-ray traversal kernels */
-  double shadow_ms;        /* shadow-ray traversal kernels */
+  uint64_t trace_launches; /* traversal launches enqueued (all pipelines) */
+  uint64_t busy_trace_launches; /* those that traced at least one ray (the
+                                   launches trace_ms covers) */
   uint64_t ext_rays, shadow_rays;
-  uint64_t ext_launches, shadow_launches;
   double mega_ms;          /* mega-kernel launches */
   uint64_t mega_launches;
   uint32_t profiled;

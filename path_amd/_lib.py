@@ -100,10 +100,9 @@ class PtOutput(C.Structure):
 class PtRenderStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double),
                 ("rays", C.c_uint64), ("samples", C.c_uint64),
-                ("trace_launches", C.c_uint64), ("ext_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("ext_rays", C.c_uint64),
-                ("shadow_rays", C.c_uint64), ("ext_launches", C.c_uint64),
-                ("shadow_launches", C.c_uint64), ("mega_ms", C.c_double),
+                ("trace_launches", C.c_uint64), ("busy_trace_launches", C.c_uint64),
+                ("ext_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("mega_ms", C.c_double),
                 ("mega_launches", C.c_uint64), ("profiled", C.c_uint32),
                 ("pipelines", C.c_uint32), ("replays", C.c_uint32)]
 

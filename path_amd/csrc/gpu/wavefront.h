@@ -118,12 +118,11 @@ struct WfParams {
 // graph).
 hipError_t launch_wf_reset(uint32_t *counters, uint32_t *group_counters, hipStream_t s);
 hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s);
-// The extension trace also writes the iteration's queue totals into
-// counters[kExtCount], counters[kShadowCount].
-hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
-                               uint32_t grid_blocks, hipStream_t s);
-hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,
-                                  uint32_t grid_blocks, hipStream_t s);
+// Closest hits of the iteration's extension rays and occlusion of its shadow
+// rays, both queues in one launch; also writes the iteration's queue totals
+// into counters[kExtCount], counters[kShadowCount].
+hipError_t launch_wf_trace(const WfParams &p, bool batched, uint32_t grid_blocks,
+                           hipStream_t s);
 // Per-pixel sum of the per-sample radiance in the mega-kernel's order
 // (sequential per thread range, then the lane tree), divided by spp.
 hipError_t launch_wf_reduce(const float *samples, const WorkDivision &wd,

@@ -378,8 +378,8 @@ hipError_t launch_wf_shade(const WfParams &p, bool batched, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Persistent traversal. Static schedule: wave w takes the 64-entry batches
-// w, w + W, w + 2W, ... of its queue (no same-address atomics).
+// Persistent traversal. Static schedule: wave w takes one contiguous range of
+// the concatenated queues (no same-address atomics).
 // ---------------------------------------------------------------------------
 // Lanes are refilled once at least this many are idle (or all are): a refill
 // runs the ray-setup code for the whole wave, so batching it pays (measured
@@ -405,32 +405,32 @@ constexpr uint32_t kRefillMinIdle = 16;
 // 760 / 754 / 749 / 728 / 682; one triangle per leaf step instead of the
 // whole leaf: 748.
 constexpr uint32_t kTraceVote = 3;
-template <bool kShadow, bool kBatch>
+// Both queues of an iteration in one launch: the extension segments, then
+// the shadow segments, as one concatenated range; a lane's entry decides its
+// ray kind (a shadow ray carries its target window and stops at the first
+// occluder). One launch instead of an extension launch followed by a shadow
+// launch measured 757 -> 856 Msamples/s on the 82k-triangle scene (C3 fresh
+// scenes 4,671 -> 5,056 scenes/s): one tail per iteration instead of two, and
+// waves that finish their extension rays continue with shadow rays.
+template <bool kBatch>
 __device__ __forceinline__ void wf_trace(const WfParams &p) {
   __shared__ int lds_stack[kLdsStackInts * 4];
-  // exclusive prefix of the group segment sizes (the shade step's counters)
-  __shared__ uint32_t s_prefix[kQueueGroups + 1];
+  __shared__ uint32_t s_prefix[2 * kQueueGroups + 1];
   if (threadIdx.x == 0) {
     uint32_t sum = 0;
-    for (uint32_t g = 0; g < kQueueGroups; ++g) {
+    for (uint32_t g = 0; g < 2 * kQueueGroups; ++g) {
       s_prefix[g] = sum;
-      sum += *group_counter(p.group_counters, kShadow ? kGroupShadow : kGroupExt, g);
+      sum += *group_counter(p.group_counters, g < kQueueGroups ? kGroupExt : kGroupShadow,
+                            g % kQueueGroups);
     }
-    s_prefix[kQueueGroups] = sum;
-    // the totals the host polls: written once per iteration, before the
-    // host's copy of the counters (which follows the extension trace)
-    if (!kShadow && blockIdx.x == 0) {
-      uint32_t shadow = 0;
-      for (uint32_t g = 0; g < kQueueGroups; ++g)
-        shadow += *group_counter(p.group_counters, kGroupShadow, g);
-      p.counters[kExtCount] = sum;
-      p.counters[kShadowCount] = shadow;
+    s_prefix[2 * kQueueGroups] = sum;
+    if (blockIdx.x == 0) {
+      p.counters[kExtCount] = s_prefix[kQueueGroups];
+      p.counters[kShadowCount] = sum - s_prefix[kQueueGroups];
     }
   }
   __syncthreads();
-  const uint32_t count = s_prefix[kQueueGroups];
-  const uint32_t *q = kShadow ? p.q.shadow : p.q.ext;
-  uint2 *hits = kShadow ? p.st.hit_light : p.st.hit_ext;
+  const uint32_t count = s_prefix[2 * kQueueGroups];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t wave = gtid >> 6, n_waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t per = (count + n_waves - 1) / n_waves;
@@ -442,7 +442,7 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
   TraceState ts;
   const float4 *nodes = p.scene.nodes, *tris = p.scene.tris;
   uint32_t slot = 0;
-  bool has = false;
+  bool has = false, shadow = false;
   for (;;) {
     const uint64_t idle = __ballot(!has);
     const uint32_t n_idle = uint32_t(__popcll(idle));
@@ -450,16 +450,17 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
       const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
           uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
       if (!has && next + pre < end) {
-        // entry v of the concatenated group segments
         const uint32_t v = next + pre;
         uint32_t g = 0;
-        for (uint32_t step = kQueueGroups / 2; step; step >>= 1)
+        for (uint32_t step = kQueueGroups; step; step >>= 1)
           if (s_prefix[g + step] <= v)
             g += step;
-        slot = q[size_t(g) * p.q.group_cap + (v - s_prefix[g])];
+        shadow = g >= kQueueGroups;
+        const uint32_t *q = shadow ? p.q.shadow : p.q.ext;
+        slot = q[size_t(g % kQueueGroups) * p.q.group_cap + (v - s_prefix[g])];
         const F3 o = p.st.origin[slot];
         float4 d;
-        if constexpr (kShadow) {
+        if (shadow) {
           d = p.st.light_dir[slot];
         } else {
           const F3 e = p.st.dir[slot];
@@ -473,12 +474,12 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
           tris = p.batch.tris + 3 * size_t(sd.tri_off);
           has_nodes = sd.has_nodes != 0;
         }
-        ts.init(ray, kShadow ? d.w + 2e-6f : 3.4028235e38f, kShadow ? d.w - 2e-6f : -1.0f);
+        ts.init(ray, shadow ? d.w + 2e-6f : 3.4028235e38f, shadow ? d.w - 2e-6f : -1.0f);
         stack.sp = 0;
         if (has_nodes)
           has = true;
         else
-          hits[slot] = pack_hit(Hit<float>{0.f, kNoHit, 0});
+          (shadow ? p.st.hit_light : p.st.hit_ext)[slot] = pack_hit(Hit<float>{0.f, kNoHit, 0});
       }
       next = min(next + n_idle, end);
     }
@@ -488,32 +489,23 @@ __device__ __forceinline__ void wf_trace(const WfParams &p) {
       continue;
     }
     if (ts.step_voted(nodes, tris, stack, has, kTraceVote)) {
-      hits[slot] = pack_hit(ts.hit());
+      (shadow ? p.st.hit_light : p.st.hit_ext)[slot] = pack_hit(ts.hit());
       has = false;
     }
   }
 }
 
-template <bool kShadow, bool kBatch>
+template <bool kBatch>
 __global__ __launch_bounds__(256) void k_wf_trace(WfParams p) {
-  wf_trace<kShadow, kBatch>(p);
+  wf_trace<kBatch>(p);
 }
 
-hipError_t launch_wf_trace_ext(const WfParams &p, bool batched,
-                               uint32_t grid_blocks, hipStream_t s) {
+hipError_t launch_wf_trace(const WfParams &p, bool batched, uint32_t grid_blocks,
+                           hipStream_t s) {
   if (batched)
-    hipLaunchKernelGGL((k_wf_trace<false, true>), dim3(grid_blocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((k_wf_trace<true>), dim3(grid_blocks), dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((k_wf_trace<false, false>), dim3(grid_blocks), dim3(256), 0, s, p);
-  return hipGetLastError();
-}
-
-hipError_t launch_wf_trace_shadow(const WfParams &p, bool batched,
-                                  uint32_t grid_blocks, hipStream_t s) {
-  if (batched)
-    hipLaunchKernelGGL((k_wf_trace<true, true>), dim3(grid_blocks), dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((k_wf_trace<true, false>), dim3(grid_blocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((k_wf_trace<false>), dim3(grid_blocks), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 

@@ -442,7 +442,6 @@ int pt_renderer::render(const pt_sample_spec &spec, const pt_output &out,
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
   stats.kernel_ms = ms;
-  stats.trace_ms = stats.ext_ms + stats.shadow_ms;
   stats.rays = settings.kernel_approach == PT_STREAMING
                    ? stats.ext_rays + stats.shadow_rays
                    : rays_traced;

@@ -96,8 +96,8 @@ int wavefront_alloc(pt_renderer &r, WfPipes &w, const gpu::WfParams &base,
 }
 
 // Enqueue one iteration of one pipeline: fold the previous queue counts into
-// the totals and zero them, shade, trace extension and shadow rays, copy the
-// counts to pinned host memory.
+// the totals and zero them, shade, trace the extension and shadow rays (one
+// launch), copy the counts to pinned host memory.
 static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p, int pipe,
                                     bool batched, uint32_t trace_blocks,
                                     bool profile_events) {
@@ -110,13 +110,9 @@ static hipError_t enqueue_iteration(pt_renderer &r, const gpu::WfParams &p, int 
     return e;
   if (profile_events && (e = hipEventRecord(ev[0], s)) != hipSuccess)
     return e;
-  if ((e = gpu::launch_wf_trace_ext(p, batched, trace_blocks, s)) != hipSuccess)
+  if ((e = gpu::launch_wf_trace(p, batched, trace_blocks, s)) != hipSuccess)
     return e;
   if (profile_events && (e = hipEventRecord(ev[1], s)) != hipSuccess)
-    return e;
-  if ((e = gpu::launch_wf_trace_shadow(p, batched, trace_blocks, s)) != hipSuccess)
-    return e;
-  if (profile_events && (e = hipEventRecord(ev[2], s)) != hipSuccess)
     return e;
   // queue totals of this iteration (written by the extension trace)
   return hipMemcpyAsync(r.wf_host_counts + size_t(pipe) * gpu::kNumCounters, p.counters,
@@ -155,7 +151,7 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
     hipError_t err = hipSuccess;
     std::string what;
     uint64_t launches = 0;
-    double ext_ms = 0.0, shadow_ms = 0.0;
+    double trace_ms = 0.0;
   };
   PipeResult res[kMaxPipes];
   auto run_pipe = [&](int h) {
@@ -187,11 +183,11 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
     for (;;) {
       if (use_graph) {
         out.err = hipGraphLaunch(exec, s);
-        out.launches += 2 * iters_per_sync;
+        out.launches += iters_per_sync;
       } else {
         for (uint32_t k = 0; k < iters_per_sync && out.err == hipSuccess; ++k) {
           out.err = enqueue_iteration(r, w.p[h], h, batched, trace_blocks, prof);
-          out.launches += 2;
+          out.launches += 1;
         }
       }
       hipError_t e = hipStreamSynchronize(s);
@@ -205,11 +201,9 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
       const uint32_t *c = r.wf_host_counts + size_t(h) * gpu::kNumCounters;
       const uint32_t ne = c[gpu::kExtCount], ns = c[gpu::kShadowCount];
       if (prof && ne + ns) {
-        float a = 0.f, b = 0.f;
+        float a = 0.f;
         (void)hipEventElapsedTime(&a, r.ev_prof[h][0], r.ev_prof[h][1]);
-        (void)hipEventElapsedTime(&b, r.ev_prof[h][1], r.ev_prof[h][2]);
-        out.ext_ms += a;
-        out.shadow_ms += b;
+        out.trace_ms += a;
       }
       // a slot holds a path iff it queued at least one ray
       if (ne + ns == 0)
@@ -240,8 +234,7 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
       return set_error(PT_ERR_HIP, "wavefront pipeline " + std::to_string(h) + " " +
                                        res[h].what + ": " + hipGetErrorString(res[h].err));
     launches += res[h].launches;
-    r.stats.ext_ms += res[h].ext_ms;
-    r.stats.shadow_ms += res[h].shadow_ms;
+    r.stats.trace_ms += res[h].trace_ms;
   }
   // every pipeline's last iteration queued nothing, so the totals are complete
   uint32_t c[kMaxPipes * gpu::kNumCounters];
@@ -258,9 +251,8 @@ int wavefront_run(pt_renderer &r, WfPipes &w, bool batched, uint32_t trace_block
   }
   r.stats.ext_rays = ext;
   r.stats.shadow_rays = shadow;
-  // launches of each traversal kernel over all pipelines
-  r.stats.ext_launches = busy_iters;
-  r.stats.shadow_launches = busy_iters;
+  // traversal launches over all pipelines (each traces both queues)
+  r.stats.busy_trace_launches = busy_iters;
   r.stats.trace_launches = launches;
   r.stats.pipelines = uint32_t(w.n);
   return PT_OK;

@@ -178,12 +178,9 @@ class RenderStats:
     rays: int
     samples: int
     trace_launches: int
-    ext_ms: float
-    shadow_ms: float
+    busy_trace_launches: int
     ext_rays: int
     shadow_rays: int
-    ext_launches: int
-    shadow_launches: int
     mega_ms: float
     mega_launches: int
     profiled: bool
@@ -407,9 +404,9 @@ class Renderer:
         s = L.PtRenderStats()
         L.check(L.lib().pt_last_render_stats(self.handle, C.byref(s)))
         return RenderStats(s.kernel_ms, s.trace_ms, s.rays, s.samples, s.trace_launches,
-                           s.ext_ms, s.shadow_ms, s.ext_rays, s.shadow_rays, s.ext_launches,
-                           s.shadow_launches, s.mega_ms, s.mega_launches, bool(s.profiled),
-                           int(s.pipelines), int(s.replays))
+                           s.busy_trace_launches, s.ext_rays, s.shadow_rays, s.mega_ms,
+                           s.mega_launches, bool(s.profiled), int(s.pipelines),
+                           int(s.replays))
 
     def l2_loss_gradient(self, image: DeviceBuffer, target: DeviceBuffer, n_floats: int,
                          out: DeviceBuffer) -> DeviceBuffer:
