@@ -193,6 +193,8 @@ def bench_datagen(a, cfg):
         kernel_ms += st.kernel_ms
         del batch
     tree_ms, tree_levels = last_bvh_build(r)
+    from path_amd.datagen import last_batch_phases
+    phases = {k: round(v, 2) for k, v in last_batch_phases(r).items()}
     elapsed = sum(fresh_s)
     _progress(f"datagen: {a.steps} new-scene batches in {elapsed:.2f} s")
     value = n * a.steps / elapsed
@@ -297,6 +299,7 @@ def bench_datagen(a, cfg):
         "gpu_ms_per_batch": round(kernel_ms / a.steps, 2),
         "gpu_tree_build_ms": round(tree_ms, 2),
         "gpu_tree_levels": tree_levels,
+        "fresh_batch_phases_ms": phases,
         "images_differing_from_host_trees": fresh_mismatch,
         "per_scene_launches": {"value": round(per_scene_value, 2), "unit": "scenes/s",
                                "seconds": round(per_scene_s, 3),
@@ -916,7 +919,7 @@ def other_workloads(a):
                 "mrays_per_s", "msamples_per_s", "speedup_vs_per_scene",
                 "images_differing_from_host_trees", "per_scene_launches",
                 "max_rel_err_vs_central_differences_fp32_renders", "forward_ms", "adjoint_ms",
-                "resident_batch_scenes_per_s", "gpu_tree_build_ms"]
+                "resident_batch_scenes_per_s", "gpu_tree_build_ms", "fresh_batch_phases_ms"]
         res[name] = {k: o[k] for k in keep if k in o}
     return res
 

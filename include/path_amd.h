@@ -410,6 +410,12 @@ int pt_set_bvh_builder(pt_renderer *r, int builder);
 /* Wall time (ms, host clock around the device work) and levels of the 4-wide
  * trees of the last GPU tree build. */
 int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels);
+/* Host wall time (ms) of the phases of the last batch whose scenes were not
+ * already resident: ms[0] scene preparation (flattening, shading records and,
+ * with host trees, the SAH builds; 16 host threads), ms[1] packing into the
+ * pinned staging buffer, ms[2] upload, ms[3] GPU tree builds (0 with host
+ * trees). */
+int pt_last_batch_phases(const pt_renderer *r, double ms[4]);
 /* Time candidate (path pool, traversal grid, iterations per sync, pipelines)
  * on the given items, keep the fastest on the renderer; out[5] = path pool,
  * blocks per CU, iterations per sync, graphs used, pipelines. */

@@ -143,6 +143,7 @@ SIGNATURES = {
     "pt_sobol_table": (C.c_int, [_P, C.c_uint32, C.c_uint32, _F]),
     "pt_sobol_direction_numbers": (C.c_int, [C.c_uint32, _U]),
     "pt_work_division": (C.c_int, [C.POINTER(PtWorkDivisionSettings), C.c_uint32, C.c_uint32, C.c_uint32, _U]),
+    "pt_last_batch_phases": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "pt_sort_pairs": (C.c_int, [_P, C.POINTER(C.c_uint64), _U, C.c_uint32, C.c_uint32,
                                 C.POINTER(C.c_uint64), _U]),
     "pt_reduce_samples": (C.c_int, [_P, C.POINTER(PtWorkDivisionSettings), _F, C.c_uint32,

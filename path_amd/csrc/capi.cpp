@@ -651,6 +651,14 @@ int pt_last_bvh_build(const pt_renderer *r, double *ms, uint32_t *levels) {
   return PT_OK;
 }
 
+int pt_last_batch_phases(const pt_renderer *r, double ms[4]) {
+  if (!r || !ms)
+    return fail(PT_ERR_INVALID, "null argument");
+  for (int k = 0; k < 4; ++k)
+    ms[k] = r->last_batch_phase_ms[k];
+  return PT_OK;
+}
+
 int pt_autotune_batch(pt_renderer *r, const pt_batch_item *items, uint32_t n,
                       uint32_t samples_per, pt_settings *settings,
                       uint32_t out[5]) {

@@ -305,10 +305,19 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
     }
   } else {
   r.batch_key.clear();
+  using clock = std::chrono::steady_clock;
+  auto ms_since = [](clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(clock::now() - t).count();
+  };
+  auto t_phase = clock::now();
   build_all(items, n, gpu_bvh);
+  r.last_batch_phase_ms[0] = ms_since(t_phase);
+  t_phase = clock::now();
   rc = pack(r, items, n, spp, gpu_bvh, &desc, &lay, &n_pixels, &total);
   if (rc != PT_OK)
     return rc;
+  r.last_batch_phase_ms[1] = ms_since(t_phase);
+  t_phase = clock::now();
   {
     const char *base = static_cast<const char *>(r.b_stage_in.ptr);
     auto up = [&](auto &buf, int sec) {
@@ -323,7 +332,10 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
     HIP_TRY(up(r.b_light_cdf, Layout::kLightCdf));
     HIP_TRY(up(r.b_desc, Layout::kDesc));
     HIP_TRY(up(r.b_sample_begin, Layout::kBegin));
+    HIP_TRY(hipStreamSynchronize(s));
   }
+  r.last_batch_phase_ms[2] = ms_since(t_phase);
+  r.last_batch_phase_ms[3] = 0.0;
   if (gpu_bvh) {
     bool fallback = false;
     uint32_t levels = 0;
@@ -341,6 +353,7 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
     r.last_gpu_bvh_ms = std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now() - tb).count();
     r.last_gpu_bvh_levels = levels;
+    r.last_batch_phase_ms[3] = r.last_gpu_bvh_ms;
   }
   HIP_TRY(hipStreamSynchronize(s));
   r.batch_key = std::move(key);

@@ -257,6 +257,10 @@ struct pt_renderer {
   bool gpu_bvh = false; // batched renders build their trees on the GPU (LBVH)
   double last_gpu_bvh_ms = 0.0;     // host wall time of the last GPU tree build
   uint32_t last_gpu_bvh_levels = 0; // levels of its deepest 4-wide tree
+  // phases of the last batch that was not resident (host wall ms): scene
+  // preparation (flattening, shading records, host trees), packing into the
+  // pinned staging buffer, upload, GPU tree builds
+  double last_batch_phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
   // GPU tree build scratch
   pt::host::DeviceBuf<uint32_t> adj_overflow;
   pt::host::DeviceBuf<float> lb_scene_box, lb_ibox, lb_lbox;

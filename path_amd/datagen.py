@@ -139,6 +139,14 @@ def last_bvh_build(renderer: Renderer) -> tuple:
     return float(ms.value), int(lv.value)
 
 
+def last_batch_phases(renderer: Renderer) -> dict:
+    """Host wall ms of the phases of the last batch that was not resident:
+    prepare (flatten, shading records, host trees), pack, upload, GPU trees."""
+    ms = (C.c_double * 4)()
+    L.check(L.lib().pt_last_batch_phases(renderer.handle, ms))
+    return {"prepare": ms[0], "pack": ms[1], "upload": ms[2], "gpu_trees": ms[3]}
+
+
 def datagen_settings() -> Settings:
     """Streaming integrator, Sobol sampling, BSDF directions, NEE."""
     s = Settings()
