@@ -367,7 +367,9 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ f
                                W + size_t(wave * 64u + 32u + l32) * kWidth + khalf};
     // Weight fragments are loaded kWAhead K steps before their MFMAs (a ring
     // of registers); scheduling barriers keep the compiler from sinking the
-    // loads next to their use, which left every step waiting on L2.
+    // loads next to their use, which left every step waiting on L2. Reading
+    // the activation fragments one K step ahead as well measured slower
+    // (0.96 vs 0.88 ms per 2^20 queries; 82 instead of 58 VGPRs).
     constexpr uint32_t kSteps = kWidth / 16;
     bf16x8_t bq[kWAhead + 1][2];
 #pragma unroll
