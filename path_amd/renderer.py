@@ -216,8 +216,13 @@ class DeviceBuffer:
                                         a.ctypes.data_as(C.c_void_p), a.nbytes))
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            L.lib().pt_device_free(self.renderer.handle, C.c_void_p(self.ptr))
+        # at interpreter exit the modules this needs may already be gone
+        ptr = getattr(self, "ptr", None)
+        if ptr and L is not None and L._lib is not None and C is not None:
+            try:
+                L.lib().pt_device_free(self.renderer.handle, C.c_void_p(ptr))
+            except TypeError:
+                pass
             self.ptr = None
 
 
