@@ -265,9 +265,8 @@ def bench_datagen(a, cfg):
         if not np.array_equal(img.view(np.uint32), images[i].view(np.uint32)):
             mismatched += 1
 
-    # profiled batched render for the per-kernel split
-    if tuned:
-        set_launch_params(r, tuned["trace_blocks_per_cu"], 1, False)
+    # profiled batched render for the per-kernel split (tuned or default grid)
+    set_launch_params(r, tuned["trace_blocks_per_cu"] if tuned else 0, 1, False)
     r.set_profiling(True)
     render_batch(r, gens, spp, settings, gbuffer=False)
     pst = r.last_stats()
@@ -945,7 +944,7 @@ def bench_render(a, cfg):
         set_pipelines(r, a.pipelines)
     if a.trace_blocks_per_cu or a.iters_per_sync:
         from path_amd.datagen import set_launch_params
-        set_launch_params(r, a.trace_blocks_per_cu or 4, a.iters_per_sync or 1,
+        set_launch_params(r, a.trace_blocks_per_cu, a.iters_per_sync or 1,
                           bool(a.iters_per_sync))
     size, spp = cfg["size"], cfg["spp"]
     # the image stays in HBM (a device buffer of the renderer), so the timed

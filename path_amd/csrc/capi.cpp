@@ -544,8 +544,7 @@ int pt_render_rays_batch(pt_renderer *r, const pt_ray_batch_item *items, uint32_
 
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
                          uint32_t iters_per_sync, int use_graphs) {
-  if (!r || trace_blocks_per_cu == 0 || trace_blocks_per_cu > 256 ||
-      iters_per_sync == 0 || iters_per_sync > 64)
+  if (!r || trace_blocks_per_cu > 256 || iters_per_sync == 0 || iters_per_sync > 64)
     return fail(PT_ERR_INVALID, "invalid launch parameters");
   r->trace_blocks_per_cu = trace_blocks_per_cu;
   r->iters_per_sync = iters_per_sync;

@@ -431,7 +431,7 @@ int render_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   if (slots == 0)
     return set_error(PT_ERR_INVALID, "max_num_samples_per_launch must be positive");
   slots = std::min<uint64_t>(slots, total);
-  const uint32_t trace_blocks = uint32_t(r.n_cus) * r.trace_blocks_per_cu;
+  const uint32_t trace_blocks = uint32_t(r.n_cus) * r.trace_blocks_for(true);
   gpu::WfParams p{};
   p.settings = integrator_settings(settings);
   p.rng = rng;
@@ -739,7 +739,7 @@ int render_rays_batch(pt_renderer &r, const pt_ray_batch_item *items, uint32_t n
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[5]) {
   const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
-  const uint32_t blocks[] = {4, 6, 8, 16};
+  const uint32_t blocks[] = {3, 4, 6, 8};
   const uint32_t iters[] = {1, 4};
   const uint32_t pipes[] = {2, 3, 4};
   double best = 1e30;

@@ -247,9 +247,17 @@ struct pt_renderer {
   int n_cus = 256;
   // wavefront launch parameters (tunable; results do not depend on them)
   // persistent traversal waves; lanes refill from the wave's queue range.
-  // Pipelines x blocks per CU on the 82k-triangle scene: 2 x 4 / 6 / 8
-  // measured 643 / 648 / 629 Msamples/s, 3 x 4 669.
-  uint32_t trace_blocks_per_cu = 4;
+  // Pipelines x blocks per CU on the 82k-triangle scene (one traversal
+  // launch per iteration, both ray queues shared evenly by the waves):
+  // 3 x 3 / 4 / 5 / 6 measured 894 / 881 / 867 / 843 Msamples/s, 2 x 6 / 8
+  // 870 / 838. Three pipelines of 3 blocks are the 9 traversal blocks (16.5 KB
+  // of LDS stack each) that fit a CU's 160 KB at once. Batches of many small
+  // scenes prefer 4 (1024 generated scenes: 5,205 vs 4,880 scenes/s). 0 =
+  // those defaults; pt_set_launch_params or the autotuner set a value.
+  uint32_t trace_blocks_per_cu = 0;
+  uint32_t trace_blocks_for(bool batched) const {
+    return trace_blocks_per_cu ? trace_blocks_per_cu : (batched ? 4u : 3u);
+  }
   // mega-kernel: several lanes per work-division thread (same image bits)
   bool mega_split = true;
   uint32_t iters_per_sync = 1;

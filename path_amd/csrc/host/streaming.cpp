@@ -278,7 +278,7 @@ int render_streaming(pt_renderer &r, const pt_sample_spec &spec,
     return set_error(PT_ERR_INVALID, "max_num_samples_per_launch must be positive");
   if (slots64 > total)
     slots64 = total;
-  const uint32_t trace_blocks = uint32_t(r.n_cus) * r.trace_blocks_per_cu;
+  const uint32_t trace_blocks = uint32_t(r.n_cus) * r.trace_blocks_for(false);
   gpu::WfParams p{};
   p.scene = scene;
   p.settings = integrator_settings(settings);

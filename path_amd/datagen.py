@@ -115,8 +115,10 @@ def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -
             "iters_per_sync": out[2], "graphs": bool(out[3]), "pipelines": out[4]}
 
 
-def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 4,
+def set_launch_params(renderer: Renderer, trace_blocks_per_cu: int = 0,
                       iters_per_sync: int = 1, use_graphs: bool = False) -> None:
+    """Wavefront launch parameters; trace_blocks_per_cu 0 = the defaults (3
+    per CU for a render, 4 for a batch)."""
     L.check(L.lib().pt_set_launch_params(renderer.handle, trace_blocks_per_cu,
                                          iters_per_sync, int(use_graphs)))
 
