@@ -298,7 +298,7 @@ __global__ __launch_bounds__(1024) void k_megakernel_batch(MegaBatchParams bp,
   }
   const MegaBatchItem &it = bp.items[lo];
   MegaParams p{};
-  p.scene.nodes = bp.nodes + 4 * size_t(it.node_off);
+  p.scene.nodes = bp.nodes + kNodeFloat4 * size_t(it.node_off);
   p.scene.tris = bp.tris + 3 * size_t(it.tri_off);
   p.scene.view.tris = bp.shade + it.shade_off;
   p.scene.view.materials = bp.materials + it.mat_off;
