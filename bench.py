@@ -45,6 +45,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Msamples/s on 1 GPU at fixed RMSE; MLP-assisted vs plain time-to-1%-RMSE"
 HBM_PEAK_GBPS = 8000.0
+ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 RAY_BYTES = 24 + 16          # ray read + hit write (reference AoS sizes)
 NODE_BYTES = 32              # bvh::Node
 TRI_BYTES = 36               # intersect::Triangle
@@ -108,13 +109,22 @@ def _roofline(name, ms, launches, n_rays, bpr, config, step_bytes=None, step_ms=
         out["measured_traffic_gbps"] = round(traffic / avg_s / 1e9, 2)
         out["measured_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBPS, 5)
     if prof.get("counters"):
-        out["binding"] = prof["counters"]
+        out["binding"] = dict(prof["counters"])
+        sv = prof.get("step_valu")
+        if sv and step_ms:
+            # VALU issue of all kernels of one step (counter run) over the
+            # measured step at the 2.4 GHz peak engine clock: the share of
+            # the step every SIMD spends issuing VALU instructions
+            out["binding"]["step_valu_busy"] = round(
+                sv["valu_cycles_per_simd_per_step"] / (step_ms * 1e-3 * ENGINE_CLOCK_HZ), 3)
+            out["binding"]["step_valu_cycles_per_simd"] = sv["valu_cycles_per_simd_per_step"]
     if out["frac"] > 1.0:
         out["note"] = ("frac > 1: the algorithmic bytes (the reference's binary-BVH AoS "
                        "traversal, target_objects = 4) exceed what HBM could deliver in the "
                        "step, because the 4-wide tree visits fewer nodes and its working set "
                        "stays in L2 / Infinity Cache; HBM is not the limit (measured_frac), "
-                       "the kernel is bound by issue and latency (binding)")
+                       "the kernel is bound by issue and latency (binding; step_valu_busy: "
+                       "the share of the step the SIMDs spend issuing VALU instructions)")
     return out
 
 
