@@ -48,19 +48,15 @@ template <class F> void parallel_for(size_t n, F f) {
     th.join();
 }
 
-// Flatten, build and pack the BVHs of the distinct scenes in parallel.
-void build_all(const pt_batch_item *items, uint32_t n, bool gpu_bvh) {
+// Flatten, build and pack the BVHs of the distinct scenes in parallel (host
+// trees; with GPU trees pack() writes the flattened arrays itself).
+void build_all(const pt_batch_item *items, uint32_t n) {
   std::vector<const pt_scene *> uniq;
   for (uint32_t i = 0; i < n; ++i)
     uniq.push_back(items[i].scene);
   std::sort(uniq.begin(), uniq.end());
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-  parallel_for(uniq.size(), [&](size_t k) {
-    if (gpu_bvh)
-      uniq[k]->shading();
-    else
-      uniq[k]->packed();
-  });
+  parallel_for(uniq.size(), [&](size_t k) { uniq[k]->packed(); });
 }
 
 // Byte layout of the concatenated scene arrays in the pinned staging buffer
@@ -75,7 +71,10 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // gpu_bvh: the trees are built on the device after the upload; scene i's
 // root takes node slot (its rank among the scenes with triangles) and its
-// leaf triangles sit at its shading offset.
+// leaf triangles sit at its shading offset. The shading records, materials
+// and emitters are then written straight into the pinned staging buffer from
+// the scene (the values flatten() gives), without the scene's cached
+// flattened arrays: no per-scene allocation and no second copy.
 int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, bool gpu_bvh,
          std::vector<gpu::SceneDesc> *desc, Layout *lay, uint64_t *n_pixels,
          uint64_t *total_samples) {
@@ -87,22 +86,31 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
   for (uint32_t i = 0; i < n; ++i) {
     const pt_batch_item &it = items[i];
     const pt_scene &sc = *it.scene;
-    const FlatScene &f = sc.flat();
-    const PackedScene &pk = gpu_bvh ? sc.shading() : sc.packed();
-    if (!gpu_bvh && pk.bvh.stack_need > uint32_t(gpu_stack_size()))
-      return set_error(PT_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
-                                               ": BVH too deep for the traversal stack");
+    size_t n_tris, n_lights;
+    const PackedScene *pk = nullptr;
+    if (gpu_bvh) {
+      n_tris = flat_triangle_count(sc.scene);
+      n_lights = flat_light_count(sc.scene);
+    } else {
+      const FlatScene &f = sc.flat();
+      pk = &sc.packed();
+      if (pk->bvh.stack_need > uint32_t(gpu_stack_size()))
+        return set_error(PT_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
+                                                 ": BVH too deep for the traversal stack");
+      n_tris = f.tris.size();
+      n_lights = f.light_tris.size();
+    }
     gpu::SceneDesc d{};
     d.node_off = uint32_t(c[Layout::kNodes]);
     d.tri_off = uint32_t(c[Layout::kTris]);
     d.shade_off = uint32_t(c[Layout::kShade]);
     d.mat_off = uint32_t(c[Layout::kMats]);
     d.light_off = uint32_t(c[Layout::kLightTris]);
-    d.n_lights = uint32_t(f.light_tris.size());
+    d.n_lights = uint32_t(n_lights);
     d.x_dim = it.x_dim;
     d.y_dim = it.y_dim;
     d.pixel_off = uint32_t(px_total);
-    d.has_nodes = f.tris.empty() ? 0u : 1u;
+    d.has_nodes = n_tris ? 1u : 0u;
     if (gpu_bvh) {
       d.node_off = roots;
       d.tri_off = d.shade_off;
@@ -111,13 +119,13 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
     std::memcpy(d.film_to_world, it.film_to_world, sizeof(d.film_to_world));
     (*desc)[i] = d;
     if (!gpu_bvh) {
-      c[Layout::kNodes] += pk.bvh.nodes.size();
-      c[Layout::kTris] += pk.bvh.tris.size();
+      c[Layout::kNodes] += pk->bvh.nodes.size();
+      c[Layout::kTris] += pk->bvh.tris.size();
     }
-    c[Layout::kShade] += pk.shade.size();
-    c[Layout::kMats] += pk.mats.size();
-    c[Layout::kLightTris] += f.light_tris.size();
-    c[Layout::kLightCdf] += f.light_cdf.size();
+    c[Layout::kShade] += n_tris;
+    c[Layout::kMats] += sc.scene.materials.size();
+    c[Layout::kLightTris] += n_lights;
+    c[Layout::kLightCdf] += n_lights;
     uint64_t px = uint64_t(it.x_dim) * it.y_dim;
     px_total += px;
     begin[i + 1] = begin[i] + px * spp;
@@ -144,17 +152,25 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
   std::memcpy(base + lay->off[Layout::kBegin], begin.data(), (n + 1) * sizeof(begin[0]));
   parallel_for(n, [&](size_t i) {
     const gpu::SceneDesc &d = (*desc)[i];
+    auto at = [&](int sec, size_t first) { return base + lay->off[sec] + first * elem[sec]; };
+    if (gpu_bvh) {
+      const Scene &scene = items[i].scene->scene;
+      write_shade_tris(scene, reinterpret_cast<ShadeTri<float> *>(at(Layout::kShade, d.shade_off)));
+      auto *m = reinterpret_cast<Material<float> *>(at(Layout::kMats, d.mat_off));
+      for (size_t k = 0; k < scene.materials.size(); ++k)
+        m[k] = scene.materials[k].to<float>();
+      flatten_lights(scene, reinterpret_cast<TriGeom<float> *>(at(Layout::kLightTris, d.light_off)),
+                     reinterpret_cast<float *>(at(Layout::kLightCdf, d.light_off)));
+      return;
+    }
     const FlatScene &f = items[i].scene->flat();
-    const PackedScene &pk = gpu_bvh ? items[i].scene->shading() : items[i].scene->packed();
+    const PackedScene &pk = items[i].scene->packed();
     auto put = [&](int sec, size_t first, const void *src, size_t bytes) {
       if (bytes)
-        std::memcpy(base + lay->off[sec] + first * elem[sec], src, bytes);
+        std::memcpy(at(sec, first), src, bytes);
     };
-    if (!gpu_bvh) {
-      put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(),
-          pk.bvh.nodes.size() * sizeof(GpuNode));
-      put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
-    }
+    put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(), pk.bvh.nodes.size() * sizeof(GpuNode));
+    put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
     put(Layout::kShade, d.shade_off, pk.shade.data(), pk.shade.size() * sizeof(ShadeTri<float>));
     put(Layout::kMats, d.mat_off, pk.mats.data(), pk.mats.size() * sizeof(Material<float>));
     put(Layout::kLightTris, d.light_off, f.light_tris.data(),
@@ -176,7 +192,7 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   std::vector<uint32_t> tri_begin(n + 1, 0), int_begin(n + 1, 0);
   std::vector<gpu::LbvhRoot> roots;
   for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t nt = uint32_t(items[i].scene->flat().tris.size());
+    const uint32_t nt = uint32_t(flat_triangle_count(items[i].scene->scene));
     tri_begin[i + 1] = tri_begin[i] + nt;
     int_begin[i + 1] = int_begin[i] + (nt >= 2 ? nt - 1 : 0);
     if (desc[i].has_nodes != (nt > 0 ? 1u : 0u) || (nt && desc[i].tri_off != tri_begin[i]))
@@ -310,7 +326,8 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
     return std::chrono::duration<double, std::milli>(clock::now() - t).count();
   };
   auto t_phase = clock::now();
-  build_all(items, n, gpu_bvh);
+  if (!gpu_bvh)
+    build_all(items, n);
   r.last_batch_phase_ms[0] = ms_since(t_phase);
   t_phase = clock::now();
   rc = pack(r, items, n, spp, gpu_bvh, &desc, &lay, &n_pixels, &total);

@@ -80,26 +80,45 @@ static TriGeom<float> tri_geom(V3f a, V3f b, V3f c) {
   return {a, b - a, c - a};
 }
 
-FlatScene flatten(const Scene &scene) {
-  FlatScene out;
-  out.materials = scene.materials;
+// World-space geometry and vertex normals of triangle j under transform tr
+// (flatten() and write_shade_tris() both use this, so their values agree).
+static inline void flatten_triangle(const Scene &scene, const Affine &tr, uint32_t j,
+                                    TriGeom<float> *g, V3f n[3]) {
+  const auto &v = scene.triangles[j];
+  *g = tri_geom(tr.apply(v[0]), tr.apply(v[1]), tr.apply(v[2]));
+  const TriangleData &d = scene.triangle_data[j];
+  for (int k = 0; k < 3; ++k)
+    n[k] = normalize(tr.apply_linear(d.normals[k]));
+}
+
+static inline uint32_t mesh_begin(const Scene &scene, uint32_t mesh) {
+  return mesh == 0 ? 0 : scene.mesh_ends[mesh - 1];
+}
+
+size_t flat_triangle_count(const Scene &scene) {
+  size_t n = 0;
   for (size_t i = 0; i < scene.instance_transforms.size(); ++i) {
-    uint32_t mesh = scene.instance_mesh_idx[i];
-    const Affine &tr = scene.instance_transforms[i];
-    uint32_t start = mesh == 0 ? 0 : scene.mesh_ends[mesh - 1];
-    uint32_t end = scene.mesh_ends[mesh];
-    for (uint32_t j = start; j < end; ++j) {
-      const auto &v = scene.triangles[j];
-      out.tris.push_back(
-          tri_geom(tr.apply(v[0]), tr.apply(v[1]), tr.apply(v[2])));
-      const TriangleData &d = scene.triangle_data[j];
-      for (int k = 0; k < 3; ++k)
-        out.normals.push_back(normalize(tr.apply_linear(d.normals[k])));
-      out.tri_mat.push_back(d.material_idx);
-    }
+    const uint32_t mesh = scene.instance_mesh_idx[i];
+    n += scene.mesh_ends[mesh] - mesh_begin(scene, mesh);
   }
-  // Emissive triangle CDF weighted by world-space area * emission.sum().
+  return n;
+}
+
+size_t flat_light_count(const Scene &scene) {
+  size_t n = 0;
+  for (size_t i = 0; i < scene.instance_transforms.size(); ++i) {
+    const uint32_t mesh = scene.instance_mesh_idx[i];
+    const uint32_t cstart = mesh == 0 ? 0 : scene.emissive_cluster_ends_per_mesh[mesh - 1];
+    for (uint32_t c = cstart; c < scene.emissive_cluster_ends_per_mesh[mesh]; ++c)
+      n += scene.emissive_clusters[c].end_idx - scene.emissive_clusters[c].start_idx;
+  }
+  return n;
+}
+
+// Emissive triangle CDF weighted by world-space area * emission.sum().
+void flatten_lights(const Scene &scene, TriGeom<float> *tris, float *cdf) {
   float cumulative = 0.0f;
+  size_t k = 0;
   for (size_t i = 0; i < scene.instance_transforms.size(); ++i) {
     uint32_t mesh = scene.instance_mesh_idx[i];
     const Affine &tr = scene.instance_transforms[i];
@@ -112,14 +131,51 @@ FlatScene flatten(const Scene &scene) {
         TriGeom<float> g = tri_geom(tr.apply(v[0]), tr.apply(v[1]), tr.apply(v[2]));
         float area = sqrt_(squared_norm(normal_scaled_by_area(g)));
         cumulative += area * sum(scene.materials[cl.material_idx].emission);
-        out.light_tris.push_back(g);
-        out.light_cdf.push_back(cumulative);
+        tris[k] = g;
+        cdf[k] = cumulative;
+        ++k;
       }
     }
   }
-  for (float &w : out.light_cdf)
-    w /= cumulative;
+  for (size_t j = 0; j < k; ++j)
+    cdf[j] /= cumulative;
+}
+
+FlatScene flatten(const Scene &scene) {
+  FlatScene out;
+  out.materials = scene.materials;
+  const size_t nt = flat_triangle_count(scene);
+  out.tris.resize(nt);
+  out.normals.resize(3 * nt);
+  out.tri_mat.resize(nt);
+  size_t k = 0;
+  for (size_t i = 0; i < scene.instance_transforms.size(); ++i) {
+    const uint32_t mesh = scene.instance_mesh_idx[i];
+    const Affine &tr = scene.instance_transforms[i];
+    for (uint32_t j = mesh_begin(scene, mesh); j < scene.mesh_ends[mesh]; ++j, ++k) {
+      flatten_triangle(scene, tr, j, &out.tris[k], &out.normals[3 * k]);
+      out.tri_mat[k] = scene.triangle_data[j].material_idx;
+    }
+  }
+  const size_t nl = flat_light_count(scene);
+  out.light_tris.resize(nl);
+  out.light_cdf.resize(nl);
+  flatten_lights(scene, out.light_tris.data(), out.light_cdf.data());
   return out;
+}
+
+void write_shade_tris(const Scene &scene, ShadeTri<float> *out) {
+  size_t k = 0;
+  for (size_t i = 0; i < scene.instance_transforms.size(); ++i) {
+    const uint32_t mesh = scene.instance_mesh_idx[i];
+    const Affine &tr = scene.instance_transforms[i];
+    for (uint32_t j = mesh_begin(scene, mesh); j < scene.mesh_ends[mesh]; ++j, ++k) {
+      TriGeom<float> g;
+      V3f n[3];
+      flatten_triangle(scene, tr, j, &g, n);
+      out[k] = make_shade_tri(g, n, scene.triangle_data[j].material_idx);
+    }
+  }
 }
 
 Camera camera_from_look(V3f look_f, V3f up_f, V3f pos_f, float height_angle_deg,

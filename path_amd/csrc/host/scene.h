@@ -109,6 +109,14 @@ struct Camera {
 // reference: include/intersectable_scene/flat_triangle/flat_triangle.h:79-146,
 // include/integrate/light_sampler/random_triangle/random_triangle.h:120-161
 FlatScene flatten(const Scene &scene);
+// Pieces of flatten() for callers that write the arrays in place: the
+// triangle and emitter counts, the shading records of every triangle (the
+// values make_shade_tri() gives for flatten()'s output) and the emitters with
+// their normalised CDF (flat_light_count() entries each).
+size_t flat_triangle_count(const Scene &scene);
+size_t flat_light_count(const Scene &scene);
+void write_shade_tris(const Scene &scene, ShadeTri<float> *out);
+void flatten_lights(const Scene &scene, TriGeom<float> *tris, float *cdf);
 
 // reference: src/scene/camera.cpp:4-27
 Camera camera_from_look(V3f look, V3f up, V3f pos, float height_angle_deg,
