@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -150,10 +151,25 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
   char *base = static_cast<char *>(r.b_stage_in.ptr);
   std::memcpy(base + lay->off[Layout::kDesc], desc->data(), n * sizeof(gpu::SceneDesc));
   std::memcpy(base + lay->off[Layout::kBegin], begin.data(), (n + 1) * sizeof(begin[0]));
-  parallel_for(n, [&](size_t i) {
-    const gpu::SceneDesc &d = (*desc)[i];
-    auto at = [&](int sec, size_t first) { return base + lay->off[sec] + first * elem[sec]; };
-    if (gpu_bvh) {
+  auto at = [&](int sec, size_t first) { return base + lay->off[sec] + first * elem[sec]; };
+  if (gpu_bvh) {
+    // Scenes are written in order by 16 threads; the calling thread also
+    // sends each finished prefix of scenes to the device (at least 16 MB of
+    // shading records at a time, or the rest), so the PCIe upload runs while
+    // later scenes are still being written.
+    hipError_t e = hipSuccess;
+    auto grow = [&](auto &buf, int sec) {
+      if (e == hipSuccess)
+        e = buf.ensure(std::max<size_t>(1, c[sec]));
+    };
+    grow(r.b_shade, Layout::kShade);
+    grow(r.b_mats, Layout::kMats);
+    grow(r.b_light_tris, Layout::kLightTris);
+    grow(r.b_light_cdf, Layout::kLightCdf);
+    if (e != hipSuccess)
+      return set_error(PT_ERR_HIP, std::string("batch arrays: ") + hipGetErrorString(e));
+    auto fill = [&](size_t i) {
+      const gpu::SceneDesc &d = (*desc)[i];
       const Scene &scene = items[i].scene->scene;
       write_shade_tris(scene, reinterpret_cast<ShadeTri<float> *>(at(Layout::kShade, d.shade_off)));
       auto *m = reinterpret_cast<Material<float> *>(at(Layout::kMats, d.mat_off));
@@ -161,22 +177,70 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
         m[k] = scene.materials[k].to<float>();
       flatten_lights(scene, reinterpret_cast<TriGeom<float> *>(at(Layout::kLightTris, d.light_off)),
                      reinterpret_cast<float *>(at(Layout::kLightCdf, d.light_off)));
-      return;
-    }
-    const FlatScene &f = items[i].scene->flat();
-    const PackedScene &pk = items[i].scene->packed();
-    auto put = [&](int sec, size_t first, const void *src, size_t bytes) {
-      if (bytes)
-        std::memcpy(at(sec, first), src, bytes);
     };
-    put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(), pk.bvh.nodes.size() * sizeof(GpuNode));
-    put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
-    put(Layout::kShade, d.shade_off, pk.shade.data(), pk.shade.size() * sizeof(ShadeTri<float>));
-    put(Layout::kMats, d.mat_off, pk.mats.data(), pk.mats.size() * sizeof(Material<float>));
-    put(Layout::kLightTris, d.light_off, f.light_tris.data(),
-        f.light_tris.size() * sizeof(TriGeom<float>));
-    put(Layout::kLightCdf, d.light_off, f.light_cdf.data(), f.light_cdf.size() * sizeof(float));
-  });
+    std::unique_ptr<std::atomic<uint8_t>[]> done(new std::atomic<uint8_t>[n]());
+    std::atomic<size_t> next{0};
+    size_t sent = 0; // scenes whose arrays have been queued for upload
+    auto send = [&](size_t upto) {
+      const gpu::SceneDesc &a = (*desc)[sent];
+      auto copy = [&](auto &buf, int sec, size_t from, size_t to) {
+        if (e == hipSuccess && to > from)
+          e = hipMemcpyAsync(buf.ptr + from, at(sec, from), (to - from) * elem[sec],
+                             hipMemcpyHostToDevice, r.stream);
+      };
+      const bool last = upto == n;
+      const gpu::SceneDesc *b = last ? nullptr : &(*desc)[upto];
+      copy(r.b_shade, Layout::kShade, a.shade_off, last ? c[Layout::kShade] : b->shade_off);
+      copy(r.b_mats, Layout::kMats, a.mat_off, last ? c[Layout::kMats] : b->mat_off);
+      copy(r.b_light_tris, Layout::kLightTris, a.light_off,
+           last ? c[Layout::kLightTris] : b->light_off);
+      copy(r.b_light_cdf, Layout::kLightCdf, a.light_off,
+           last ? c[Layout::kLightCdf] : b->light_off);
+      sent = upto;
+    };
+    const size_t min_send = (size_t(16) << 20) / sizeof(ShadeTri<float>);
+    auto worker = [&](bool sender) {
+      for (size_t i = next++; i < n; i = next++) {
+        fill(i);
+        done[i].store(1, std::memory_order_release);
+        if (!sender)
+          continue;
+        size_t p = sent;
+        while (p < n && done[p].load(std::memory_order_acquire))
+          ++p;
+        if (p > sent && (p == n || (*desc)[p].shade_off - (*desc)[sent].shade_off >= min_send))
+          send(p);
+      }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < std::min<size_t>(hw, n); ++t)
+      pool.emplace_back(worker, false);
+    worker(true);
+    for (auto &th : pool)
+      th.join();
+    if (sent < n)
+      send(n);
+    if (e != hipSuccess)
+      return set_error(PT_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e));
+  } else {
+    parallel_for(n, [&](size_t i) {
+      const gpu::SceneDesc &d = (*desc)[i];
+      const FlatScene &f = items[i].scene->flat();
+      const PackedScene &pk = items[i].scene->packed();
+      auto put = [&](int sec, size_t first, const void *src, size_t bytes) {
+        if (bytes)
+          std::memcpy(at(sec, first), src, bytes);
+      };
+      put(Layout::kNodes, d.node_off, pk.bvh.nodes.data(), pk.bvh.nodes.size() * sizeof(GpuNode));
+      put(Layout::kTris, d.tri_off, pk.bvh.tris.data(), pk.bvh.tris.size() * sizeof(GpuTri));
+      put(Layout::kShade, d.shade_off, pk.shade.data(), pk.shade.size() * sizeof(ShadeTri<float>));
+      put(Layout::kMats, d.mat_off, pk.mats.data(), pk.mats.size() * sizeof(Material<float>));
+      put(Layout::kLightTris, d.light_off, f.light_tris.data(),
+          f.light_tris.size() * sizeof(TriGeom<float>));
+      put(Layout::kLightCdf, d.light_off, f.light_cdf.data(), f.light_cdf.size() * sizeof(float));
+    });
+  }
   *n_pixels = px_total;
   *total_samples = begin[n];
   return PT_OK;
@@ -341,12 +405,14 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
       using T = std::remove_pointer_t<decltype(buf.ptr)>;
       return buf.upload(reinterpret_cast<const T *>(base + lay.off[sec]), lay.count[sec], s);
     };
-    HIP_TRY(up(r.b_nodes, Layout::kNodes));
-    HIP_TRY(up(r.b_tris, Layout::kTris));
-    HIP_TRY(up(r.b_shade, Layout::kShade));
-    HIP_TRY(up(r.b_mats, Layout::kMats));
-    HIP_TRY(up(r.b_light_tris, Layout::kLightTris));
-    HIP_TRY(up(r.b_light_cdf, Layout::kLightCdf));
+    if (!gpu_bvh) { // with GPU trees pack() has queued these already
+      HIP_TRY(up(r.b_nodes, Layout::kNodes));
+      HIP_TRY(up(r.b_tris, Layout::kTris));
+      HIP_TRY(up(r.b_shade, Layout::kShade));
+      HIP_TRY(up(r.b_mats, Layout::kMats));
+      HIP_TRY(up(r.b_light_tris, Layout::kLightTris));
+      HIP_TRY(up(r.b_light_cdf, Layout::kLightCdf));
+    }
     HIP_TRY(up(r.b_desc, Layout::kDesc));
     HIP_TRY(up(r.b_sample_begin, Layout::kBegin));
     HIP_TRY(hipStreamSynchronize(s));
