@@ -184,7 +184,7 @@ def bench_datagen(a, cfg):
     # and G-buffers left in HBM (one device buffer each, batch order).
     from path_amd.datagen import last_bvh_build, set_bvh_builder
     sync = _sync_fn()
-    set_bvh_builder(r, True)
+    set_bvh_builder(r, a.gpu_tree)
     dev = render_batch_device(r, [generate_scene(seed) for seed in range(n)], spp, settings)
     sync()
     fresh_s, rays, kernel_ms, gen_step_s = [], 0, 0.0, []
@@ -306,6 +306,7 @@ def bench_datagen(a, cfg):
         "msamples_per_s": round(total_px * spp * a.steps / elapsed / 1e6, 2),
         "mrays_per_s": round(rays / elapsed / 1e6, 2),
         "gpu_ms_per_batch": round(kernel_ms / a.steps, 2),
+        "gpu_tree_builder": a.gpu_tree,
         "gpu_tree_build_ms": round(tree_ms, 2),
         "gpu_tree_levels": tree_levels,
         "fresh_batch_phases_ms": phases,
@@ -855,6 +856,8 @@ def main(argv=None):
                     help="wavefront iterations enqueued per host check (hipGraph replay)")
     ap.add_argument("--no-other-workloads", action="store_true",
                     help="default run: skip the short C1 / C4 / C3 runs in other_workloads")
+    ap.add_argument("--gpu-tree", choices=["ploc", "lbvh"], default="ploc",
+                    help="GPU tree builder of the batched datagen workload (c3)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c3: keep the default launch parameters")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -928,7 +931,8 @@ def other_workloads(a):
                 "mrays_per_s", "msamples_per_s", "speedup_vs_per_scene",
                 "images_differing_from_host_trees", "per_scene_launches",
                 "max_rel_err_vs_central_differences_fp32_renders", "forward_ms", "adjoint_ms",
-                "resident_batch_scenes_per_s", "gpu_tree_build_ms", "fresh_batch_phases_ms"]
+                "resident_batch_scenes_per_s", "gpu_tree_builder", "gpu_tree_build_ms",
+                "fresh_batch_phases_ms"]
         res[name] = {k: o[k] for k in keep if k in o}
     return res
 

@@ -379,11 +379,13 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
 /* Tree builder of pt_render_batch (no reference counterpart: the reference
  * builds its SBVH on the host, src/intersect/accel/sbvh/detail/generator.cu).
  * PT_BVH_HOST_SAH (default): each scene's SAH tree is built on host threads
- * and cached with the scene. PT_BVH_GPU_LBVH: all trees of a batch are built
- * on the GPU in one pass (Morton-sorted LBVH collapsed to the same 4-wide
- * quantised nodes); a batch with a tree deeper than the traversal stack
- * allows falls back to the host builder. Images do not depend on the tree. */
-enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1 };
+ * and cached with the scene. PT_BVH_GPU_LBVH / PT_BVH_GPU_PLOC: all trees of
+ * a batch are built on the GPU in one pass over Morton-sorted triangles (the
+ * binary radix tree, or locally-ordered clustering by surface area, whose
+ * trees trace faster), collapsed to the same 4-wide quantised nodes; a batch
+ * with a tree deeper than the traversal stack allows falls back to the host
+ * builder. Images do not depend on the tree. */
+enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1, PT_BVH_GPU_PLOC = 2 };
 /* Independent wavefront pipelines a Streaming render splits its path pool
  * into (1..4, default 3; each at least 256 paths), each on its own stream and
  * host thread. Results never depend on it. */

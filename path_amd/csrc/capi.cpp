@@ -636,9 +636,12 @@ int pt_set_megakernel_split(pt_renderer *r, int enable) {
 }
 
 int pt_set_bvh_builder(pt_renderer *r, int builder) {
-  if (!r || (builder != PT_BVH_HOST_SAH && builder != PT_BVH_GPU_LBVH))
+  if (!r || (builder != PT_BVH_HOST_SAH && builder != PT_BVH_GPU_LBVH &&
+             builder != PT_BVH_GPU_PLOC))
     return fail(PT_ERR_INVALID, "unknown BVH builder");
-  r->gpu_bvh = builder == PT_BVH_GPU_LBVH;
+  r->gpu_bvh = builder != PT_BVH_HOST_SAH;
+  if (r->gpu_bvh)
+    r->gpu_builder = builder == PT_BVH_GPU_PLOC ? gpu::kLbvhPloc : gpu::kLbvhRadixTree;
   return PT_OK;
 }
 

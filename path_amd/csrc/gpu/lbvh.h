@@ -1,21 +1,30 @@
-// GPU BVH build for batched rendering (LBVH): per scene, triangles sorted by
-// the 30-bit Morton code of their box centre within the scene's centroid
-// bounds (scene id in the key's high word, one radix sort for the whole
-// batch), the binary radix tree of the sorted codes (Karras, HPG 2012: one
-// thread per internal node, ties broken by position), boxes bottom-up (the
-// second child to arrive at a node computes it), then the 4-wide node
-// records top-down, one level per launch: each node opens its largest-area
-// inner child until it has four children, subtrees of at most kLbvhLeafSize
-// triangles become leaves, and the child boxes are padded and quantised with
-// the host packer's rule (core/bvh_layout.h). Closest hits do not depend on
-// the tree (padded conservative boxes, order-independent hit rule), so a
-// batch renders the same images with this tree as with the host SAH tree.
+// GPU BVH build for batched rendering: per scene, triangles sorted by the
+// 30-bit Morton code of their box centre within the scene's centroid bounds
+// (scene id in the key's high word, one radix sort for the whole batch), then
+// a binary tree over the sorted triangles by one of two builders:
+//  * LBVH: the binary radix tree of the sorted codes (Karras, HPG 2012: one
+//    thread per internal node, ties broken by position), boxes bottom-up (the
+//    second child to arrive at a node computes it);
+//  * PLOC (Meister and Bittner, "Parallel Locally-Ordered Clustering for
+//    Bounding Volume Hierarchy Construction", IEEE TVCG 2018): clusters in
+//    Morton order, each pass pairs every cluster with the neighbour within
+//    kPlocRadius positions whose union box has the least surface area and
+//    merges the mutual pairs, until one cluster is left (one workgroup per
+//    scene). Its trees are closer to SAH trees than the radix tree's.
+// Then the 4-wide node records top-down, one level per launch: each node
+// opens its largest-area inner child until it has four children, subtrees of
+// at most kLbvhLeafSize triangles become leaves whose triangles the level
+// writes at the node's output range, and the child boxes are padded and
+// quantised with the host packer's rule (core/bvh_layout.h). Closest hits do
+// not depend on the tree (padded conservative boxes, order-independent hit
+// rule), so a batch renders the same images with these trees as with the
+// host SAH tree.
 //
 // Numbering: the batch's triangles are global positions 0..n_tris-1 (scene s
-// owns [tri_begin[s], tri_begin[s+1]) both before and after sorting); scene s
+// owns [tri_begin[s], tri_begin[s+1]) in sorted and in output order); scene s
 // owns internal nodes [int_begin[s], int_begin[s+1]), n - 1 of them for n >= 2
-// triangles. A binary child ref >= 0 is an internal node, < 0 is ~(sorted
-// triangle position).
+// triangles, its root first. A binary child ref >= 0 is an internal node,
+// < 0 is ~(sorted triangle position).
 #pragma once
 
 #include "../core/bvh_layout.h"
@@ -25,6 +34,8 @@ namespace pt {
 namespace gpu {
 
 constexpr uint32_t kLbvhLeafSize = 4;
+constexpr uint32_t kLbvhLeafEnd = 0x80000000u; // last triangle of a leaf (gpu/trace.h)
+constexpr int kPlocRadius = 8;
 // traversal stack entries a 4-wide tree of this many levels can need
 constexpr uint32_t kLbvhMaxLevels = 21; // 3 * (20 + 1) = 63 <= 64
 
@@ -37,12 +48,14 @@ struct LbvhArgs {
   unsigned long long *keys, *keys_sorted;
   uint32_t *vals, *vals_sorted; // global triangle index
   int32_t *left, *right;        // [n_internal] binary child refs
-  uint32_t *first, *last;       // [n_internal] sorted position range
+  uint32_t *count;              // [n_internal] triangles under the node
   int32_t *parent_int;          // [n_internal], -1 at a root
   int32_t *parent_leaf;         // [n_tris], -1 for a single-triangle scene
   float *ibox;                  // [n_internal][6] lo xyz, hi xyz
   float *lbox;                  // [n_tris][6]
   uint32_t *visits;             // [n_internal]
+  float *clusters[2];           // [n_tris][8] PLOC clusters: lo xyz + ref, hi xyz + count
+  int32_t *nn;                  // [n_tris] PLOC nearest neighbours
   GpuNode *nodes;               // output, global slots
   GpuTri *tris;                 // output, sorted positions
   uint32_t *counters;           // [0] nodes allocated, [1] next frontier size, [2] error
@@ -157,8 +170,7 @@ PT_INLINE void lbvh_karras(const LbvhArgs &a, uint32_t g) {
   }
   a.left[g] = lref;
   a.right[g] = rref;
-  a.first[g] = base + uint32_t(lo);
-  a.last[g] = base + uint32_t(hi);
+  a.count[g] = uint32_t(hi - lo + 1);
 }
 
 } // namespace gpu
@@ -172,30 +184,26 @@ struct LbvhRoot {
   uint32_t slot;      // its node slot (= the scene's node offset)
   uint32_t scene;
   uint32_t root_slot; // slot of the scene's root (child refs are relative to it)
+  uint32_t tri_first; // output position of the subtree's first triangle
 };
 
-// Sorted position p: the leaf triangle record and box, then the boxes of the
-// ancestors this thread is the second to reach. Ops: fence(), add(u32*, v)
-// (returns the old value), or(u32*, v).
-template <class Ops>
-PT_INLINE void lbvh_leaf(const LbvhArgs &a, uint32_t p, Ops &ops) {
-  const uint32_t t = a.vals_sorted[p];
-  const uint32_t s = lbvh_find(a.tri_begin, a.n_scenes, p);
-  const ShadeTri<float> &st = a.shade[t];
-  GpuTri o;
-  o.v0[0] = st.v0.x, o.v0[1] = st.v0.y, o.v0[2] = st.v0.z;
-  o.idx = t - a.tri_begin[s];
-  o.e1[0] = st.e1.x, o.e1[1] = st.e1.y, o.e1[2] = st.e1.z;
-  o.e2[0] = st.e2.x, o.e2[1] = st.e2.y, o.e2[2] = st.e2.z;
-  o.pad0 = o.pad1 = 0;
-  a.tris[p] = o;
+// Box of the triangle at sorted position p.
+PT_INLINE void lbvh_leaf_box(const LbvhArgs &a, uint32_t p) {
   float lo[3], hi[3];
-  lbvh_tri_box(st, lo, hi);
+  lbvh_tri_box(a.shade[a.vals_sorted[p]], lo, hi);
   float *lb = a.lbox + 6 * size_t(p);
   for (int k = 0; k < 3; ++k) {
     lb[k] = lo[k];
     lb[3 + k] = hi[k];
   }
+}
+
+// Sorted position p (LBVH): the leaf box, then the boxes of the ancestors
+// this thread is the second to reach. Ops: fence(), add(u32*, v) (returns the
+// old value), or(u32*, v).
+template <class Ops>
+PT_INLINE void lbvh_leaf(const LbvhArgs &a, uint32_t p, Ops &ops) {
+  lbvh_leaf_box(a, p);
   int32_t node = a.parent_leaf[p];
   while (node >= 0) {
     ops.fence(); // publish this child's box before arriving
@@ -220,8 +228,40 @@ PT_INLINE void lbvh_leaf(const LbvhArgs &a, uint32_t p, Ops &ops) {
   }
 }
 
+PT_INLINE uint32_t lbvh_count(const LbvhArgs &a, int32_t ref) {
+  return ref < 0 ? 1u : a.count[ref];
+}
+
 PT_INLINE bool lbvh_leafish(const LbvhArgs &a, int32_t ref) {
-  return ref < 0 || a.last[ref] - a.first[ref] + 1u <= kLbvhLeafSize;
+  return lbvh_count(a, ref) <= kLbvhLeafSize;
+}
+
+// Write the triangles under `ref` (at most kLbvhLeafSize) as one leaf at
+// output positions out, out + 1, ...; the last one carries the leaf-end bit.
+PT_INLINE void lbvh_emit_leaf(const LbvhArgs &a, int32_t ref, uint32_t out, uint32_t scene) {
+  int32_t stack[kLbvhLeafSize];
+  uint32_t sp = 0, q = out;
+  const uint32_t n = lbvh_count(a, ref), t_base = a.tri_begin[scene];
+  stack[sp++] = ref;
+  while (sp > 0) {
+    const int32_t r = stack[--sp];
+    if (r >= 0) { // at most n - 1 inner nodes, each pushing two refs
+      if (sp + 2 > kLbvhLeafSize)
+        return; // unreachable for a subtree of at most kLbvhLeafSize leaves
+      stack[sp++] = a.right[r];
+      stack[sp++] = a.left[r];
+      continue;
+    }
+    const uint32_t t = a.vals_sorted[uint32_t(~r)];
+    const ShadeTri<float> &st = a.shade[t];
+    GpuTri o;
+    o.v0[0] = st.v0.x, o.v0[1] = st.v0.y, o.v0[2] = st.v0.z;
+    o.idx = (t - t_base) | (q + 1 == out + n ? kLbvhLeafEnd : 0u);
+    o.e1[0] = st.e1.x, o.e1[1] = st.e1.y, o.e1[2] = st.e1.z;
+    o.e2[0] = st.e2.x, o.e2[1] = st.e2.y, o.e2[2] = st.e2.z;
+    o.pad0 = o.pad1 = 0;
+    a.tris[q++] = o;
+  }
 }
 
 PT_INLINE float lbvh_area(const LbvhArgs &a, int32_t ref) {
@@ -231,8 +271,9 @@ PT_INLINE float lbvh_area(const LbvhArgs &a, int32_t ref) {
 }
 
 // Item i of a level: its node takes up to four descendants (largest-area
-// inner child opened first), quantises their padded boxes, marks the last
-// triangle of each leaf and queues its inner children in `next`.
+// inner child opened first), quantises their padded boxes, writes the
+// triangles of each leaf child at its part of the node's output range and
+// queues its inner children in `next` with theirs.
 template <class Ops>
 PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *next,
                                Ops &ops) {
@@ -262,6 +303,7 @@ PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *
   const uint32_t base = a.tri_begin[it.scene];
   float clo[4][3], chi[4][3];
   int32_t child[4];
+  uint32_t pos = it.tri_first; // the kids' triangles follow each other in kid order
   for (uint32_t k = 0; k < nk; ++k) {
     const int32_t ref = kids[k];
     const float *b = ref < 0 ? a.lbox + 6 * size_t(~ref) : a.ibox + 6 * size_t(ref);
@@ -270,16 +312,15 @@ PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *
       chi[k][c] = b[3 + c] + pad;
     }
     if (lbvh_leafish(a, ref)) {
-      const uint32_t first = ref < 0 ? uint32_t(~ref) : a.first[ref];
-      const uint32_t last = ref < 0 ? uint32_t(~ref) : a.last[ref];
-      child[k] = ~int32_t(first - base);
-      a.tris[last].idx |= 0x80000000u;
+      child[k] = ~int32_t(pos - base);
+      lbvh_emit_leaf(a, ref, pos, it.scene);
     } else {
       const uint32_t slot = ops.add(&a.counters[0], 1u);
       child[k] = int32_t(slot - it.root_slot);
       const uint32_t q = ops.add(&a.counters[1], 1u);
-      next[q] = LbvhRoot{ref, slot, it.scene, it.root_slot};
+      next[q] = LbvhRoot{ref, slot, it.scene, it.root_slot, pos};
     }
+    pos += lbvh_count(a, ref);
   }
   GpuNode node;
   if (!quantise_gpu_node(clo, chi, nk, child, &node))
@@ -307,13 +348,16 @@ struct LbvhBuffers {
 
 // Bytes of radix-sort temporary storage for n keys.
 hipError_t lbvh_sort_temp_bytes(uint32_t n, uint32_t n_scenes, size_t *bytes);
-// Build the 4-wide trees of every scene. roots (host array, one per scene
-// with triangles, slots 0..n_roots-1 in order) seed the first level.
+enum LbvhBuilder : int { kLbvhRadixTree = 1, kLbvhPloc = 2 };
+
+// Build the 4-wide trees of every scene with the given builder. roots (host
+// array, one per scene with triangles, slots 0..n_roots-1 in order) seed the
+// first level.
 // *levels receives the depth of the deepest tree + 1; returns
 // hipErrorInvalidValue if a tree needs more than kLbvhMaxLevels levels or a
 // node fails the quantisation check.
 hipError_t build_lbvh(const LbvhBuffers &b, const LbvhRoot *roots, uint32_t n_roots,
-                      hipStream_t s, uint32_t *levels);
+                      LbvhBuilder builder, hipStream_t s, uint32_t *levels);
 
 } // namespace gpu
 } // namespace pt

@@ -1,6 +1,7 @@
 // GPU LBVH build kernels (see lbvh.h for the algorithm and numbering).
 #include "lbvh.h"
 #include "radix_sort.h"
+#include "wave.h"
 
 #include <vector>
 
@@ -102,6 +103,145 @@ __global__ __launch_bounds__(kLeafBlock) void k_lbvh_leaves(LbvhArgs a) {
     lbvh_leaf(a, p, ops);
 }
 
+__global__ __launch_bounds__(256) void k_lbvh_leaf_boxes(LbvhArgs a) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p < a.n_tris)
+    lbvh_leaf_box(a, p);
+}
+
+// ---------------------------------------------------------------------------
+// PLOC, one workgroup per scene. Cluster c of a pass is two float4 records
+// (box lo + ref, box hi + triangle count) in Morton order. A pass finds each
+// cluster's nearest neighbour within kPlocRadius positions by the surface
+// area of the union box, merges mutual pairs into new internal nodes (at the
+// lower position; the higher one drops out) and compacts the survivors in
+// order into the other buffer. Candidate pairs are ordered by (area, lower
+// position, higher position): the least pair of a pass is mutual, so every
+// pass merges at least one pair and n - 1 passes are the most a scene of n
+// triangles can take (a pass cap guards against NaN boxes, where the areas
+// compare false: the build then reports failure and the batch falls back to
+// host trees). Internal nodes are numbered as they are created: the merge
+// that takes the cluster count from c to c - 1 makes local node c - 2, so
+// the last merge makes the root, node 0.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPlocBlock = 1024;
+constexpr uint32_t kPlocWaves = kPlocBlock / 64;
+
+__device__ __forceinline__ float ploc_area(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+  const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+  const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+  return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ __launch_bounds__(kPlocBlock) void k_ploc(LbvhArgs a) {
+  __shared__ uint32_t s_cnt[2][kPlocWaves];
+  const uint32_t scene = blockIdx.x;
+  const uint32_t tb = a.tri_begin[scene], n0 = a.tri_begin[scene + 1] - tb;
+  if (n0 < 2)
+    return; // no internal nodes (the whole block returns)
+  const uint32_t ib = a.int_begin[scene];
+  float4 *cur = reinterpret_cast<float4 *>(a.clusters[0]) + 2 * size_t(tb);
+  float4 *nxt = reinterpret_cast<float4 *>(a.clusters[1]) + 2 * size_t(tb);
+  int32_t *nn = a.nn + tb;
+  for (uint32_t p = threadIdx.x; p < n0; p += kPlocBlock) {
+    const float *lb = a.lbox + 6 * size_t(tb + p);
+    cur[2 * p] = make_float4(lb[0], lb[1], lb[2], __int_as_float(~int32_t(tb + p)));
+    cur[2 * p + 1] = make_float4(lb[3], lb[4], lb[5], __uint_as_float(1u));
+  }
+  __syncthreads();
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t n = n0, passes = 0;
+  while (n > 1) {
+    if (++passes >= n0) { // n0 - 1 passes always suffice (see above)
+      if (threadIdx.x == 0)
+        atomicOr(&a.counters[2], 2u);
+      return;
+    }
+    // nearest neighbours
+    for (uint32_t i = threadIdx.x; i < n; i += kPlocBlock) {
+      const float4 lo = cur[2 * i], hi = cur[2 * i + 1];
+      float best = INFINITY;
+      int32_t bj = -1;
+      const uint32_t j0 = i > uint32_t(kPlocRadius) ? i - uint32_t(kPlocRadius) : 0u;
+      const uint32_t j1 = min(n - 1, i + uint32_t(kPlocRadius));
+      for (uint32_t j = j0; j <= j1; ++j) {
+        if (j == i)
+          continue;
+        const float ar = ploc_area(lo, hi, cur[2 * j], cur[2 * j + 1]);
+        // pairs with equal areas: (min, max) of the positions decides; for
+        // this i, a candidate below it orders before one above it, and
+        // among the lower ones the smaller, among the upper the smaller
+        bool better = ar < best;
+        if (!better && ar == best && bj >= 0) {
+          const uint32_t b = uint32_t(bj);
+          const uint32_t jmin = min(i, j), jmax = max(i, j);
+          const uint32_t bmin = min(i, b), bmax = max(i, b);
+          better = jmin < bmin || (jmin == bmin && jmax < bmax);
+        }
+        if (better) {
+          best = ar;
+          bj = int32_t(j);
+        }
+      }
+      nn[i] = bj;
+    }
+    __syncthreads();
+    // merge mutual pairs and compact, 1024 positions at a time
+    uint32_t n_keep = 0, n_merge = 0;
+    for (uint32_t base = 0; base < n; base += kPlocBlock) {
+      const uint32_t i = base + threadIdx.x;
+      const int32_t j = i < n ? nn[i] : -1;
+      const bool mutual = j >= 0 && nn[j] == int32_t(i);
+      const bool merge = mutual && int32_t(i) < j;
+      const bool keep = i < n && !(mutual && int32_t(i) > j);
+      uint32_t wk, wm;
+      const uint32_t pk = ballot_prefix(keep, &wk), pm = ballot_prefix(merge, &wm);
+      if (lane_id() == 0) {
+        s_cnt[0][wave] = wk;
+        s_cnt[1][wave] = wm;
+      }
+      __syncthreads();
+      uint32_t bk = 0, bm = 0, tk = 0, tm = 0;
+      for (uint32_t w = 0; w < kPlocWaves; ++w) {
+        const uint32_t x = s_cnt[0][w], y = s_cnt[1][w];
+        bk += w < wave ? x : 0u;
+        bm += w < wave ? y : 0u;
+        tk += x;
+        tm += y;
+      }
+      if (keep) {
+        float4 lo = cur[2 * i], hi = cur[2 * i + 1];
+        if (merge) {
+          const float4 lo2 = cur[2 * j], hi2 = cur[2 * j + 1];
+          const uint32_t g = ib + (n - 2u - (n_merge + bm + pm));
+          const uint32_t cnt = __float_as_uint(hi.w) + __float_as_uint(hi2.w);
+          a.left[g] = __float_as_int(lo.w);
+          a.right[g] = __float_as_int(lo2.w);
+          a.count[g] = cnt;
+          lo = make_float4(fminf(lo.x, lo2.x), fminf(lo.y, lo2.y), fminf(lo.z, lo2.z),
+                           __int_as_float(int32_t(g)));
+          hi = make_float4(fmaxf(hi.x, hi2.x), fmaxf(hi.y, hi2.y), fmaxf(hi.z, hi2.z),
+                           __uint_as_float(cnt));
+          float *ob = a.ibox + 6 * size_t(g);
+          ob[0] = lo.x, ob[1] = lo.y, ob[2] = lo.z;
+          ob[3] = hi.x, ob[4] = hi.y, ob[5] = hi.z;
+        }
+        const uint32_t o = n_keep + bk + pk;
+        nxt[2 * o] = lo;
+        nxt[2 * o + 1] = hi;
+      }
+      n_keep += tk;
+      n_merge += tm;
+      __syncthreads(); // s_cnt is rewritten by the next chunk; nxt complete
+    }
+    n = n_keep;
+    float4 *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_lbvh_level(LbvhArgs a, const LbvhRoot *items,
                                                     uint32_t n_items, LbvhRoot *next) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -125,7 +265,7 @@ hipError_t lbvh_sort_temp_bytes(uint32_t n, uint32_t n_scenes, size_t *bytes) {
 }
 
 hipError_t build_lbvh(const LbvhBuffers &b, const LbvhRoot *roots, uint32_t n_roots,
-                      hipStream_t s, uint32_t *levels) {
+                      LbvhBuilder builder, hipStream_t s, uint32_t *levels) {
   const LbvhArgs &a = b.a;
   *levels = 0;
   if (a.n_tris == 0 || n_roots == 0)
@@ -142,9 +282,14 @@ hipError_t build_lbvh(const LbvhBuffers &b, const LbvhRoot *roots, uint32_t n_ro
                             key_end_bit(a.n_scenes), b.sort_temp, b.sort_temp_bytes, s)) !=
       hipSuccess)
     return e;
-  if (a.n_internal)
-    hipLaunchKernelGGL(k_lbvh_karras, dim3((a.n_internal + 255) / 256), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_lbvh_leaves, dim3(a.n_scenes), dim3(kLeafBlock), 0, s, a);
+  if (builder == kLbvhPloc) {
+    hipLaunchKernelGGL(k_lbvh_leaf_boxes, dim3((a.n_tris + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ploc, dim3(a.n_scenes), dim3(kPlocBlock), 0, s, a);
+  } else {
+    if (a.n_internal)
+      hipLaunchKernelGGL(k_lbvh_karras, dim3((a.n_internal + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_lbvh_leaves, dim3(a.n_scenes), dim3(kLeafBlock), 0, s, a);
+  }
   if ((e = hipGetLastError()) != hipSuccess)
     return e;
   // levels: roots take slots 0..n_roots-1, the others are allocated in order

@@ -246,7 +246,7 @@ int pack(pt_renderer &r, const pt_batch_item *items, uint32_t n, uint32_t spp, b
   return PT_OK;
 }
 
-// Trees of every scene of the batch on the GPU (LBVH, gpu/lbvh.h). Returns
+// Trees of every scene of the batch on the GPU (gpu/lbvh.h, r.gpu_builder). Returns
 // PT_OK with *fallback set if some tree is too deep for the traversal stack
 // (or fails the quantisation check): the caller then builds on the host.
 int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
@@ -263,7 +263,7 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
       return set_error(PT_ERR_INVALID, "batch layout does not match the GPU tree build");
     if (nt)
       roots.push_back({nt >= 2 ? int32_t(int_begin[i]) : ~int32_t(tri_begin[i]),
-                       desc[i].node_off, i, desc[i].node_off});
+                       desc[i].node_off, i, desc[i].node_off, tri_begin[i]});
   }
   const uint32_t nt = tri_begin[n], ni = int_begin[n];
   *levels = 0;
@@ -279,8 +279,13 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   HIP_TRY(r.lb_vals_sorted.ensure(nt));
   HIP_TRY(r.lb_left.ensure(std::max(one, size_t(ni))));
   HIP_TRY(r.lb_right.ensure(std::max(one, size_t(ni))));
-  HIP_TRY(r.lb_first.ensure(std::max(one, size_t(ni))));
-  HIP_TRY(r.lb_last.ensure(std::max(one, size_t(ni))));
+  HIP_TRY(r.lb_count.ensure(std::max(one, size_t(ni))));
+  const bool ploc = r.gpu_builder == gpu::kLbvhPloc;
+  if (ploc) {
+    HIP_TRY(r.lb_clusters[0].ensure(8 * size_t(nt)));
+    HIP_TRY(r.lb_clusters[1].ensure(8 * size_t(nt)));
+    HIP_TRY(r.lb_nn.ensure(nt));
+  }
   HIP_TRY(r.lb_parent_int.ensure(std::max(one, size_t(ni))));
   HIP_TRY(r.lb_visits.ensure(std::max(one, size_t(ni))));
   HIP_TRY(r.lb_ibox.ensure(std::max(one, size_t(ni)) * 6));
@@ -311,8 +316,10 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   a.vals_sorted = r.lb_vals_sorted.ptr;
   a.left = r.lb_left.ptr;
   a.right = r.lb_right.ptr;
-  a.first = r.lb_first.ptr;
-  a.last = r.lb_last.ptr;
+  a.count = r.lb_count.ptr;
+  a.clusters[0] = r.lb_clusters[0].ptr;
+  a.clusters[1] = r.lb_clusters[1].ptr;
+  a.nn = r.lb_nn.ptr;
   a.parent_int = r.lb_parent_int.ptr;
   a.parent_leaf = r.lb_parent_leaf.ptr;
   a.ibox = r.lb_ibox.ptr;
@@ -325,7 +332,9 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   b.frontier[1] = r.lb_frontier[1].ptr;
   b.sort_temp = r.lb_sort_temp.ptr;
   b.sort_temp_bytes = temp;
-  const hipError_t e = gpu::build_lbvh(b, roots.data(), uint32_t(roots.size()), s, levels);
+  const hipError_t e =
+      gpu::build_lbvh(b, roots.data(), uint32_t(roots.size()),
+                      ploc ? gpu::kLbvhPloc : gpu::kLbvhRadixTree, s, levels);
   if (e == hipErrorInvalidValue) {
     *fallback = true;
     return PT_OK;
@@ -351,7 +360,7 @@ int ensure_batch_scenes(pt_renderer &r, const pt_batch_item *items, uint32_t n, 
   std::vector<uint64_t> key;
   key.reserve(3 * size_t(n) + 2);
   key.push_back(spp);
-  key.push_back(gpu_bvh ? 1u : 0u);
+  key.push_back(gpu_bvh ? uint64_t(r.gpu_builder) : 0u);
   for (uint32_t i = 0; i < n; ++i) {
     key.push_back(items[i].scene->generation);
     key.push_back(items[i].scene->mat_generation);

@@ -262,7 +262,8 @@ struct pt_renderer {
   bool mega_split = true;
   uint32_t iters_per_sync = 1;
   bool use_graphs = false;
-  bool gpu_bvh = false; // batched renders build their trees on the GPU (LBVH)
+  bool gpu_bvh = false; // batched renders build their trees on the GPU
+  int gpu_builder = pt::gpu::kLbvhPloc; // which GPU builder (gpu/lbvh.h LbvhBuilder)
   double last_gpu_bvh_ms = 0.0;     // host wall time of the last GPU tree build
   uint32_t last_gpu_bvh_levels = 0; // levels of its deepest 4-wide tree
   // phases of the last batch that was not resident (host wall ms): scene
@@ -273,9 +274,10 @@ struct pt_renderer {
   pt::host::DeviceBuf<uint32_t> adj_overflow;
   pt::host::DeviceBuf<float> lb_scene_box, lb_ibox, lb_lbox;
   pt::host::DeviceBuf<unsigned long long> lb_keys, lb_keys_sorted;
-  pt::host::DeviceBuf<uint32_t> lb_vals, lb_vals_sorted, lb_first, lb_last, lb_visits,
-      lb_counters, lb_tri_begin, lb_int_begin;
-  pt::host::DeviceBuf<int32_t> lb_left, lb_right, lb_parent_int, lb_parent_leaf;
+  pt::host::DeviceBuf<uint32_t> lb_vals, lb_vals_sorted, lb_count, lb_visits, lb_counters,
+      lb_tri_begin, lb_int_begin;
+  pt::host::DeviceBuf<int32_t> lb_left, lb_right, lb_parent_int, lb_parent_leaf, lb_nn;
+  pt::host::DeviceBuf<float> lb_clusters[2];
   pt::host::DeviceBuf<pt::gpu::LbvhRoot> lb_frontier[2];
   pt::host::DeviceBuf<char> lb_sort_temp;
 

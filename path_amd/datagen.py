@@ -128,10 +128,16 @@ def set_pipelines(renderer: Renderer, n: int) -> None:
     L.check(L.lib().pt_set_pipelines(renderer.handle, n))
 
 
-def set_bvh_builder(renderer: Renderer, gpu: bool) -> None:
-    """Trees of render_batch: GPU LBVH build of the whole batch (True) or the
-    host SAH builder, cached per scene (False, the default)."""
-    L.check(L.lib().pt_set_bvh_builder(renderer.handle, 1 if gpu else 0))
+BVH_BUILDERS = {False: 0, "host": 0, "lbvh": 1, "ploc": 2, True: 2}
+
+
+def set_bvh_builder(renderer: Renderer, gpu) -> None:
+    """Trees of render_batch: built on the GPU for the whole batch (True or
+    "ploc": locally-ordered clustering; "lbvh": the binary radix tree) or by
+    the host SAH builder, cached per scene (False or "host", the default)."""
+    if gpu not in BVH_BUILDERS:
+        raise ValueError(f"unknown BVH builder {gpu!r}")
+    L.check(L.lib().pt_set_bvh_builder(renderer.handle, BVH_BUILDERS[gpu]))
 
 
 def last_bvh_build(renderer: Renderer) -> tuple:

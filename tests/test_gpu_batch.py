@@ -52,16 +52,19 @@ def test_batch_small_pool_and_graphs_bit_exact(renderer, gen_scenes):
     set_launch_params(renderer, 8, 1, False)
 
 
-def test_gpu_tree_build_bit_exact(renderer, gen_scenes):
-    """Trees built on the GPU (LBVH) render the same images and G-buffers as
-    the host SAH trees: closest hits do not depend on the tree."""
+@pytest.mark.parametrize("builder", ["ploc", "lbvh"])
+def test_gpu_tree_build_bit_exact(renderer, gen_scenes, builder):
+    """Trees built on the GPU (locally-ordered clustering or the binary radix
+    tree) render the same images and G-buffers as the host SAH trees: closest
+    hits do not depend on the tree, and every triangle is in exactly one
+    leaf."""
     from path_amd.datagen import (datagen_settings, generate_scene, last_bvh_build,
                                   render_batch, set_bvh_builder, set_launch_params)
     set_launch_params(renderer, 6, 1, False)
     st = datagen_settings()
     scenes = gen_scenes + [generate_scene(s) for s in (40, 41, 42)]
     host, host_g, _ = render_batch(renderer, scenes, SPP, st)
-    set_bvh_builder(renderer, True)
+    set_bvh_builder(renderer, builder)
     try:
         gpu, gpu_g, _ = render_batch(renderer, scenes, SPP, st)
         ms, levels = last_bvh_build(renderer)

@@ -114,7 +114,7 @@ def test_batch_with_an_empty_scene(renderer):
     scenes = [generate_scene(3), GeneratedScene(Scene(), _camera(), 32, -1), generate_scene(4)]
     singles = [renderer.render(g.scene, g.film_to_world, g.resolution, g.resolution, 4,
                                settings=st)[0] for g in scenes]
-    for gpu_tree in (False, True):
+    for gpu_tree in ("host", "ploc", "lbvh"):
         set_bvh_builder(renderer, gpu_tree)
         try:
             images, gbufs, _ = render_batch(renderer, scenes, 4, st)
