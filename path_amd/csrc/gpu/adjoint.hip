@@ -9,9 +9,12 @@
 //     once (single-lobe BSDFs: sample weight and NEE evaluation are linear in
 //     the albedo color), so it adds  dI * (L - radiance so far) / albedo[m]
 //     to dAlbedo[m] (channels with zero albedo are skipped).
-// Nothing is stored per vertex. Gradients accumulate in per-thread LDS slots
-// (no atomics), are summed per block in thread order and finally over blocks
-// in FP64, so the result is deterministic.
+// Nothing is stored per vertex. A block is one wave; its gradients accumulate
+// in one LDS row of 6 x n_mats floats with LDS float atomics (ds_add_f32;
+// all of a row's updates come from its one wave, in program order), and the
+// rows are summed over blocks in FP64. A row per lane instead (no atomics)
+// took 6 x n_mats x 65 floats per wave, 14 KB for 9 materials, which held
+// the kernel to 2 waves per SIMD; the row per wave leaves the VGPR limit (4).
 //
 // Single pass (paths of at most kAdjointMaxVertices vertices, e.g. NIters):
 // dI (L - R_k) / albedo = dI L / albedo - dI R_k / albedo, so one replay adds
@@ -41,16 +44,19 @@ __device__ __forceinline__ R adjoint_rng(const RngParams &rp, uint32_t location,
 
 enum ReplayMode { kRadiance, kBackprop, kSinglePass };
 
+// Add v to LDS gradient slot `slot` (shared by the lanes of the wave).
+__device__ __forceinline__ void acc_add(float *slot, float v) { atomicAdd(slot, v); }
+
 // One replay of a sample. kRadiance: returns the radiance. kBackprop:
-// accumulates into acc[param * stride] (this thread's slots) given the
-// sample's radiance. kSinglePass: accumulates without knowing the radiance
-// in advance; vlist (this thread's LDS column, stride kAdjointBlock) keeps
-// the vertex materials.
+// accumulates into acc[param] (the wave's row) given the sample's radiance.
+// kSinglePass: accumulates without knowing the radiance in advance; vlist
+// (this thread's LDS column, stride kAdjointBlock) keeps the vertex
+// materials.
 template <class R, int kMode>
 __device__ __forceinline__ V3<float>
 replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
-              V3<float> radiance, V3<float> d_i, float *acc, uint32_t stride,
-              LdsStack &stack, uint8_t *vlist = nullptr) {
+              V3<float> radiance, V3<float> d_i, float *acc, LdsStack &stack,
+              uint8_t *vlist = nullptr) {
   uint32_t n_vertices = 0;
   const uint32_t location = px + py * p.x_dim;
   R rng = adjoint_rng<R>(p.rng, location, sample);
@@ -81,29 +87,29 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
                                              no_vertex, tape_ptr);
     if constexpr (kMode != kRadiance) {
       for (uint32_t j = 0; j < tape.n_emit; ++j) {
-        float *e = acc + (n3 + 3 * tape.emit_mat[j]) * stride;
-        e[0] += d_i.x * tape.emit_mult[j].x;
-        e[stride] += d_i.y * tape.emit_mult[j].y;
-        e[2 * stride] += d_i.z * tape.emit_mult[j].z;
+        float *e = acc + n3 + 3 * tape.emit_mat[j];
+        acc_add(e, d_i.x * tape.emit_mult[j].x);
+        acc_add(e + 1, d_i.y * tape.emit_mult[j].y);
+        acc_add(e + 2, d_i.z * tape.emit_mult[j].z);
       }
       if (tape.vertex_mat != kNoMaterial) {
         V3<float> rho = material_albedo(p.scene.view.materials[tape.vertex_mat]);
-        float *a = acc + 3 * tape.vertex_mat * stride;
+        float *a = acc + 3 * tape.vertex_mat;
         if constexpr (kMode == kBackprop) {
           V3<float> rest = radiance - ns.total;
           if (rho.x != 0.f)
-            a[0] += d_i.x * rest.x / rho.x;
+            acc_add(a, d_i.x * rest.x / rho.x);
           if (rho.y != 0.f)
-            a[stride] += d_i.y * rest.y / rho.y;
+            acc_add(a + 1, d_i.y * rest.y / rho.y);
           if (rho.z != 0.f)
-            a[2 * stride] += d_i.z * rest.z / rho.z;
+            acc_add(a + 2, d_i.z * rest.z / rho.z);
         } else {
           if (rho.x != 0.f)
-            a[0] -= d_i.x * ns.total.x / rho.x;
+            acc_add(a, -(d_i.x * ns.total.x / rho.x));
           if (rho.y != 0.f)
-            a[stride] -= d_i.y * ns.total.y / rho.y;
+            acc_add(a + 1, -(d_i.y * ns.total.y / rho.y));
           if (rho.z != 0.f)
-            a[2 * stride] -= d_i.z * ns.total.z / rho.z;
+            acc_add(a + 2, -(d_i.z * ns.total.z / rho.z));
           if (n_vertices < kAdjointMaxVertices)
             vlist[n_vertices * kAdjointBlock] = uint8_t(tape.vertex_mat);
           else
@@ -119,13 +125,13 @@ replay_sample(const AdjointParams &p, uint32_t px, uint32_t py, uint32_t sample,
         for (uint32_t v = 0; v < nv; ++v) {
           const uint32_t m = vlist[v * kAdjointBlock];
           const V3<float> rho = material_albedo(p.scene.view.materials[m]);
-          float *a = acc + 3 * m * stride;
+          float *a = acc + 3 * m;
           if (rho.x != 0.f)
-            a[0] += d_i.x * l.x / rho.x;
+            acc_add(a, d_i.x * l.x / rho.x);
           if (rho.y != 0.f)
-            a[stride] += d_i.y * l.y / rho.y;
+            acc_add(a + 1, d_i.y * l.y / rho.y);
           if (rho.z != 0.f)
-            a[2 * stride] += d_i.z * l.z / rho.z;
+            acc_add(a + 2, d_i.z * l.z / rho.z);
         }
       }
       return ns.total;
@@ -143,11 +149,10 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
   __shared__ int stack_lds[kLdsStackInts * (kAdjointBlock / 64)];
   int spill[kOverflowDepth];
   LdsStack stack{lds_stack_column(stack_lds), spill, 1};
-  const uint32_t stride = kAdjointBlock + 1; // odd stride: conflict-free sums
   const uint32_t n_params = 6 * p.n_mats;
-  float *acc = acc_all + threadIdx.x;
-  for (uint32_t i = 0; i < n_params; ++i)
-    acc[i * stride] = 0.f;
+  for (uint32_t i = threadIdx.x; i < n_params; i += kAdjointBlock)
+    acc_all[i] = 0.f;
+  __syncthreads();
   const uint32_t t = blockIdx.x * kAdjointBlock + threadIdx.x;
   if (t < p.n_threads) {
     const uint32_t pixel = t / p.threads_per_pixel;
@@ -161,28 +166,23 @@ __global__ __launch_bounds__(kAdjointBlock) void k_adjoint(AdjointParams p) {
     for (uint32_t sample = s0; sample < s1; ++sample) {
       V3<float> zero = {0.f, 0.f, 0.f};
       if constexpr (kSingle) {
-        replay_sample<R, kSinglePass>(p, px, py, sample, zero, d_i, acc, stride, stack,
+        replay_sample<R, kSinglePass>(p, px, py, sample, zero, d_i, acc_all, stack,
                                       vlist_all + threadIdx.x);
       } else {
         V3<float> radiance =
-            replay_sample<R, kRadiance>(p, px, py, sample, zero, zero, nullptr, stride, stack);
-        replay_sample<R, kBackprop>(p, px, py, sample, radiance, d_i, acc, stride, stack);
+            replay_sample<R, kRadiance>(p, px, py, sample, zero, zero, nullptr, stack);
+        replay_sample<R, kBackprop>(p, px, py, sample, radiance, d_i, acc_all, stack);
       }
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_params; i += kAdjointBlock) {
-    float sum = 0.f;
-    const float *row = acc_all + i * stride;
-    for (uint32_t j = 0; j < kAdjointBlock; ++j)
-      sum += row[j];
-    p.partials[size_t(blockIdx.x) * n_params + i] = sum;
-  }
+  for (uint32_t i = threadIdx.x; i < n_params; i += kAdjointBlock)
+    p.partials[size_t(blockIdx.x) * n_params + i] = acc_all[i];
 }
 
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s) {
   const uint32_t blocks = (p.n_threads + kAdjointBlock - 1) / kAdjointBlock;
-  const size_t lds = size_t(6) * p.n_mats * (kAdjointBlock + 1) * sizeof(float);
+  const size_t lds = size_t(6) * p.n_mats * sizeof(float);
   const bool sobol = p.rng.type == kRngSobel;
   if (p.single_pass) {
     if (sobol)

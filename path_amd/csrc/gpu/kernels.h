@@ -79,7 +79,7 @@ struct MegaBatchParams {
 
 // Path-replay adjoint of a SquareImage render: gradients of sum(dI * I) with
 // respect to every material's albedo color (material_albedo) and emission.
-constexpr uint32_t kMaxAdjointMaterials = 40; // LDS: 6 * 40 * 65 floats
+constexpr uint32_t kMaxAdjointMaterials = 40; // LDS: 6 * 40 floats per wave
 constexpr uint32_t kAdjointBlock = 64;
 
 struct AdjointParams {
