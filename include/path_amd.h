@@ -382,8 +382,9 @@ int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,
  * and cached with the scene. PT_BVH_GPU_LBVH / PT_BVH_GPU_PLOC: all trees of
  * a batch are built on the GPU in one pass over Morton-sorted triangles (the
  * binary radix tree, or locally-ordered clustering by surface area, whose
- * trees trace faster), collapsed to the same 4-wide quantised nodes; a batch
- * with a tree deeper than the traversal stack allows falls back to the host
+ * trees trace faster; a batch with a scene above 2^18 triangles uses the
+ * radix tree), collapsed to the same 4-wide quantised nodes; a batch with a
+ * tree deeper than the traversal stack allows falls back to the host
  * builder. Images do not depend on the tree. */
 enum { PT_BVH_HOST_SAH = 0, PT_BVH_GPU_LBVH = 1, PT_BVH_GPU_PLOC = 2 };
 /* Independent wavefront pipelines a Streaming render splits its path pool

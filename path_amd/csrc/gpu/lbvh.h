@@ -41,6 +41,10 @@ namespace gpu {
 constexpr uint32_t kLbvhLeafSize = 2;
 constexpr uint32_t kLbvhLeafEnd = 0x80000000u; // last triangle of a leaf (gpu/trace.h)
 constexpr int kPlocRadius = 8;
+// PLOC runs a scene in one workgroup; a batch with a larger scene is built
+// with the radix tree (one thread per node), which spreads a big scene over
+// the whole GPU.
+constexpr uint32_t kPlocMaxSceneTris = 1u << 18;
 // traversal stack entries a 4-wide tree of this many levels can need
 constexpr uint32_t kLbvhMaxLevels = 21; // 3 * (20 + 1) = 63 <= 64
 

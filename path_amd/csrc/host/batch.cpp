@@ -280,7 +280,10 @@ int build_gpu_trees(pt_renderer &r, const pt_batch_item *items, uint32_t n,
   HIP_TRY(r.lb_left.ensure(std::max(one, size_t(ni))));
   HIP_TRY(r.lb_right.ensure(std::max(one, size_t(ni))));
   HIP_TRY(r.lb_count.ensure(std::max(one, size_t(ni))));
-  const bool ploc = r.gpu_builder == gpu::kLbvhPloc;
+  uint32_t max_tris = 0;
+  for (uint32_t i = 0; i < n; ++i)
+    max_tris = std::max(max_tris, tri_begin[i + 1] - tri_begin[i]);
+  const bool ploc = r.gpu_builder == gpu::kLbvhPloc && max_tris <= gpu::kPlocMaxSceneTris;
   if (ploc) {
     HIP_TRY(r.lb_clusters[0].ensure(8 * size_t(nt)));
     HIP_TRY(r.lb_clusters[1].ensure(8 * size_t(nt)));

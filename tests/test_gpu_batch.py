@@ -254,3 +254,57 @@ def test_ray_batch_rejects_mixed_specs(renderer, gen_scenes):
     items[2] = (items[2][0], items[2][1], None)
     with pytest.raises(PathError):
         renderer.render_rays_batch(items, 4, 2)
+
+
+def _mesh_scene(tris, emitter=True):
+    """Scene from an (n, 9) triangle array (one diffuse material) plus, when
+    asked, a quad emitter above the origin; with a camera looking at it."""
+    from path_amd.datagen import GeneratedScene
+    from path_amd.renderer import Scene, camera_from_look, material
+    sc = Scene()
+    tris = np.asarray(tris, np.float32).reshape(-1, 9)[:, [0, 1, 2, 6, 7, 8, 3, 4, 5]]  # face +y
+    mats = [material("Diffuse", diffuse=(0.6, 0.5, 0.4))]
+    tm = np.zeros(len(tris), np.uint32)
+    if emitter:
+        q = np.array([[-0.3, 1.5, -0.3, 0.3, 1.5, -0.3, 0.3, 1.5, 0.3],
+                      [-0.3, 1.5, -0.3, 0.3, 1.5, 0.3, -0.3, 1.5, 0.3]], np.float32)
+        tris = np.concatenate([tris, q])
+        mats.append(material("Diffuse", diffuse=(0.5, 0.5, 0.5), emission=(10.0, 10.0, 10.0)))
+        tm = np.concatenate([tm, np.ones(2, np.uint32)])
+    sc.add_mesh(tris, tm, mats)
+    cam = camera_from_look((0.0, -0.4, -1.0), (0.0, 1.0, 0.0), (0.0, 1.2, 3.0), 50.0, 1.0)
+    return GeneratedScene(sc, cam, 32, -1)
+
+
+def test_ploc_degenerate_scenes(renderer):
+    """GPU trees of scenes that stress the clustering's tie rule and small
+    cases (1, 2 and 3 triangles, 300 coincident triangles whose candidate
+    areas all tie, a regular grid of equal squares, a batch mixing them)
+    render the same images as host SAH trees with both GPU builders."""
+    from path_amd.datagen import (datagen_settings, render_batch, set_bvh_builder,
+                                  set_launch_params)
+    set_launch_params(renderer, 6, 1, False)
+    st = datagen_settings()
+    tri = np.array([-1, 0, -1, 1, 0, -1, 0, 0, 1], np.float32)
+    grid = []
+    for i in range(12):
+        for j in range(12):
+            x, z = -1.2 + 0.2 * i, -1.2 + 0.2 * j
+            grid += [[x, 0, z, x + 0.2, 0, z, x + 0.2, 0, z + 0.2],
+                     [x, 0, z, x + 0.2, 0, z + 0.2, x, 0, z + 0.2]]
+    scenes = [_mesh_scene(tri[None], emitter=False),  # one triangle, no light
+              _mesh_scene(tri[None]),                  # 3 with the emitter
+              _mesh_scene(np.stack([tri, tri + np.float32(0.01)]), emitter=False),
+              _mesh_scene(np.repeat(tri[None], 300, axis=0)),
+              _mesh_scene(np.array(grid, np.float32))]
+    host, host_g, _ = render_batch(renderer, scenes, 8, st)
+    assert any(h.max() > 0 for h in host)
+    for builder in ("ploc", "lbvh"):
+        set_bvh_builder(renderer, builder)
+        try:
+            gpu, gpu_g, _ = render_batch(renderer, scenes, 8, st)
+        finally:
+            set_bvh_builder(renderer, False)
+        for k, (a, b, ga, gb) in enumerate(zip(host, gpu, host_g, gpu_g)):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (builder, k)
+            assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), (builder, k)
