@@ -403,8 +403,11 @@ constexpr uint32_t kRefillMinIdle = 16;
 // and the 82k-triangle workload from 680 to 755-760 Msamples/s. Vote weights
 // (inner if w * inner lanes >= 4 * leaf lanes) 2 / 3 / 4 / 6 / 8 / 16: 746 /
 // 760 / 754 / 749 / 728 / 682; one triangle per leaf step instead of the
-// whole leaf: 748.
-constexpr uint32_t kTraceVote = 3;
+// whole leaf: 748. With both queues in one launch and balanced shares,
+// 2 / 3 / 4 / 5 / 6: 881-883 / 895-898 / 902-903 / 899-900 / 898-899 (the
+// batched scenes: no difference between 3 and 4); refill thresholds 8 / 16 /
+// 24 / 32: 884 / 895-898 / 899-900 / 887.
+constexpr uint32_t kTraceVote = 4;
 // Both queues of an iteration in one launch (the extension segments, then
 // the shadow segments, concatenated); a lane's entry decides its ray kind (a shadow ray carries its target window and stops at the first
 // occluder). One launch instead of an extension launch followed by a shadow
