@@ -67,7 +67,7 @@ struct LbvhArgs {
   int32_t *nn;                  // [n_tris] PLOC nearest neighbours
   GpuNode *nodes;               // output, global slots
   GpuTri *tris;                 // output, sorted positions
-  uint32_t *counters;           // [0] nodes allocated, [1] next frontier size, [2] error
+  uint32_t *counters;           // [0] node slots handed out, [2] error bits
 };
 
 PT_INLINE uint32_t lbvh_find(const uint32_t *begin, uint32_t n, uint32_t x) {
@@ -279,14 +279,10 @@ PT_INLINE float lbvh_area(const LbvhArgs &a, int32_t ref) {
   return dx * dy + dy * dz + dz * dx;
 }
 
-// Item i of a level: its node takes up to four descendants (largest-area
-// inner child opened first), quantises their padded boxes, writes the
-// triangles of each leaf child at its part of the node's output range and
-// queues its inner children in `next` with theirs.
-template <class Ops>
-PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *next,
-                               Ops &ops) {
-  int32_t kids[4];
+// The up-to-four descendants a level item's node takes (largest-area inner
+// child opened first); returns how many of them are inner (not leaves).
+PT_INLINE uint32_t lbvh_level_kids(const LbvhArgs &a, const LbvhRoot &it, int32_t kids[4],
+                                   uint32_t *nk_out) {
   uint32_t nk = 0;
   if (lbvh_leafish(a, it.ref)) {
     kids[nk++] = it.ref;
@@ -308,11 +304,27 @@ PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *
       kids[nk++] = a.right[open];
     }
   }
+  uint32_t n_inner = 0;
+  for (uint32_t k = 0; k < nk; ++k)
+    n_inner += lbvh_leafish(a, kids[k]) ? 0u : 1u;
+  *nk_out = nk;
+  return n_inner;
+}
+
+// The node of a level item: quantises its children's padded boxes, writes
+// the triangles of each leaf child at its part of the node's output range
+// and gives its inner children the node slots first_slot, first_slot + 1, ...
+// (queued in `next` at slot - next_base, with their output ranges). Returns
+// false if the quantised node is not conservative.
+PT_INLINE bool lbvh_level_node(const LbvhArgs &a, const LbvhRoot &it, const int32_t kids[4],
+                               uint32_t nk, uint32_t first_slot, uint32_t next_base,
+                               LbvhRoot *next) {
   const float pad = a.scene_box[8 * size_t(it.scene) + 6];
   const uint32_t base = a.tri_begin[it.scene];
   float clo[4][3], chi[4][3];
   int32_t child[4];
   uint32_t pos = it.tri_first; // the kids' triangles follow each other in kid order
+  uint32_t slot = first_slot;
   for (uint32_t k = 0; k < nk; ++k) {
     const int32_t ref = kids[k];
     const float *b = ref < 0 ? a.lbox + 6 * size_t(~ref) : a.ibox + 6 * size_t(ref);
@@ -324,17 +336,16 @@ PT_INLINE void lbvh_level_item(const LbvhArgs &a, const LbvhRoot &it, LbvhRoot *
       child[k] = ~int32_t(pos - base);
       lbvh_emit_leaf(a, ref, pos, it.scene);
     } else {
-      const uint32_t slot = ops.add(&a.counters[0], 1u);
       child[k] = int32_t(slot - it.root_slot);
-      const uint32_t q = ops.add(&a.counters[1], 1u);
-      next[q] = LbvhRoot{ref, slot, it.scene, it.root_slot, pos};
+      next[slot - next_base] = LbvhRoot{ref, slot, it.scene, it.root_slot, pos};
+      ++slot;
     }
     pos += lbvh_count(a, ref);
   }
   GpuNode node;
-  if (!quantise_gpu_node(clo, chi, nk, child, &node))
-    ops.bit_or(&a.counters[2], 1u);
+  const bool ok = quantise_gpu_node(clo, chi, nk, child, &node);
   a.nodes[it.slot] = node;
+  return ok;
 }
 
 } // namespace gpu
