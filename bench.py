@@ -175,6 +175,8 @@ def bench_datagen(a, cfg):
     if not a.no_autotune:
         tuned = autotune(r, gens[: min(n, 128)], spp, settings)
         _progress(f"datagen: autotuned {tuned}")
+    elif a.trace_blocks_per_cu or a.iters_per_sync:
+        set_launch_params(r, a.trace_blocks_per_cu, a.iters_per_sync or 1, bool(a.iters_per_sync))
     for _ in range(max(a.warmup - 1, 0)):
         render_batch(r, gens, spp, settings)
     # Timed steps: every step renders a batch of freshly generated scene

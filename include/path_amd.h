@@ -371,7 +371,7 @@ int pt_render_rays_batch(pt_renderer *r, const pt_ray_batch_item *items, uint32_
  * each, on their own streams driven by their own host threads, so one's
  * shading overlaps another's traversal.
  * trace_blocks_per_cu: traversal grid per pipeline = CUs x this x 256
- * threads (default 3 for a render, 4 for a batch; 0 restores the
+ * threads (default 3 for a render, 6 for a batch; 0 restores the
  * defaults); iters_per_sync: iterations enqueued between host
  * checks; use_graphs: replay those iterations as one captured hipGraph. */
 int pt_set_launch_params(pt_renderer *r, uint32_t trace_blocks_per_cu,

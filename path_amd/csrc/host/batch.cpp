@@ -833,7 +833,7 @@ int render_rays_batch(pt_renderer &r, const pt_ray_batch_item *items, uint32_t n
 
 int autotune_batch(pt_renderer &r, const pt_batch_item *items, uint32_t n,
                    uint32_t spp, pt_settings &settings, uint32_t out[5]) {
-  const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+  const uint32_t pools[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
   const uint32_t blocks[] = {3, 4, 6, 8};
   const uint32_t iters[] = {1, 4};
   const uint32_t pipes[] = {2, 3, 4};

@@ -252,11 +252,13 @@ struct pt_renderer {
   // 3 x 3 / 4 / 5 / 6 measured 894 / 881 / 867 / 843 Msamples/s, 2 x 6 / 8
   // 870 / 838. Three pipelines of 3 blocks are the 9 traversal blocks (16.5 KB
   // of LDS stack each) that fit a CU's 160 KB at once. Batches of many small
-  // scenes prefer 4 (1024 generated scenes: 5,205 vs 4,880 scenes/s). 0 =
-  // those defaults; pt_set_launch_params or the autotuner set a value.
+  // scenes prefer more (1024 generated scenes, batch render ms with a pool
+  // of 8 M paths: 4 / 6 / 8 blocks 139.6 / 136.1 / 138.0; with 16 M, 6 / 8:
+  // 133.7 / 134.3). 0 = those defaults; pt_set_launch_params or the
+  // autotuner set a value.
   uint32_t trace_blocks_per_cu = 0;
   uint32_t trace_blocks_for(bool batched) const {
-    return trace_blocks_per_cu ? trace_blocks_per_cu : (batched ? 4u : 3u);
+    return trace_blocks_per_cu ? trace_blocks_per_cu : (batched ? 6u : 3u);
   }
   // mega-kernel: several lanes per work-division thread (same image bits)
   bool mega_split = true;
