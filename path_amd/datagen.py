@@ -36,22 +36,40 @@ def generate_scene(seed: int) -> GeneratedScene:
     return GeneratedScene(Scene(h), ftw, int(res.value), seed)
 
 
+# pt_batch_item as a numpy record (same 80-byte layout): a batch's items are
+# filled column by column (filling 1024 ctypes structures field by field took
+# ~7 ms of Python per batch).
+_ITEM = np.dtype([("scene", "<u8"), ("x_dim", "<u4"), ("y_dim", "<u4"),
+                  ("film_to_world", "<f4", (12,)), ("out_rgb", "<u8"), ("out_gbuffer", "<u8")])
+assert _ITEM.itemsize == C.sizeof(L.PtBatchItem)
+assert [_ITEM.fields[f][1] for f in _ITEM.names] == [
+    getattr(L.PtBatchItem, f).offset for f, _ in L.PtBatchItem._fields_]
+
+
+def _item_records(scenes, out_rgb=None, out_gbuffer=None):
+    """Records of `scenes` (scene, size, camera) with the given output
+    addresses (sequences of ints, or None for null), and the pointer to pass."""
+    a = np.zeros(len(scenes), _ITEM)
+    a["scene"] = [g.scene.handle.value or 0 for g in scenes]
+    res = np.fromiter((g.resolution for g in scenes), np.uint32, len(scenes))
+    a["x_dim"] = res
+    a["y_dim"] = res
+    if len(scenes):
+        a["film_to_world"] = np.stack([np.asarray(g.film_to_world, np.float32) for g in scenes])
+    if out_rgb is not None:
+        a["out_rgb"] = out_rgb
+    if out_gbuffer is not None:
+        a["out_gbuffer"] = out_gbuffer
+    return a, a.ctypes.data_as(C.POINTER(L.PtBatchItem))
+
+
 def _items(scenes, gbuffer: bool):
-    items = (L.PtBatchItem * len(scenes))()
-    images, gbufs = [], []
-    for i, g in enumerate(scenes):
-        r = g.resolution
-        img = np.zeros((r, r, 3), np.float32)
-        gb = np.zeros((r, r, 12), np.float32) if gbuffer else None
-        items[i].scene = g.scene.handle
-        items[i].x_dim = r
-        items[i].y_dim = r
-        items[i].film_to_world[:] = [float(v) for v in g.film_to_world]
-        items[i].out_rgb = L.fptr(img)
-        items[i].out_gbuffer = L.fptr(gb)
-        images.append(img)
-        gbufs.append(gb)
-    return items, images, gbufs
+    images = [np.zeros((g.resolution, g.resolution, 3), np.float32) for g in scenes]
+    gbufs = [np.zeros((g.resolution, g.resolution, 12), np.float32) if gbuffer else None
+             for g in scenes]
+    rec, ptr = _item_records(scenes, [im.ctypes.data for im in images],
+                             [gb.ctypes.data for gb in gbufs] if gbuffer else None)
+    return (rec, ptr), images, gbufs
 
 
 def render_batch(renderer: Renderer, scenes, samples_per: int,
@@ -59,7 +77,7 @@ def render_batch(renderer: Renderer, scenes, samples_per: int,
     """Render every scene of `scenes` (GeneratedScene list) in one launch
     sequence. Returns (images, gbuffers, seconds)."""
     settings = settings or datagen_settings()
-    items, images, gbufs = _items(scenes, gbuffer)
+    (rec, items), images, gbufs = _items(scenes, gbuffer)
     secs = C.c_double()
     L.check(L.lib().pt_render_batch(renderer.handle, items, len(scenes), samples_per,
                                     C.byref(settings.raw), C.byref(secs)))
@@ -85,17 +103,8 @@ def render_batch_device(renderer: Renderer, scenes, samples_per: int,
     if rgb is None:
         rgb = DeviceBuffer(renderer, max(total, 1) * 12)
         gb = DeviceBuffer(renderer, max(total, 1) * 48) if gbuffer else None
-    items = (L.PtBatchItem * len(scenes))()
-    fp = C.POINTER(C.c_float)
-    for i, g in enumerate(scenes):
-        r = g.resolution
-        items[i].scene = g.scene.handle
-        items[i].x_dim = r
-        items[i].y_dim = r
-        items[i].film_to_world[:] = [float(v) for v in g.film_to_world]
-        items[i].out_rgb = C.cast(C.c_void_p(rgb.ptr + 12 * int(offsets[i])), fp)
-        items[i].out_gbuffer = (C.cast(C.c_void_p(gb.ptr + 48 * int(offsets[i])), fp)
-                                if gbuffer else None)
+    rec, items = _item_records(scenes, rgb.ptr + 12 * offsets[:-1].astype(np.uint64),
+                               gb.ptr + 48 * offsets[:-1].astype(np.uint64) if gbuffer else None)
     secs = C.c_double()
     L.check(L.lib().pt_render_batch(renderer.handle, items, len(scenes), samples_per,
                                     C.byref(settings.raw), C.byref(secs)))
@@ -106,7 +115,7 @@ def autotune(renderer: Renderer, scenes, samples_per: int, settings: Settings) -
     """Pick the fastest (path pool, traversal blocks per CU, iterations per
     host sync) for this workload; the renderer and `settings` keep it."""
     # the output arrays must outlive the call: the renders write into them
-    items, images, _ = _items(scenes, False)
+    (rec, items), images, _ = _items(scenes, False)
     out = (C.c_uint32 * 5)()
     L.check(L.lib().pt_autotune_batch(renderer.handle, items, len(scenes), samples_per,
                                       C.byref(settings.raw), out))
