@@ -173,7 +173,11 @@ def bench_datagen(a, cfg):
     _progress(f"datagen: first batch {first_s:.1f} s")
     tuned = None
     if not a.no_autotune:
-        tuned = autotune(r, gens[: min(n, 128)], spp, settings)
+        # tuned on a batch of the same size from other seeds, then applied to
+        # the measured batches
+        tune_set = [generate_scene(seed) for seed in range(n, 2 * n)]
+        tuned = autotune(r, tune_set, spp, settings)
+        del tune_set
         _progress(f"datagen: autotuned {tuned}")
     elif a.trace_blocks_per_cu or a.iters_per_sync:
         set_launch_params(r, a.trace_blocks_per_cu, a.iters_per_sync or 1, bool(a.iters_per_sync))
@@ -325,6 +329,7 @@ def bench_datagen(a, cfg):
         "host_output_resident_scenes_per_s": round(host_value, 2),
         "new_scenes_host_sah_trees_scenes_per_s": round(n / min(host_tree_s), 2),
         "autotuned": tuned,
+        "autotune_set": (f"{n} generated scenes of seeds {n}..{2 * n - 1}" if tuned else None),
         "host_seconds": {"generate_per_batch": round(min(gen_step_s), 3),
                          "first_batch_incl_host_bvh_builds": round(first_s, 3),
                          "generate_first": round(gen_s, 3)},
