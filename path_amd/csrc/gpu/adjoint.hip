@@ -200,32 +200,74 @@ hipError_t launch_adjoint(const AdjointParams &p, hipStream_t s) {
   return hipGetLastError();
 }
 
-// One block per parameter: each thread sums a fixed stride of blocks in FP64,
-// then a fixed-shape tree over the threads (deterministic).
-__global__ __launch_bounds__(256) void k_adjoint_finalize(const float *__restrict__ partials,
-                                                          uint32_t n_blocks, uint32_t n_params,
+// Sum of the per-wave gradient rows (n_blocks rows of n_params floats) in
+// two passes, both in a fixed order (deterministic). Pass 1: workgroup g sums
+// the rows of its contiguous range; its threads cover floor(256 / n_params)
+// whole rows per step, so each step loads consecutive floats, and thread t
+// keeps parameter t % n_params of row offset t / n_params in FP64; the row
+// offsets' sums are then added in order. Pass 2: one workgroup per parameter
+// adds the ranges' sums (at most 256, one per thread) with a fixed-shape tree
+// and scales. (Pass 2 as one thread per parameter looping over the ranges
+// took 59 us: 256 dependent loads per thread.)
+__global__ __launch_bounds__(256) void k_adjoint_rows(const float *__restrict__ partials,
+                                                      uint32_t n_blocks, uint32_t n_params,
+                                                      uint32_t rows_per_group,
+                                                      double *__restrict__ group_sums) {
+  __shared__ double sums[256];
+  const uint32_t per_step = 256u / n_params;
+  const uint32_t t = threadIdx.x;
+  const uint32_t first = blockIdx.x * rows_per_group;
+  const uint32_t last = min(first + rows_per_group, n_blocks);
+  double sum = 0.0;
+  if (t < per_step * n_params) {
+    const uint32_t param = t % n_params;
+    for (uint32_t r = first + t / n_params; r < last; r += per_step)
+      sum += double(partials[size_t(r) * n_params + param]);
+  }
+  sums[t] = sum;
+  __syncthreads();
+  if (t < n_params) {
+    double total = 0.0;
+    for (uint32_t k = 0; k < per_step; ++k)
+      total += sums[k * n_params + t];
+    group_sums[size_t(blockIdx.x) * n_params + t] = total;
+  }
+}
+
+static_assert(kAdjointFinalizeGroups <= 256, "pass 2 takes one range per thread");
+
+__global__ __launch_bounds__(256) void k_adjoint_finalize(const double *__restrict__ group_sums,
+                                                          uint32_t n_groups, uint32_t n_params,
                                                           double scale,
                                                           float *__restrict__ out) {
   __shared__ double sums[256];
-  const uint32_t i = blockIdx.x;
-  double sum = 0.0;
-  for (uint32_t b = threadIdx.x; b < n_blocks; b += 256u)
-    sum += double(partials[size_t(b) * n_params + i]);
-  sums[threadIdx.x] = sum;
+  const uint32_t i = blockIdx.x, g = threadIdx.x;
+  sums[g] = g < n_groups ? group_sums[size_t(g) * n_params + i] : 0.0;
   __syncthreads();
   for (uint32_t off = 128u; off >= 1u; off >>= 1) {
-    if (threadIdx.x < off)
-      sums[threadIdx.x] += sums[threadIdx.x + off];
+    if (g < off)
+      sums[g] += sums[g + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
+  if (g == 0)
     out[i] = float(sums[0] * scale);
 }
 
+uint32_t adjoint_finalize_groups(uint32_t n_blocks) {
+  const uint32_t rows = (n_blocks + kAdjointFinalizeGroups - 1) / kAdjointFinalizeGroups;
+  return rows ? (n_blocks + rows - 1) / rows : 0;
+}
+
 hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
-                                   uint32_t n_params, double scale, float *out,
-                                   hipStream_t s) {
-  hipLaunchKernelGGL(k_adjoint_finalize, dim3(n_params), dim3(256), 0, s, partials, n_blocks,
+                                   uint32_t n_params, double scale, double *group_sums,
+                                   float *out, hipStream_t s) {
+  if (n_params == 0 || n_params > 256 || n_blocks == 0)
+    return hipErrorInvalidValue;
+  const uint32_t rows = (n_blocks + kAdjointFinalizeGroups - 1) / kAdjointFinalizeGroups;
+  const uint32_t groups = adjoint_finalize_groups(n_blocks);
+  hipLaunchKernelGGL(k_adjoint_rows, dim3(groups), dim3(256), 0, s, partials, n_blocks,
+                     n_params, rows, group_sums);
+  hipLaunchKernelGGL(k_adjoint_finalize, dim3(n_params), dim3(256), 0, s, group_sums, groups,
                      n_params, scale, out);
   return hipGetLastError();
 }

@@ -151,11 +151,15 @@ hipError_t launch_nrc_first_vertex(const NrcParams &p, hipStream_t s);
 hipError_t launch_nrc_combine(const float *base, const float *weight, const float *pred,
                               uint32_t n_pixels, uint32_t spp, float *out_rgb,
                               hipStream_t s);
-// out[i] = scale * sum over blocks of partials[block][i] (FP64 accumulate, in
-// block order).
+// out[i] = scale * sum over blocks of partials[block][i], accumulated in FP64
+// in a fixed order: first over contiguous ranges of rows (at most
+// kAdjointFinalizeGroups ranges), then over the ranges. group_sums holds
+// adjoint_finalize_groups(n_blocks) * n_params doubles; n_params <= 256.
+constexpr uint32_t kAdjointFinalizeGroups = 256;
+uint32_t adjoint_finalize_groups(uint32_t n_blocks);
 hipError_t launch_adjoint_finalize(const float *partials, uint32_t n_blocks,
-                                   uint32_t n_params, double scale, float *out,
-                                   hipStream_t s);
+                                   uint32_t n_params, double scale, double *group_sums,
+                                   float *out, hipStream_t s);
 // d[i] = 2 (image[i] - target[i]) (device arrays of n floats).
 hipError_t launch_l2_gradient(const float *image, const float *target, uint64_t n, float *d,
                               hipStream_t s);

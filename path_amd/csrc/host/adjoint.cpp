@@ -73,10 +73,13 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   p.x_dim = x_dim;
   p.y_dim = y_dim;
   const uint64_t n_pixels = uint64_t(x_dim) * y_dim;
-  // about 1024 threads per CU (4 waves per SIMD, the kernel's VGPR limit; 2
-  // waves measured 7% slower), whole pixels each
+  // about 8192 threads per CU, whole pixels each: 8 rounds of the 4 waves per
+  // SIMD that the kernel's VGPRs allow, so a SIMD whose waves end early (short
+  // paths) starts new ones instead of idling until the launch's slowest wave
+  // ends (256 x 256 at 32 spp, adjoint ms: 1024 threads per CU, one round,
+  // 1.14; 2048: 1.00; 4096: 0.94; 8192: 0.92)
   const uint64_t total = n_pixels * spp;
-  uint64_t spt = total / (uint64_t(r.n_cus) * 1024);
+  uint64_t spt = total / (uint64_t(r.n_cus) * 8192);
   spt = spt < 1 ? 1 : (spt > spp ? spp : spt);
   p.samples_per_thread = uint32_t(spt);
   p.threads_per_pixel = uint32_t((spp + spt - 1) / spt);
@@ -102,6 +105,7 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
   }
   HIP_TRY(r.adj_partials.ensure(size_t(n_blocks) * n_params));
   HIP_TRY(r.adj_out.ensure(n_params));
+  HIP_TRY(r.adj_group_sums.ensure(size_t(gpu::adjoint_finalize_groups(n_blocks)) * n_params));
   p.d_image = d_on_device ? d_image : r.adj_d_image.ptr;
   p.partials = r.adj_partials.ptr;
   // one replay per path when the path length is bounded (NIters: at most
@@ -120,7 +124,8 @@ int render_adjoint(pt_renderer &r, const pt_scene &scene, const float film_to_wo
     HIP_TRY(hipEventRecord(r.ev_start, s));
     HIP_TRY(gpu::launch_adjoint(p, s));
     HIP_TRY(gpu::launch_adjoint_finalize(r.adj_partials.ptr, n_blocks, n_params,
-                                         1.0 / double(spp), r.adj_out.ptr, s));
+                                         1.0 / double(spp), r.adj_group_sums.ptr,
+                                         r.adj_out.ptr, s));
     HIP_TRY(hipEventRecord(r.ev_stop, s));
     HIP_TRY(hipMemcpyAsync(out.data(), r.adj_out.ptr, n_params * sizeof(float),
                            hipMemcpyDeviceToHost, s));

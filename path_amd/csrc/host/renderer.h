@@ -237,6 +237,7 @@ struct pt_renderer {
 
   // adjoint buffers
   pt::host::DeviceBuf<float> adj_d_image, adj_partials, adj_out;
+  pt::host::DeviceBuf<double> adj_group_sums;
 
   // last Streaming SquareImage render (for the FP64 re-summation)
   uint32_t last_wf_x = 0, last_wf_y = 0, last_wf_spp = 0;
