@@ -72,6 +72,32 @@ def test_adjoint_matches_fp64_oracle(renderer, dbox, n_iters):
     dr.set_parameters(*orig)
 
 
+# Work divisions the 256^2 x 32 spp case does not reach: 131 x 97 at 5 spp is
+# 993 wave rows of gradients (ranges of 4 rows, the last of 1; 48 parameters
+# put 5 rows in a step, so some threads sum none); 256^2 at 67 spp runs 2
+# samples per thread, a pixel's last thread 1.
+@pytest.mark.parametrize("x,y,spp", [(131, 97, 5), (256, 256, 67)])
+def test_adjoint_ragged_work_division(renderer, dbox, x, y, spp):
+    from oracle import oracle as O
+    from path_amd.diff import DiffRenderer
+    scene, ftw = dbox
+    st = _settings(4)
+    dr = DiffRenderer(renderer, scene, ftw, x, y, spp, st)
+    target, alb, em, orig = _target_and_start(dr)
+    dr.set_parameters(alb, em)
+    img = dr.render().astype(np.float64)
+    d_img = (2.0 * (img - target)).astype(np.float32)
+    ga, ge = dr.adjoint(d_img)
+    tab = O.sobol_table(O.joe_kuo_directions(st.max_sample_size), st.max_sample_size, spp)
+    ra, re, _, _ = O.adjoint(scene.flat(), ftw, st, spp, x, y, tab, d_img.astype(np.float64))
+    g = np.concatenate([ga.ravel(), ge[7]])
+    ref = np.concatenate([ra.ravel(), re[7]])
+    err = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    print(f"{x}x{y} {spp} spp: max rel err vs FP64 adjoint", err.max())
+    assert err.max() <= 1e-3
+    dr.set_parameters(*orig)
+
+
 def _central_differences(alb, em, loss, h=1e-3):
     """dloss over the 8 albedo colors and the emitter radiance (material 7),
     with the step actually representable in float32."""
