@@ -353,8 +353,9 @@ __device__ __forceinline__ void wf_shade(const WfParams &p) {
 // All shade kernels require blockDim.x == 256 (block-level aggregation).
 // Batches run it at 8 waves per SIMD (62 VGPRs and 80 B of spills instead of
 // 74 VGPRs at 6 waves; 1024 generated scenes: render 132.0 -> 128.3 ms, and
-// 7 waves measured 133.5); a single render keeps the compiler's 7 (68 VGPRs;
-// 8 spilled more and measured slower).
+// 7 waves measured 133.5). A single render asks for at least 6, which the
+// compiler's own allocation (68 VGPRs, 7 waves, no spills) already meets;
+// asking 8 there spilled more and measured slower.
 template <class R, bool kBatch>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBatch ? 8 : 6))) void
 k_wf_shade(WfParams p) {
